@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 batched Ed25519 ElGamal library on one MI355X.
+
+Workload (one "step"): the scalar-multiplication configuration with
+N = 2^22 of each kind, every input resident in HBM before timing starts:
+  fixed-base      k_i * B                       (N scalar-mults)
+  variable-base   k_i * P_i, P_i = s_i * B      (N scalar-mults)
+  ct x constant   (a_i C1_i, a_i C2_i), a_i u32 (2N scalar-mults)
+`value` = scalar-mults per second over the whole step.  Point-add rates of
+the aggregation kernels (elementwise ciphertext add, reduce, scan, CSR
+segmented sum) are reported under "detail".
+
+Timing uses HIP events recorded on the stream the kernels run on.  The
+roofline object is for the kernel with the largest share of the step.  The
+CPU baseline is the oracle (a C port of the reference CPU path's algorithm)
+on a bounded sample, all host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n LOG2N]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X: 256 CUs, 2.4 GHz peak engine clock, 64 lanes issued per CU per
+# clock for full-rate VALU ops; v_mad_u64_u32 (one 32x32->64 product) issues
+# at that full rate on gfx950 (measured, see profiles/).
+PEAK_MUL32 = 256 * 64 * 2.4e9
+HBM_PEAK = 8.0e12
+
+# algorithmic cost model: 32x32->64 products per field mul / square (8x32-bit
+# schoolbook + fold), group operations counted from the reference formulas.
+M, S = 72, 44
+PRODUCTS = {
+    "fixed_base": 461 * M + 16 * S,
+    "variable_base": 1316 * M + 1024 * S,
+    "mul_u32": 216 * M + 144 * S,
+    "point_add": 9 * M,
+}
+POINT_BYTES = 128
+
+
+class Hip:
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        L = self.L
+        L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        L.hipFree.argtypes = [ctypes.c_void_p]
+        L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        L.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        if self.L.hipMalloc(ctypes.byref(p), nbytes) != 0:
+            raise MemoryError("hipMalloc(%d)" % nbytes)
+        return p
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.malloc(arr.nbytes)
+        if self.L.hipMemcpy(p, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy")
+        return p
+
+    def event(self):
+        e = ctypes.c_void_p()
+        self.L.hipEventCreate(ctypes.byref(e))
+        return e
+
+    def elapsed_ms(self, a, b):
+        self.L.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        self.L.hipEventElapsedTime(ctypes.byref(ms), a, b)
+        return ms.value
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run(args):
+    import libftcrypto_amd as ft
+    hip = Hip()
+    ft.init()
+    L = ft.lib()
+    stream = L.ftx_default_stream()
+    n = 1 << args.n
+    seed = 0xF7C00001
+
+    log("[bench] generating %d x 4 inputs" % n)
+    t0 = time.time()
+    k_fixed = ft.random_scalars(n, seed, 1)
+    k_var = ft.random_scalars(n, seed, 2)
+    s_var = ft.random_scalars(n, seed, 3)
+    r_ct = ft.random_scalars(n, seed, 4)
+    rng = np.random.default_rng(seed)
+    a_u32 = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    m_ct = rng.integers(0, 2, size=n, dtype=np.uint64)
+    log("[bench] inputs generated in %.1fs" % (time.time() - t0))
+
+    d_kf, d_kv, d_a = hip.upload(k_fixed), hip.upload(k_var), hip.upload(a_u32)
+    out_fixed = ft.GeArray.alloc(n)
+    pts = ft.GeArray.from_scalars(s_var)                    # P_i = s_i B (setup)
+    x = ft.random_scalars(1, seed, 5)
+    key = ft.Key(ft.GeArray.from_scalars(x))
+    c1, c2 = ft.encrypt(key, m_ct, r_ct, msg_bits=1)          # setup ciphertexts
+    # aggregation inputs: an independent ciphertext array and CSR segments
+    d1, d2 = ft.encrypt(key, m_ct[::-1].copy(), ft.random_scalars(n, seed, 6), msg_bits=1)
+    lens = np.clip(np.round(rng.lognormal(np.log(16), 1.5, size=max(1, n // 32))), 1, 65536).astype(np.int64)
+    lens = lens[: np.searchsorted(np.cumsum(lens), n)]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    d_off = hip.upload(off)
+    nseg, nel = len(lens), int(off[-1])
+    seg_out = ft.GeArray.alloc(nseg)
+    seg_scr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(nel))
+    red_out = ft.GeArray.alloc(1)
+    ft.check(L.ftx_synchronize(None))
+
+    ops = {
+        "fixed_base": lambda: ft.check(L.ftx_from_scalars_dev(out_fixed.h, d_kf, stream)),
+        "variable_base": lambda: ft.check(L.ftx_mul_dev(pts.h, d_kv, stream)),
+        "mul_u32": lambda: ft.check(L.ftx_mul_u32_dev(c1.h, c2.h, d_a, stream)),
+    }
+    mults = {"fixed_base": n, "variable_base": n, "mul_u32": 2 * n}
+    agg = {
+        "ct_add": (lambda: (ft.check(L.ftx_add_dev(c1.h, d1.h, stream)),
+                            ft.check(L.ftx_add_dev(c2.h, d2.h, stream))), 2 * n, 3 * POINT_BYTES),
+        "reduce": (lambda: ft.check(L.ftx_reduce_dev(red_out.h, d1.h, stream)), n - 1, POINT_BYTES),
+        "segsum": (lambda: ft.check(L.ftx_segmented_sum_dev(seg_out.h, d_off, nseg, nel, None, d1.h,
+                                                            seg_scr.h, stream)), nel - nseg, None),
+    }
+
+    ev = [hip.event() for _ in range(2)]
+
+    def timed(fn, reps):
+        hip.L.hipEventRecord(ev[0], stream)
+        for _ in range(reps):
+            fn()
+        hip.L.hipEventRecord(ev[1], stream)
+        return hip.elapsed_ms(ev[0], ev[1]) / reps
+
+    log("[bench] warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        for f in ops.values():
+            f()
+    ft.check(L.ftx_synchronize(None))
+
+    log("[bench] timing %d steps" % args.steps)
+    per = {k: 0.0 for k in ops}
+    hip.L.hipEventRecord(ev[0], stream)
+    t_wall = time.time()
+    for _ in range(args.steps):
+        for k, f in ops.items():
+            f()
+    hip.L.hipEventRecord(ev[1], stream)
+    step_ms = hip.elapsed_ms(ev[0], ev[1]) / args.steps
+    wall_ms = (time.time() - t_wall) * 1e3 / args.steps
+    for k, f in ops.items():
+        per[k] = timed(f, max(1, args.steps))
+        log("[bench] %-14s %.3f ms" % (k, per[k]))
+    total_mults = sum(mults.values())
+    value = total_mults / (step_ms * 1e-3)
+
+    detail = {}
+    for k in ops:
+        rate = mults[k] / (per[k] * 1e-3)
+        ach = rate * PRODUCTS[k]
+        detail[k] = {"ms": round(per[k], 4), "scalar_mults_per_s": rate,
+                     "mul32_per_s": ach, "valu_frac": ach / PEAK_MUL32}
+    for k, (f, units, bpu) in agg.items():
+        f()
+        ft.check(L.ftx_synchronize(None))
+        ms = timed(f, 3)
+        rate = units / (ms * 1e-3)
+        d = {"ms": round(ms, 4), "point_adds_per_s": rate,
+             "valu_frac": rate * PRODUCTS["point_add"] / PEAK_MUL32, "n": units}
+        if bpu:
+            d["hbm_GBps"] = rate * bpu / 1e9
+            d["hbm_frac"] = rate * bpu / HBM_PEAK
+        detail[k] = d
+        log("[bench] %-14s %.3f ms  %.3e point-adds/s" % (k, ms, rate))
+
+    dom = max(per, key=lambda k: per[k])
+    ach = mults[dom] / (per[dom] * 1e-3) * PRODUCTS[dom]
+    roofline = {"kernel": dom, "bound": "valu", "achieved": ach / 1e12, "peak": PEAK_MUL32 / 1e12,
+                "unit": "Tmul32/s", "frac": ach / PEAK_MUL32, "traffic": None}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n, k_fixed, k_var, s_var, a_u32)
+    return {
+        "metric": "scalar-mults/s (fixed-base, variable-base, ciphertext x u32) on 1 MI355X",
+        "value": value, "unit": "scalar-mults/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": step_ms, "wall_ms_per_step": wall_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32 limbs (exact GF(2^255-19))", "data": "synthetic (seeded SplitMix64 scalars mod l)",
+        "config": {"workload": "scalar-mult arrays N=2^%d each: k*B, k*P, (a*C1, a*C2) with a u32" % args.n,
+                   "n": n, "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "detail": detail,
+    }
+
+
+def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
+    """The oracle on a bounded sample of the same step, all host cores."""
+    from oracle import oracle as O
+    nt = O.threads()
+    samp = max(256, min(n, 1 << 14))
+    t = {}
+    t0 = time.time()
+    O.Array.from_scalars(k_fixed[:samp], nt=nt)
+    t["fixed_base"] = time.time() - t0
+    P = O.Array.from_scalars(s_var[:samp], nt=nt)
+    t0 = time.time()
+    P.mul(k_var[:samp], nt=nt)
+    t["variable_base"] = time.time() - t0
+    t0 = time.time()
+    P.mul_u32(a_u32[:samp], nt=nt)
+    t["mul_u32"] = (time.time() - t0) * 2          # two points per ciphertext
+    total = sum(t.values())
+    rate = 4 * samp / total
+    return {"value": rate, "unit": "scalar-mults/s", "cores": nt, "kind": "port",
+            "sample": "%d of each of the N=2^k fixed-base, variable-base and u32 mults (ct = 2 point mults)" % samp,
+            "seconds": round(total, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=22, help="log2 of the array length")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this library's hot path runs on a single device "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+    res = run(args)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

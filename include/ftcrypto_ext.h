@@ -1,0 +1,100 @@
+/*
+ * ftcrypto_ext.h -- extensions of the MI355X library beyond the reference's
+ * 38 functions.  The reference has no ciphertext type and no encryption
+ * routine: its caller composes Enc(m; r) = (r B, m B + r H) from
+ * ft_array_from_scalars, ft_array_init_point, ft_mul and ft_add
+ * (ftcrypto.h:86-97, 275-277, 335-337, 383-385).  The ftx_ functions below
+ * fuse those compositions, add the CSR segmented sum and sparse gather-sum
+ * of the CPU path (ftcrypto.c:435-474), and give device-resident, stream-
+ * ordered variants for callers that keep their data in HBM.
+ *
+ * Streams are passed as `void *` (a hipStream_t; NULL = the library stream).
+ * Functions with a _dev suffix take device pointers and return as soon as
+ * the work is enqueued; all others consume host buffers before returning.
+ */
+#ifndef FT_CRYPTO_EXT_H
+#define FT_CRYPTO_EXT_H
+
+#include "ftcrypto.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque comb table of a public key H: 32 x 8 affine multiples (28 KiB). */
+struct ftx_key_s;
+typedef struct ftx_key_s *ftx_key;
+
+/* ---------------------------------------------------------- arrays */
+
+/* New array of n elements whose contents are unspecified until written. */
+ft_error ftx_array_alloc(ft_ge25519_array *array, size_t n);
+/* Device address of coordinate plane 0 and the plane stride (elements). */
+ft_error ftx_array_device_view(const_ft_ge25519_array array, void **planes, size_t *capacity);
+/* Library stream (hipStream_t) used by the reference-compatible functions. */
+void *ftx_default_stream(void);
+ft_error ftx_synchronize(void *stream);
+
+/* ---------------------------------------------------------- ElGamal */
+
+/* Comb table of H = pk[pk_idx]; reused by every encryption under that key. */
+ft_error ftx_key_create(ftx_key *key, const_ft_ge25519_array pk, size_t pk_idx);
+void ftx_key_free(ftx_key key);
+
+/* c1[i] = r_i B, c2[i] = m_i B + r_i H for host m (uint64) and r (32-byte
+ * scalars).  msg_bits bounds the bit length of every m_i (0: derive it from
+ * the largest m_i); it fixes the number of message windows. */
+ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key,
+                     const uint64_t *m, const uint8_t *r, size_t n, int msg_bits);
+/* c1[i] += r_i B, c2[i] += r_i H */
+ft_error ftx_rerandomize(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
+                         const uint8_t *r);
+/* c1[i] = k_i c1[i] and c2[i] = k_i c2[i] (c2 may be NULL) for uint32 k */
+ft_error ftx_mul_u32(ft_ge25519_array c1, ft_ge25519_array c2, const uint32_t *k, size_t n);
+
+/* ---------------------------------------------------------- aggregation */
+
+/* res[s] = sum of src[offsets[s] .. offsets[s+1]-1]; empty segment -> zero.
+ * offsets: n_seg + 1 non-decreasing int64 values, offsets[n_seg] <= len(src). */
+ft_error ftx_segmented_sum(ft_ge25519_array *res, const int64_t *offsets, size_t n_seg,
+                           const_ft_ge25519_array src);
+/* res[v] = sum of src[cols[j]] for j in [row_off[v], row_off[v+1]) -- the CSR
+ * form of the CPU path's ft_sparse_matmul (ftcrypto.c:435-474). */
+ft_error ftx_gather_sum(ft_ge25519_array *res, const int64_t *row_off, const int32_t *cols,
+                        size_t n_rows, const_ft_ge25519_array src);
+
+/* ---------------------------------------------------------- device-resident */
+
+ft_error ftx_from_scalars_dev(ft_ge25519_array out, const void *d_scalars, void *stream);
+ft_error ftx_mul_dev(ft_ge25519_array array, const void *d_scalars, void *stream);
+ft_error ftx_mul_u32_dev(ft_ge25519_array c1, ft_ge25519_array c2, const void *d_k, void *stream);
+ft_error ftx_add_dev(ft_ge25519_array res, const_ft_ge25519_array summand, void *stream);
+ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
+                         const void *d_m, const void *d_r, int msg_bits, void *stream);
+ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
+                             const void *d_r, void *stream);
+/* out must have n_seg elements; d_gather may be NULL (plain CSR); d_scratch
+ * must hold ftx_segsum_scratch_points(n_elems) points (NULL: library-owned). */
+size_t ftx_segsum_scratch_points(size_t n_elems);
+ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg,
+                               size_t n_elems, const void *d_gather,
+                               const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream);
+/* out[0] = sum of src (out needs 1 element) */
+ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, void *stream);
+ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, void *stream);
+/* 32-byte canonical compressed encodings into device memory d_out */
+ft_error ftx_encode_dev(void *d_out, const_ft_ge25519_array array, void *stream);
+
+/* ---------------------------------------------------------- inputs */
+
+/* n scalars uniform mod l: 64 bytes of SplitMix64(seed, stream) output per
+ * scalar, reduced modulo l = 2^252 + 27742317777372353535851937790883648493. */
+void ftx_random_scalars(uint8_t *out, size_t n, uint64_t seed, uint64_t stream);
+/* 64-byte little-endian integers -> 32-byte residues mod l */
+void ftx_scalar_reduce64(uint8_t *out, const uint8_t *in, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,181 @@
+// Scan and CSR segmented-sum kernels for gfx950 (combine = point addition).
+//
+// Both split the input into fixed chunks of 16 consecutive elements, one
+// chunk per lane, so the work per lane is the same whatever the segment
+// lengths are (the reference runs one thread per segment, ftcrypto.cu:1116-1127,
+// and a recursive Blelloch scan, :1474-1562).
+//
+// Scan: level sums of every chunk are scanned recursively (exclusive), then
+// every lane walks its chunk once more starting from its chunk's prefix.
+//
+// Segmented sum: level 1 walks the elements; a segment that lies inside one
+// chunk is written to the output directly, one that crosses a chunk edge
+// leaves a partial sum in the chunk's head slot (segment began before the
+// chunk) or tail slot (segment continues after it).  The slots form the
+// element array of level 2, where segment s covers slots
+// [2 c_first + 1, 2 c_last + 1) -- a range computed from the offsets alone --
+// and the same kernel runs again, until one chunk holds everything.
+#include "kernels.h"
+
+namespace ft {
+
+constexpr int ATPB = 256;
+constexpr int CH = SEG_CHUNK_LOG2;
+constexpr int CHN = 1 << CH;
+
+// ------------------------------------------------------------------ scan
+
+size_t scan_tiles(size_t n) { return (n + CHN - 1) >> CH; }
+
+// sums[t] = sum of elements [16 t, 16 t + 16)
+__global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size_t n) {
+  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const size_t lo = t << CH;
+  if (lo >= n) return;
+  const size_t hi = (lo + CHN < n) ? lo + CHN : n;
+  ge_p3 acc, x;
+  load_p3(acc, v, lo);
+#pragma unroll 1
+  for (size_t i = lo + 1; i < hi; ++i) {
+    load_p3(x, v, i);
+    ge_p3_add(acc, acc, x);
+  }
+  store_p3(sums, t, acc);
+}
+
+// walk chunk t from its exclusive prefix (identity when pre.w is null)
+__global__ void __launch_bounds__(ATPB) k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
+  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const size_t lo = t << CH;
+  if (lo >= n) return;
+  const size_t hi = (lo + CHN < n) ? lo + CHN : n;
+  ge_p3 run, x;
+  if (pre.w) load_p3(run, pre, t);
+  else ge_p3_0(run);
+#pragma unroll 1
+  for (size_t i = lo; i < hi; ++i) {
+    load_p3(x, v, i);
+    if (exclusive) {
+      store_p3(v, i, run);
+      ge_p3_add(run, run, x);
+    } else {
+      ge_p3_add(run, run, x);
+      store_p3(v, i, run);
+    }
+  }
+}
+
+hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s) {
+  const size_t T = scan_tiles(n);
+  if (!T) return hipSuccess;
+  hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)((T + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, sums, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_apply(PtView v, size_t n, PtView pre, bool exclusive, hipStream_t s) {
+  const size_t T = scan_tiles(n);
+  if (!T) return hipSuccess;
+  hipLaunchKernelGGL(k_chunk_scan, dim3((unsigned)((T + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, v, n, pre,
+                     (int)exclusive);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------- segmented sum
+
+// range of segment s in the element space of `level` (1 = the input)
+__device__ __forceinline__ void seg_range(const int64_t *off, size_t s, int level, int64_t &lo,
+                                          int64_t &hi) {
+  lo = off[s];
+  hi = off[s + 1];
+#pragma unroll 1
+  for (int l = 1; l < level; ++l) {
+    const int64_t c0 = lo >> CH;
+    if (hi > lo && c0 != ((hi - 1) >> CH)) {
+      lo = 2 * c0 + 1;
+      hi = 2 * ((hi - 1) >> CH) + 1;
+    } else {
+      lo = hi = 2 * c0 + 1;
+    }
+  }
+}
+
+// first segment whose range ends after position p (ranges are sorted)
+__device__ __forceinline__ size_t seg_find(const int64_t *off, size_t nseg, int level, int64_t p) {
+  size_t a = 0, b = nseg;
+  while (a < b) {
+    const size_t mid = (a + b) >> 1;
+    int64_t lo, hi;
+    seg_range(off, mid, level, lo, hi);
+    if (hi <= p) a = mid + 1;
+    else b = mid;
+  }
+  return a;
+}
+
+__device__ __forceinline__ void seg_flush(PtView out, PtView rec, size_t s, int64_t lo, int64_t hi,
+                                          int64_t clo, int64_t chi, size_t t, const ge_p3 &acc) {
+  if (lo >= clo && hi <= chi) store_p3(out, s, acc);      // complete in this chunk
+  else if (lo < clo) store_p3(rec, 2 * t, acc);           // began before the chunk
+  else store_p3(rec, 2 * t + 1, acc);                     // continues after it
+}
+
+__global__ void __launch_bounds__(ATPB) k_seg_level(PtView out, PtView rec, PtView elems, size_t n,
+                                                    const int64_t *off, size_t nseg,
+                                                    const int32_t *gather, int level) {
+  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int64_t clo = (int64_t)(t << CH);
+  if (clo >= (int64_t)n) return;
+  const int64_t chi = clo + CHN;
+  const int64_t cend = chi < (int64_t)n ? chi : (int64_t)n;
+  if (rec.w) {
+    store_identity(rec, 2 * t);
+    store_identity(rec, 2 * t + 1);
+  }
+  size_t s = seg_find(off, nseg, level, clo);
+  int64_t lo = 0, hi = 0;
+  if (s < nseg) seg_range(off, s, level, lo, hi);
+  ge_p3 acc, x;
+  ge_p3_0(acc);
+  bool touched = false;
+#pragma unroll 1
+  for (int64_t i = clo; i < cend; ++i) {
+    while (s < nseg && hi <= i) {
+      if (touched) seg_flush(out, rec, s, lo, hi, clo, chi, t, acc);
+      touched = false;
+      ge_p3_0(acc);
+      ++s;
+      if (s < nseg) seg_range(off, s, level, lo, hi);
+    }
+    if (s < nseg && lo <= i) {
+      const size_t j = gather ? (size_t)gather[i] : (size_t)i;
+      load_p3(x, elems, j);
+      if (touched) ge_p3_add(acc, acc, x);
+      else acc = x;
+      touched = true;
+    }
+  }
+  if (touched) seg_flush(out, rec, s, lo, hi, clo, chi, t, acc);
+}
+
+__global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *off, size_t nseg) {
+  const size_t s = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  if (s < nseg && off[s] == off[s + 1]) store_identity(out, s);
+}
+
+hipError_t launch_seg_level(PtView out, PtView rec, PtView elems, size_t n, const int64_t *off,
+                            size_t nseg, const int32_t *gather, int level, hipStream_t s) {
+  const size_t T = (n + CHN - 1) >> CH;
+  if (!T || !nseg) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_level, dim3((unsigned)((T + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, out, rec,
+                     elems, n, off, nseg, gather, level);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s) {
+  if (!nseg) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_empty, dim3((unsigned)((nseg + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, out, off,
+                     nseg);
+  return hipGetLastError();
+}
+
+}  // namespace ft

@@ -1,0 +1,36 @@
+// Host-side launchers of the gfx950 kernels (kernels.hip, aggregate.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "layout.h"
+
+namespace ft {
+
+hipError_t launch_set_identity(PtView v, size_t lo, size_t hi, hipStream_t s);
+hipError_t launch_broadcast(PtView d, size_t n, PtView src, size_t idx, hipStream_t s);
+hipError_t launch_copy(PtView d, PtView src, size_t n, hipStream_t s);
+hipError_t launch_add(PtView res, PtView b, size_t n, bool sub, hipStream_t s);
+hipError_t launch_neg(PtView v, size_t n, hipStream_t s);
+hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table,
+                             hipStream_t s);
+hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s);
+hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s);
+hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
+hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
+                          const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
+                          hipStream_t s);
+hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s);
+hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long long *bad, hipStream_t s);
+unsigned reduce_grid(size_t n);
+hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, hipStream_t s);
+
+// aggregate.hip
+constexpr int SEG_CHUNK_LOG2 = 4;                     // elements per lane per level
+size_t scan_tiles(size_t n);
+hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s);
+hipError_t launch_scan_apply(PtView v, size_t n, PtView tile_prefix, bool exclusive, hipStream_t s);
+hipError_t launch_seg_level(PtView out, PtView rec_out, PtView elems, size_t n_elems,
+                            const int64_t *off, size_t nseg, const int32_t *gather, int level,
+                            hipStream_t s);
+hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s);
+
+}  // namespace ft

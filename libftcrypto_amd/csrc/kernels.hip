@@ -1,0 +1,452 @@
+// Elementwise, scalar-multiplication, encoding and reduction kernels for
+// gfx950, plus their host-side launchers (namespace ft, kernels.h).
+//
+// One lane owns one element (point or ciphertext) for the whole kernel; the
+// hot loops are pure VALU integer work in registers.  Secret scalar digits
+// never drive a branch or an address: they are consumed from register shift
+// streams in window order, and every table entry (registers or LDS) is read
+// and merged with v_cndmask.
+#include "kernels.h"
+#include "scalarmult.h"
+
+namespace ft {
+
+static inline unsigned grid_for(size_t n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
+
+// ------------------------------------------------------------ elementwise
+
+__global__ void __launch_bounds__(TPB) k_set_identity(PtView v, size_t lo, size_t hi) {
+  const size_t i = lo + blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i < hi) store_identity(v, i);
+}
+
+__global__ void __launch_bounds__(TPB) k_broadcast(PtView d, size_t n, PtView s, size_t idx) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i < n) copy_point(d, i, s, idx);
+}
+
+__global__ void __launch_bounds__(TPB) k_copy(PtView d, PtView s, size_t n) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i < n) copy_point(d, i, s, i);
+}
+
+// res[i] = res[i] +/- b[i]  (ftcrypto.cu:1089-1110, 1220-1241)
+template <bool SUB>
+__global__ void __launch_bounds__(TPB) k_add(PtView res, PtView b, size_t n) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  ge_p3 p, q;
+  load_p3(p, res, i);
+  load_p3(q, b, i);
+  ge_cached c;
+  ge_p3_to_cached(c, q);
+  if (SUB) ge_cached_cneg(c, 1u);
+  ge_p1p1 t;
+  ge_add(t, p, c);
+  ge_p1p1_to_p3(p, t);
+  store_p3(res, i, p);
+}
+
+__global__ void __launch_bounds__(TPB) k_neg(PtView v, size_t n) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  ge_p3 p;
+  load_p3(p, v, i);
+  ge_p3_neg(p, p);
+  store_p3(v, i, p);
+}
+
+// ------------------------------------------------------ fixed-base comb
+
+// out[i] = k_i * P with P's comb table (sodium_ge.cu:743-792 structure; the
+// reference routes this through the variable-base routine on the GPU,
+// ftcrypto.cu:895-913; the result is the same point).  SRC: 0 = 32-byte
+// scalars, 1 = uint64 scalars (ftcrypto.cu:937-963 packing).
+template <int SRC>
+__global__ void __launch_bounds__(TPB) k_fixed_base(PtView out, const void *sc, size_t n,
+                                                    const uint32_t *table) {
+  __shared__ __align__(16) uint32_t tab[COMB_TABLE_WORDS];
+  lds_load_table(tab, table);
+  __syncthreads();
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (SRC == 0) {
+    load_scalar(k, static_cast<const uint32_t *>(sc), i);
+  } else {
+    const uint64_t x = static_cast<const uint64_t *>(sc)[i];
+    k[0] = (uint32_t)x;
+    k[1] = (uint32_t)(x >> 32);
+  }
+  uint32_t e[16], odd[8], even[8];
+  recode16(e, k);
+  split_odd_even(odd, even, e);
+  ge_p3 h;
+  ge_p3_0(h);
+  comb_pass<8>(h, tab, odd, 32);
+  dbl4(h);
+  comb_pass<8>(h, tab, even, 32);
+  store_p3(out, i, h);
+}
+
+// ------------------------------------------------------ variable base
+
+// pts[i] = k_i * pts[i]; scalar stride 0 applies one scalar to every element
+// (ft_scale, ftcrypto.cu:966-983), stride 8 words is ft_mul (:986-1011).
+__global__ void __launch_bounds__(TPB) k_varbase(PtView pts, const uint32_t *sc, size_t sc_stride,
+                                                 size_t n) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k[8], e[16];
+  ld_words(k, sc + i * sc_stride);
+  recode16(e, k);
+  ge_p3 p, h;
+  load_p3(p, pts, i);
+  varbase_windows(h, p, e, 0);
+  store_p3(pts, i, h);
+}
+
+// 32-bit multipliers: 9 signed digits (the bit length is public), placed at
+// stream positions 55..63 so the window loop runs over nine windows only
+__device__ __forceinline__ void recode_u32(uint32_t e[16], uint32_t k) {
+  int carry = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) e[q] = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    int d = (i < 8 ? (int)((k >> (4 * i)) & 15u) : 0) + carry;
+    if (i < 8) {
+      carry = (d + 8) >> 4;
+      d -= carry << 4;
+    }
+    const int v = i + 55;
+    e[v >> 2] |= ((uint32_t)d & 0xffu) << (8 * (v & 3));
+  }
+}
+
+// a[i] = k_i * a[i] (and b[i] = k_i * b[i] when two): homomorphic
+// multiplication of a ciphertext by a 32-bit constant.
+__global__ void __launch_bounds__(TPB) k_mul_u32(PtView a, PtView b, int two, const uint32_t *k,
+                                                 size_t n) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t kk = k[i];
+  uint32_t e[16];
+  ge_p3 p, h;
+  recode_u32(e, kk);
+  load_p3(p, a, i);
+  varbase_windows(h, p, e, 55);
+  store_p3(a, i, h);
+  if (two) {
+    recode_u32(e, kk);
+    load_p3(p, b, i);
+    varbase_windows(h, p, e, 55);
+    store_p3(b, i, h);
+  }
+}
+
+// ------------------------------------------------------ comb table build
+
+// table[r][j] = (j+1) * 256^r * P in affine (y+x, y-x, 2dxy) form
+// (sodium_ge.cu:480 layout).  One block of 256 lanes, one entry per lane;
+// P is public, so the generic constant-time routine is simply reused.
+__global__ void __launch_bounds__(TPB) k_build_comb(uint32_t *table, PtView src, size_t idx,
+                                                    int use_base) {
+  const int t = threadIdx.x, r = t >> 3, j = t & 7;
+  uint32_t k[8] = {0, 0, 0, 0, 0, 0, 0, 0}, e[16];
+  const int bit = 8 * r;                       // (j+1) << (8 r)
+  k[bit >> 5] = (uint32_t)(j + 1) << (bit & 31);
+  recode16(e, k);
+  ge_p3 p, h;
+  if (use_base) {
+    // base point B = (x, 4/5) from its encoding 58 66 .. 66 (ftcrypto.cu:120-125)
+    uint32_t w[8];
+    w[0] = 0x66666658u;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) w[q] = 0x66666666u;
+    ge_decode(p, w);
+  } else {
+    load_p3(p, src, idx);
+  }
+  varbase_windows(h, p, e, 0);
+  ge_precomp en;
+  ge_p3_to_precomp(en, h);
+  comb_entry_store(table + (r * 8 + j) * COMB_ENTRY_WORDS, en);
+}
+
+// ------------------------------------------------------ ElGamal encryption
+
+// c1[i] = r_i B, c2[i] = m_i B + r_i H (or, with ACC, c1 += r_i B, c2 += r_i H:
+// rerandomisation).  The comb runs for both tables at once so the four
+// middle doublings of r H are shared with m B; the message uses ND digits
+// (ND = 0 for rerandomisation), a public bound on its bit length.
+template <bool ACC>
+__global__ void __launch_bounds__(TPB) k_encrypt(PtView c1, PtView c2, const uint32_t *btab,
+                                                 const uint32_t *htab, const uint64_t *m, int nd,
+                                                 const uint32_t *r, size_t n) {
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k[8], e[16], odd[8], even[8];
+  load_scalar(k, r, i);
+  recode16(e, k);
+  // message digits (at most 17): odd ones and even ones, low first
+  uint32_t mo[3] = {0, 0, 0}, me[3] = {0, 0, 0};
+  if (!ACC) {
+    const uint64_t x = m[i];
+    int carry = 0;
+#pragma unroll
+    for (int q = 0; q < 17; ++q) {
+      int d = (q < 16 ? (int)((x >> (4 * q)) & 15u) : 0) + carry;
+      carry = (d + 8) >> 4;
+      d -= carry << 4;
+      if (q == nd - 1) d += carry << 4;       // top digit keeps the final carry
+      const uint32_t db = q < nd ? ((uint32_t)d & 0xffu) : 0u;
+      const int h = q >> 1;
+      if (q & 1) mo[h >> 2] |= db << (8 * (h & 3));
+      else me[h >> 2] |= db << (8 * (h & 3));
+      if (q >= nd - 1) carry = 0;
+    }
+  }
+  ge_p3 a1, a2;
+  // C1 = r B
+  split_odd_even(odd, even, e);
+  ge_p3_0(a1);
+  comb_pass<8>(a1, tb, odd, 32);
+  dbl4(a1);
+  comb_pass<8>(a1, tb, even, 32);
+  // C2 = r H + m B, sharing the four doublings
+  split_odd_even(odd, even, e);
+  ge_p3_0(a2);
+  comb_pass<8>(a2, th, odd, 32);
+  if (!ACC) comb_pass<3>(a2, tb, mo, nd >> 1);
+  dbl4(a2);
+  comb_pass<8>(a2, th, even, 32);
+  if (!ACC) comb_pass<3>(a2, tb, me, (nd + 1) >> 1);
+  if (ACC) {
+    ge_p3 o;
+    load_p3(o, c1, i);
+    ge_p3_add(a1, o, a1);
+    load_p3(o, c2, i);
+    ge_p3_add(a2, o, a2);
+  }
+  store_p3(c1, i, a1);
+  store_p3(c2, i, a2);
+}
+
+// ------------------------------------------------------ encode / decode
+
+// Compressed encodings (sodium_ge.cu:382-394) of n points; every lane batch-
+// inverts BATCH Z coordinates with one field inversion (Montgomery's trick).
+// A zero Z (never produced by the group law) encodes as 32 zero bytes, as the
+// reference's invert(0) = 0 does, without disturbing the other lanes' batch.
+constexpr int ENC_BATCH = 8;
+
+__global__ void __launch_bounds__(TPB) k_encode(uint32_t *out, PtView v, size_t n) {
+  const size_t lanes = (size_t)gridDim.x * TPB;
+  const size_t i0 = blockIdx.x * (size_t)TPB + threadIdx.x;
+  fe z[ENC_BATCH], acc[ENC_BATCH];
+  uint32_t zero[ENC_BATCH];
+  fe one;
+  fe_1(one);
+  fe run = one;
+#pragma unroll
+  for (int b = 0; b < ENC_BATCH; ++b) {
+    const size_t i = i0 + b * lanes;
+    if (i < n) load_fe(z[b], v.w + (2 * v.cap + i) * 8);
+    else z[b] = one;
+    zero[b] = fe_iszero(z[b]);
+    fe_cmov(z[b], one, zero[b]);
+    acc[b] = run;                  // product of z[0..b-1]
+    fe_mul(run, run, z[b]);
+  }
+  fe inv;
+  fe_invert(inv, run);
+#pragma unroll
+  for (int b = ENC_BATCH - 1; b >= 0; --b) {
+    fe zi;
+    fe_mul(zi, inv, acc[b]);       // 1 / z[b]
+    fe_mul(inv, inv, z[b]);
+    const size_t i = i0 + b * lanes;
+    if (i < n) {
+      fe X, Y, x, y;
+      load_fe(X, v.w + (0 * v.cap + i) * 8);
+      load_fe(Y, v.w + (1 * v.cap + i) * 8);
+      fe_mul(x, X, zi);
+      fe_mul(y, Y, zi);
+      uint32_t w[8];
+      ge_encode_xy(w, x, y);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) w[q] = zero[b] ? 0u : w[q];
+      st_words(out + i * 8, w);
+    }
+  }
+}
+
+// Decode n encodings; bad[0] receives 1 + the largest failing index (0: none).
+__global__ void __launch_bounds__(TPB) k_decode(PtView v, const uint32_t *in, size_t n,
+                                                unsigned long long *bad) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  long long fail = -1;
+  if (i < n) {
+    uint32_t w[8];
+    ld_words(w, in + i * 8);
+    ge_p3 p;
+    const uint32_t ok = ge_decode(p, w);
+    store_p3(v, i, p);
+    fail = ok ? -1 : (long long)i;
+  }
+  // wave-wide max, then one atomic per wave; bad holds (largest index + 1)
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(fail, off);
+    fail = o > fail ? o : fail;
+  }
+  if ((threadIdx.x & 63) == 0 && fail >= 0) atomicMax(bad, (unsigned long long)fail + 1ull);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// ------------------------------------------------------ reduction
+
+// Each block sums a grid-strided share of its points, then a 256-lane tree
+// in LDS; part[blockIdx.x] receives the block sum.
+__global__ void __launch_bounds__(TPB) k_reduce_blocks(PtView part, PtView v, size_t n) {
+  __shared__ __align__(16) uint32_t sh[TPB / 2 * 32];     // 128 points x 128 B
+  const size_t lanes = (size_t)gridDim.x * TPB;
+  size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  ge_p3 acc;
+  ge_p3_0(acc);
+  if (i < n) { load_p3(acc, v, i); i += lanes; }
+#pragma unroll 1
+  for (; i < n; i += lanes) {
+    ge_p3 q;
+    load_p3(q, v, i);
+    ge_p3_add(acc, acc, q);
+  }
+  // tree: lanes [s, 2s) hand their sums to lanes [0, s)
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    const int t = threadIdx.x;
+    if (t >= s && t < 2 * s) {
+      uint32_t w[8];
+      uint32_t *dst = sh + (t - s) * 32;
+      fe_store_words(w, acc.X); for (int q = 0; q < 8; ++q) dst[q] = w[q];
+      fe_store_words(w, acc.Y); for (int q = 0; q < 8; ++q) dst[8 + q] = w[q];
+      fe_store_words(w, acc.Z); for (int q = 0; q < 8; ++q) dst[16 + q] = w[q];
+      fe_store_words(w, acc.T); for (int q = 0; q < 8; ++q) dst[24 + q] = w[q];
+    }
+    __syncthreads();
+    if (t < s) {
+      ge_p3 q;
+      const uint32_t *src = sh + t * 32;
+      fe_from_words(q.X, src);
+      fe_from_words(q.Y, src + 8);
+      fe_from_words(q.Z, src + 16);
+      fe_from_words(q.T, src + 24);
+      ge_p3_add(acc, acc, q);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) store_p3(part, blockIdx.x, acc);
+}
+
+// ------------------------------------------------------ host launchers
+
+#define FT_LAUNCH_CHECK() return hipGetLastError()
+
+hipError_t launch_set_identity(PtView v, size_t lo, size_t hi, hipStream_t s) {
+  if (hi <= lo) return hipSuccess;
+  hipLaunchKernelGGL(k_set_identity, dim3(grid_for(hi - lo)), dim3(TPB), 0, s, v, lo, hi);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_broadcast(PtView d, size_t n, PtView src, size_t idx, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_broadcast, dim3(grid_for(n)), dim3(TPB), 0, s, d, n, src, idx);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_copy(PtView d, PtView src, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(TPB), 0, s, d, src, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_add(PtView res, PtView b, size_t n, bool sub, hipStream_t s) {
+  if (!n) return hipSuccess;
+  if (sub) hipLaunchKernelGGL(k_add<true>, dim3(grid_for(n)), dim3(TPB), 0, s, res, b, n);
+  else hipLaunchKernelGGL(k_add<false>, dim3(grid_for(n)), dim3(TPB), 0, s, res, b, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_neg(PtView v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_neg, dim3(grid_for(n)), dim3(TPB), 0, s, v, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table,
+                             hipStream_t s) {
+  if (!n) return hipSuccess;
+  if (u64) hipLaunchKernelGGL(k_fixed_base<1>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table);
+  else hipLaunchKernelGGL(k_fixed_base<0>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_varbase, dim3(grid_for(n)), dim3(TPB), 0, s, pts, sc, sc_stride_words, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_mul_u32, dim3(grid_for(n)), dim3(TPB), 0, s, a, b, (int)two, k, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_comb, dim3(1), dim3(TPB), 0, s, table, src, idx, (int)use_base);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
+                          const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
+                          hipStream_t s) {
+  if (!n) return hipSuccess;
+  if (accumulate)
+    hipLaunchKernelGGL(k_encrypt<true>, dim3(grid_for(n)), dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+  else
+    hipLaunchKernelGGL(k_encrypt<false>, dim3(grid_for(n)), dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  const size_t per = (size_t)TPB * ENC_BATCH;
+  const unsigned grid = (unsigned)((n + per - 1) / per);
+  hipLaunchKernelGGL(k_encode, dim3(grid), dim3(TPB), 0, s, out, v, n);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long long *bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, bad, 0ull);
+  if (!n) return hipGetLastError();
+  hipLaunchKernelGGL(k_decode, dim3(grid_for(n)), dim3(TPB), 0, s, v, in, n, bad);
+  FT_LAUNCH_CHECK();
+}
+
+unsigned reduce_grid(size_t n) {
+  const size_t g = (n + TPB - 1) / TPB;
+  return (unsigned)(g < 2048 ? (g ? g : 1) : 2048);
+}
+
+hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_blocks, dim3(grid), dim3(TPB), 0, s, part, v, n);
+  FT_LAUNCH_CHECK();
+}
+
+}  // namespace ft

@@ -1,0 +1,121 @@
+// Resident layout of point arrays in HBM and the comb tables in LDS.
+//
+// A point array of capacity `cap` is four coordinate planes X | Y | Z | T,
+// each `cap` x 32 bytes: element i's coordinate c is the 256-bit value (not
+// reduced mod p) at word offset (c * cap + i) * 8.  A wavefront reading one
+// coordinate of 64 consecutive elements touches one contiguous 2 KiB run, in
+// two 16-byte loads per lane.  (The reference keeps an array-of-structures of
+// 160-byte ge25519_p3 in managed memory, ftcrypto.cu:21-27, 215-242.)
+//
+// A comb table holds 32 rows x 8 affine entries (y+x, y-x, 2dxy) in the
+// 9-limb form, each entry padded to 28 words (112 bytes): 28,672 bytes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "ge25519.h"
+
+namespace ft {
+
+struct PtView {
+  uint32_t *w;    // base of plane X
+  size_t cap;     // elements per plane
+};
+
+constexpr int COMB_ROWS = 32;
+constexpr int COMB_ENTRY_WORDS = 28;
+constexpr int COMB_TABLE_WORDS = COMB_ROWS * 8 * COMB_ENTRY_WORDS;   // 7168 words
+
+__device__ __forceinline__ void ld_words(uint32_t w[8], const uint32_t *p) {
+  const uint4 a = *reinterpret_cast<const uint4 *>(p);
+  const uint4 b = *reinterpret_cast<const uint4 *>(p + 4);
+  w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+  w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+}
+
+__device__ __forceinline__ void st_words(uint32_t *p, const uint32_t w[8]) {
+  *reinterpret_cast<uint4 *>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+  *reinterpret_cast<uint4 *>(p + 4) = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+__device__ __forceinline__ void load_fe(fe &r, const uint32_t *p) {
+  uint32_t w[8];
+  ld_words(w, p);
+  fe_from_words(r, w);
+}
+
+__device__ __forceinline__ void store_fe(uint32_t *p, const fe &a) {
+  uint32_t w[8];
+  fe_store_words(w, a);
+  st_words(p, w);
+}
+
+__device__ __forceinline__ void load_p3(ge_p3 &r, const PtView v, size_t i) {
+  load_fe(r.X, v.w + (0 * v.cap + i) * 8);
+  load_fe(r.Y, v.w + (1 * v.cap + i) * 8);
+  load_fe(r.Z, v.w + (2 * v.cap + i) * 8);
+  load_fe(r.T, v.w + (3 * v.cap + i) * 8);
+}
+
+__device__ __forceinline__ void store_p3(const PtView v, size_t i, const ge_p3 &r) {
+  store_fe(v.w + (0 * v.cap + i) * 8, r.X);
+  store_fe(v.w + (1 * v.cap + i) * 8, r.Y);
+  store_fe(v.w + (2 * v.cap + i) * 8, r.Z);
+  store_fe(v.w + (3 * v.cap + i) * 8, r.T);
+}
+
+// identity (0, 1, 1, 0) written without arithmetic
+__device__ __forceinline__ void store_identity(const PtView v, size_t i) {
+  const uint32_t z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t o[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  st_words(v.w + (0 * v.cap + i) * 8, z);
+  st_words(v.w + (1 * v.cap + i) * 8, o);
+  st_words(v.w + (2 * v.cap + i) * 8, o);
+  st_words(v.w + (3 * v.cap + i) * 8, z);
+}
+
+// copy of the 32 packed bytes of each coordinate, no arithmetic
+__device__ __forceinline__ void copy_point(const PtView d, size_t i, const PtView s, size_t j) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    uint32_t w[8];
+    ld_words(w, s.w + (c * s.cap + j) * 8);
+    st_words(d.w + (c * d.cap + i) * 8, w);
+  }
+}
+
+// ---------------------------------------------------------- comb entries
+
+__device__ __forceinline__ void comb_entry_store(uint32_t *dst, const ge_precomp &e) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { dst[i] = e.ypx.v[i]; dst[9 + i] = e.ymx.v[i]; dst[18 + i] = e.xy2d.v[i]; }
+  dst[27] = 0;
+}
+
+// Constant-time selection of |b| * row entry from an LDS-resident row:
+// every entry is read (the LDS address never depends on b) and merged with
+// v_cndmask; b < 0 negates the result.  b outside [-8, 8] selects the
+// identity (the reference's cmov8 behaviour, sodium_ge.cu:458-488).
+__device__ __forceinline__ void comb_select(ge_precomp &t, const uint32_t *row, int b) {
+  const uint32_t neg = (uint32_t)b >> 31;
+  const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
+  ge_precomp_0(t);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
+    const uint32_t *e = row + j * COMB_ENTRY_WORDS;
+    uint32_t w[28];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const uint4 x = reinterpret_cast<const uint4 *>(e)[q];
+      w[4 * q] = x.x; w[4 * q + 1] = x.y; w[4 * q + 2] = x.z; w[4 * q + 3] = x.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      t.ypx.v[i] = m ? w[i] : t.ypx.v[i];
+      t.ymx.v[i] = m ? w[9 + i] : t.ymx.v[i];
+      t.xy2d.v[i] = m ? w[18 + i] : t.xy2d.v[i];
+    }
+  }
+  ge_precomp_cneg(t, neg);
+}
+
+}  // namespace ft

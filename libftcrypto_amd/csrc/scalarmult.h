@@ -1,0 +1,183 @@
+// Scalar-multiplication building blocks shared by the kernels: signed
+// radix-16 digit streams, the fixed-base comb pass and the constant-time
+// variable-base window loop.
+#pragma once
+#include "layout.h"
+
+namespace ft {
+
+constexpr int TPB = 256;
+
+// ------------------------------------------------------------ digits
+//
+// Signed radix-16 digits (sodium_ge.cu:693-708) are packed four to a word
+// as bytes and consumed through a register shift stream: the byte taken in
+// a window depends only on the window counter, so neither a branch nor an
+// address ever depends on a secret digit.
+
+// 64 digits of k: stream words e[0..15], digit i at byte i&3 of e[i>>2]
+__device__ __forceinline__ void recode16(uint32_t e[16], const uint32_t k[8]) {
+  int carry = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) e[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    int d = (int)((k[i >> 3] >> (4 * (i & 7))) & 15u) + carry;
+    if (i < 63) {
+      carry = (d + 8) >> 4;
+      d -= carry << 4;
+    }
+    e[i >> 2] |= ((uint32_t)d & 0xffu) << (8 * (i & 3));
+  }
+}
+
+// split into the odd-digit and even-digit streams of the comb
+__device__ __forceinline__ void split_odd_even(uint32_t odd[8], uint32_t even[8], const uint32_t e[16]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) { odd[q] = 0; even[q] = 0; }
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    const uint32_t d = (e[i >> 2] >> (8 * (i & 3))) & 0xffu;
+    const int q = i >> 1;
+    if (i & 1) odd[q >> 2] |= d << (8 * (q & 3));
+    else even[q >> 2] |= d << (8 * (q & 3));
+  }
+}
+
+// pop the lowest digit of a low-first stream of NW words (q = pops so far)
+template <int NW>
+__device__ __forceinline__ int pop_lo(uint32_t s[NW], int q) {
+  const int b = (int)(int8_t)(uint8_t)(s[0] & 0xffu);
+  s[0] >>= 8;
+  if ((q & 3) == 3) {
+#pragma unroll
+    for (int k = 0; k < NW - 1; ++k) s[k] = s[k + 1];
+    s[NW - 1] = 0;
+  }
+  return b;
+}
+
+// pop the highest digit of a 16-word high-first stream (v = its digit index)
+__device__ __forceinline__ int pop_hi(uint32_t s[16], int v) {
+  const int b = (int)(int8_t)(uint8_t)(s[15] >> 24);
+  s[15] <<= 8;
+  if ((v & 3) == 0) {
+#pragma unroll
+    for (int k = 15; k > 0; --k) s[k] = s[k - 1];
+    s[0] = 0;
+  }
+  return b;
+}
+
+__device__ __forceinline__ void load_scalar(uint32_t k[8], const uint32_t *sc, size_t i) {
+  ld_words(k, sc + i * 8);
+}
+
+// comb over one table: h += sum over NWIN windows of d * row(q), rows in LDS,
+// digits from a low-first stream (q-th digit uses row q of the table)
+template <int NW>
+__device__ __forceinline__ void comb_pass(ge_p3 &h, const uint32_t *tab, uint32_t s[NW], int nwin,
+                                          int row_step = 1) {
+  ge_precomp t;
+  ge_p1p1 r;
+#pragma unroll 1
+  for (int q = 0; q < nwin; ++q) {
+    const int b = pop_lo<NW>(s, q);
+    comb_select(t, tab + q * row_step * 8 * COMB_ENTRY_WORDS, b);
+    ge_madd(r, h, t);
+    ge_p1p1_to_p3(h, r);
+  }
+}
+
+__device__ __forceinline__ void dbl4(ge_p3 &h) {
+  ge_p1p1 r;
+  ge_p2 s;
+  ge_p3_dbl(r, h);
+  ge_p1p1_to_p2(s, r);
+  ge_p2_dbl(r, s);
+  ge_p1p1_to_p2(s, r);
+  ge_p2_dbl(r, s);
+  ge_p1p1_to_p2(s, r);
+  ge_p2_dbl(r, s);
+  ge_p1p1_to_p3(h, r);
+}
+
+__device__ __forceinline__ void lds_load_table(uint32_t *lds, const uint32_t *g) {
+  for (int i = threadIdx.x; i < COMB_TABLE_WORDS / 4; i += TPB)
+    reinterpret_cast<uint4 *>(lds)[i] = reinterpret_cast<const uint4 *>(g)[i];
+}
+
+// cached multiples 1P..8P (sodium_ge.cu:664-689)
+__device__ __forceinline__ void build_cached8(ge_cached tab[8], const ge_p3 &p) {
+  ge_p3 m[8];
+  ge_p1p1 t;
+  m[0] = p;
+  ge_p3_dbl(t, p);  ge_p1p1_to_p3(m[1], t);
+  ge_p3_to_cached(tab[0], m[0]);
+  ge_p3_to_cached(tab[1], m[1]);
+  ge_add(t, m[0], tab[1]); ge_p1p1_to_p3(m[2], t);     // 3P
+  ge_p3_dbl(t, m[1]); ge_p1p1_to_p3(m[3], t);          // 4P
+  ge_p3_to_cached(tab[2], m[2]);
+  ge_p3_to_cached(tab[3], m[3]);
+  ge_add(t, m[0], tab[3]); ge_p1p1_to_p3(m[4], t);     // 5P
+  ge_p3_dbl(t, m[2]); ge_p1p1_to_p3(m[5], t);          // 6P
+  ge_p3_to_cached(tab[4], m[4]);
+  ge_p3_to_cached(tab[5], m[5]);
+  ge_add(t, m[0], tab[5]); ge_p1p1_to_p3(m[6], t);     // 7P
+  ge_p3_dbl(t, m[3]); ge_p1p1_to_p3(m[7], t);          // 8P
+  ge_p3_to_cached(tab[6], m[6]);
+  ge_p3_to_cached(tab[7], m[7]);
+}
+
+// |b| * P from the register table (all eight entries merged), b < 0 negates.
+__device__ __forceinline__ void cached_select(ge_cached &t, const ge_cached tab[8], int b) {
+  const uint32_t neg = (uint32_t)b >> 31;
+  const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
+  ge_cached_0(t);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      t.YpX.v[l] = m ? tab[j].YpX.v[l] : t.YpX.v[l];
+      t.YmX.v[l] = m ? tab[j].YmX.v[l] : t.YmX.v[l];
+      t.Z.v[l] = m ? tab[j].Z.v[l] : t.Z.v[l];
+      t.T2d.v[l] = m ? tab[j].T2d.v[l] : t.T2d.v[l];
+    }
+  }
+  ge_cached_cneg(t, neg);
+}
+
+// h = k * p over digits vlo..63 of a high-first stream (sodium_ge.cu:650-731:
+// per window one table add and four doublings)
+__device__ __forceinline__ void varbase_windows(ge_p3 &h, const ge_p3 &p, uint32_t e[16], int vlo) {
+  ge_cached tab[8];
+  build_cached8(tab, p);
+  ge_p3_0(h);
+  ge_cached c;
+  ge_p1p1 r;
+  ge_p2 s;
+#pragma unroll 1
+  for (int v = 63; v > vlo; --v) {
+    cached_select(c, tab, pop_hi(e, v));
+    ge_add(r, h, c);
+    ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s);
+    ge_p1p1_to_p3(h, r);
+  }
+  cached_select(c, tab, pop_hi(e, vlo));
+  ge_add(r, h, c);
+  ge_p1p1_to_p3(h, r);
+}
+
+
+__device__ __forceinline__ void recode16_words(uint32_t e[16], const uint32_t k[8]) { recode16(e, k); }
+
+__device__ __forceinline__ void varbase_mul(ge_p3 &h, const ge_p3 &p, uint32_t e[16]) {
+  varbase_windows(h, p, e, 0);
+}
+
+}  // namespace ft

@@ -1,0 +1,57 @@
+// Host build of the library's field and group headers, for CPU tests of the
+// exact arithmetic the gfx950 kernels run (bounds, reduction, encoding).
+#include <cstring>
+#include "../../libftcrypto_amd/csrc/fe25519.h"
+#include "../../libftcrypto_amd/csrc/ge25519.h"
+
+using namespace ft;
+
+static fe ld(const uint32_t *a) { fe x; std::memcpy(x.v, a, 36); return x; }
+static void st(uint32_t *r, const fe &x) { std::memcpy(r, x.v, 36); }
+
+extern "C" {
+void h_fe_mul(uint32_t *r, const uint32_t *a, const uint32_t *b) { fe z; fe_mul(z, ld(a), ld(b)); st(r, z); }
+void h_fe_sq(uint32_t *r, const uint32_t *a) { fe z; fe_sq(z, ld(a)); st(r, z); }
+void h_fe_carry(uint32_t *r, const uint32_t *a) { fe z = ld(a); fe_carry(z); st(r, z); }
+void h_fe_invert(uint32_t *r, const uint32_t *a) { fe z; fe_invert(z, ld(a)); st(r, z); }
+void h_fe_pow22523(uint32_t *r, const uint32_t *a) { fe z; fe_pow22523(z, ld(a)); st(r, z); }
+void h_fe_canonical(uint32_t *w, const uint32_t *a) { fe_canonical_words(w, ld(a)); }
+void h_fe_store(uint32_t *w, const uint32_t *a) { fe_store_words(w, ld(a)); }
+void h_fe_from_words(uint32_t *r, const uint32_t *w) { fe z; fe_from_words(z, w); st(r, z); }
+void h_fe_sub(uint32_t *r, const uint32_t *a, const uint32_t *b) { fe z; fe_sub(z, ld(a), ld(b)); st(r, z); }
+void h_constants(uint32_t *d, uint32_t *d2, uint32_t *s) { st(d, FE_D); st(d2, FE_D2); st(s, FE_SQRTM1); }
+int h_decode_encode(uint32_t *out, const uint32_t *in) {
+  ge_p3 p;
+  uint32_t ok = ge_decode(p, in);
+  ge_p3_encode(out, p);
+  return (int)ok;
+}
+void h_add_encoded(uint32_t *out, const uint32_t *a, const uint32_t *b) {
+  ge_p3 p, q;
+  ge_decode(p, a);
+  ge_decode(q, b);
+  ge_p3_add(p, p, q);
+  ge_p3_encode(out, p);
+}
+void h_sub_encoded(uint32_t *out, const uint32_t *a, const uint32_t *b) {
+  ge_p3 p, q;
+  ge_decode(p, a);
+  ge_decode(q, b);
+  ge_p3_sub(p, p, q);
+  ge_p3_encode(out, p);
+}
+void h_dbl_encoded(uint32_t *out, const uint32_t *a) {
+  ge_p3 p;
+  ge_p1p1 t;
+  ge_decode(p, a);
+  ge_p3_dbl(t, p);
+  ge_p1p1_to_p3(p, t);
+  ge_p3_encode(out, p);
+}
+void h_neg_encoded(uint32_t *out, const uint32_t *a) {
+  ge_p3 p;
+  ge_decode(p, a);
+  ge_p3_neg(p, p);
+  ge_p3_encode(out, p);
+}
+}

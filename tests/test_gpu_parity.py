@@ -1,0 +1,395 @@
+"""Parity of the gfx950 library (through its C ABI) with the CPU oracle.
+
+Every comparison is byte-exact on the canonical 32-byte compressed encodings
+(or on the exact int64 / index outputs), on identical seeded inputs.  Sizes
+keep the oracle to a few seconds; the full-size runs are in
+tests/test_gpu_configs.py.
+"""
+import numpy as np
+import pytest
+
+import edref
+import libftcrypto_amd as ft
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = edref.L
+IDENT = bytes([1] + [0] * 31)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    ft.init()
+
+
+def scalars(n, seed, stream=0):
+    return ft.random_scalars(n, seed, stream)
+
+
+def sb(k):
+    return np.frombuffer(int.to_bytes(k, 32, "little"), dtype=np.uint8)
+
+
+def edge_scalars():
+    ks = [0, 1, 2, 7, 8, 9, 15, 16, 255, 256, L - 1, L, L + 1, 2**252, 2**253 - 1, 2**255 - 1,
+          2**254 + 2**251, int("8" * 63, 16), int("7" * 63, 16)]
+    return np.stack([sb(k) for k in ks])
+
+
+def same(gpu_arr, oracle_arr):
+    a = gpu_arr.to_bytes_folded()
+    b = oracle_arr.to_bytes()
+    assert len(a) == len(b)
+    if not np.array_equal(a, b):
+        bad = np.nonzero((a.reshape(-1, 32) != b.reshape(-1, 32)).any(axis=1))[0]
+        raise AssertionError("%d of %d points differ, first at %d" % (len(bad), len(a) // 32, bad[0]))
+
+
+# ------------------------------------------------------------ scalar mults
+
+def test_fixed_base_edges_and_random():
+    s = np.concatenate([edge_scalars(), scalars(3000, 101)])
+    same(ft.GeArray.from_scalars(s), O.Array.from_scalars(s))
+
+
+def test_fixed_base_top_digit_beyond_precondition():
+    # byte 31 > 127 breaks the reference precondition; the digit rule is kept
+    rng = np.random.default_rng(5)
+    s = rng.integers(0, 256, size=(64, 32), dtype=np.uint8)
+    s[:, 31] |= 0x80
+    same(ft.GeArray.from_scalars(s), O.Array.from_scalars(s))
+
+
+def test_small_scalars():
+    rng = np.random.default_rng(6)
+    m = np.concatenate([np.array([0, 1, 2, 5, 2**32, 2**63, 2**64 - 1], dtype=np.uint64),
+                        rng.integers(0, 2**63, size=500, dtype=np.uint64)])
+    same(ft.GeArray.from_small_scalars(m), O.Array.from_small_scalars(m))
+
+
+def test_variable_base_mul_and_scale():
+    base = scalars(1500, 102)
+    k = np.concatenate([edge_scalars(), scalars(1500 - len(edge_scalars()), 103)])
+    g = ft.GeArray.from_scalars(base)
+    o = O.Array.from_scalars(base)
+    g.mul(k)
+    o.mul(k)
+    same(g, o)
+    one = bytes(scalars(1, 104)[0])
+    g.scale(one)
+    o.scale(one)
+    same(g, o)
+    with pytest.raises(ft.FtError) as e:
+        g.mul(k[:10])
+    assert e.value.code == ft.FT_ERR_SIZE_MISMATCH
+
+
+def test_mul_u32_pair():
+    n = 1200
+    a, b = scalars(n, 105), scalars(n, 106)
+    k = np.random.default_rng(7).integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    k[:4] = [0, 1, 2**32 - 1, 2**31]
+    g1, g2 = ft.GeArray.from_scalars(a), ft.GeArray.from_scalars(b)
+    o1, o2 = O.Array.from_scalars(a), O.Array.from_scalars(b)
+    g1.mul_u32(k, g2)
+    o1.mul_u32(k)
+    o2.mul_u32(k)
+    same(g1, o1)
+    same(g2, o2)
+
+
+# ------------------------------------------------------------ group ops
+
+def test_add_sub_neg_and_aliasing():
+    n = 2000
+    a, b = scalars(n, 107), scalars(n, 108)
+    ga, gb = ft.GeArray.from_scalars(a), ft.GeArray.from_scalars(b)
+    oa, ob = O.Array.from_scalars(a), O.Array.from_scalars(b)
+    ga.add(gb)
+    oa.add(ob)
+    same(ga, oa)
+    ga.sub(gb)
+    oa.sub(ob)
+    same(ga, oa)
+    ga.add(ga)              # ft_add(arr, arr): doubling through the unified formula
+    oa.add(oa.copy())
+    same(ga, oa)
+    ga.neg()
+    oa.neg()
+    same(ga, oa)
+    z = ft.GeArray.init_scalar(n, None)
+    z.add(gb)
+    same(z, ob)
+
+
+def test_init_scalar_and_point():
+    z = ft.GeArray.init_scalar(16)
+    assert bytes(z.to_bytes_folded()) == IDENT * 16
+    k = scalars(1, 109)[0]
+    g = ft.GeArray.init_scalar(40, k)
+    want = O.scalarmult_base(bytes(k))
+    assert bytes(g.to_bytes_folded()) == want * 40
+    src = ft.GeArray.from_scalars(scalars(5, 110))
+    bc = ft.GeArray.init_point(33, src, 3)
+    assert bytes(bc.to_bytes_folded()) == bytes(src.to_bytes_folded()[96:128]) * 33
+    with pytest.raises(ft.FtError):
+        ft.GeArray.init_point(3, src, 5)
+
+
+# ------------------------------------------------------------ I/O
+
+def test_encode_decode_roundtrip_and_failures():
+    s = scalars(700, 111)
+    enc = O.Array.from_scalars(s).to_bytes()
+    g, bad = ft.GeArray.from_bytes_folded(enc)
+    assert bad == -1
+    assert np.array_equal(g.to_bytes_folded(), enc)
+    rng = np.random.default_rng(8)
+    junk = rng.integers(0, 256, size=600 * 32, dtype=np.uint8)
+    g, bad = ft.GeArray.from_bytes_folded(junk)
+    o, obad = O.Array.from_bytes(junk)
+    assert bad == obad and bad > 0
+    # points that decode must match the oracle's decoding
+    ge = g.to_bytes_folded().reshape(-1, 32)
+    oe = o.to_bytes().reshape(-1, 32)
+    ok = [i for i in range(600) if O.decode_roundtrip(bytes(junk[32 * i:32 * i + 32]))[0] == 0]
+    assert len(ok) > 100
+    for i in ok:
+        assert bytes(ge[i]) == bytes(oe[i])
+    with pytest.raises(ft.FtError) as e:
+        ft.GeArray.from_bytes_folded(junk[:33])
+    assert e.value.code == ft.FT_ERR_BAD_SIZE
+
+
+def test_affine_and_extended_roundtrip_and_validate():
+    s = scalars(300, 112)
+    g = ft.GeArray.from_scalars(s)
+    want = g.to_bytes_folded()
+    aff = g.to_bytes_affine()
+    for i in (0, 7, 299):
+        pt = edref.mul(int.from_bytes(bytes(s[i]), "little"), edref.G)
+        assert bytes(aff[64 * i:64 * i + 32]) == edref.fe_bytes(pt[0])
+        assert bytes(aff[64 * i + 32:64 * i + 64]) == edref.fe_bytes(pt[1])
+    ga = ft.GeArray.from_bytes_affine(aff)
+    assert np.array_equal(ga.to_bytes_folded(), want)
+    assert ga.validate() == -1
+    raw = g.to_bytes()
+    assert len(raw) == 300 * 160
+    gr = ft.GeArray.from_bytes(raw)
+    assert np.array_equal(gr.to_bytes_folded(), want)
+    bad = aff.copy()
+    bad[64 * 41 + 32] ^= 1           # y changed: off the curve
+    bad[64 * 7] ^= 4
+    assert ft.GeArray.from_bytes_affine(bad).validate() == 41
+
+
+# ------------------------------------------------------------ aggregation
+
+@pytest.mark.parametrize("n", [1, 9, 256, 257, 4097, 70000])
+def test_reduce(n):
+    s = scalars(n, 113 + n)
+    g = ft.GeArray.from_scalars(s)
+    g.reduce()
+    assert len(g) == 1
+    assert bytes(g.to_bytes_folded()) == bytes(O.Array.from_scalars(s).reduce())
+
+
+def test_reduce_run_tests_case():
+    g = ft.GeArray.from_small_scalars(np.arange(9, dtype=np.uint64))
+    g.reduce()
+    assert bytes(g.to_bytes_folded()) == O.scalarmult_base(bytes(sb(36)))
+
+
+@pytest.mark.parametrize("n", [1, 7, 16, 17, 300, 4500])
+def test_scan_and_prescan(n):
+    s = scalars(n, 200 + n)
+    for excl in (False, True):
+        g = ft.GeArray.from_scalars(s)
+        o = O.Array.from_scalars(s)
+        if excl:
+            g.prescan()
+        else:
+            g.scan()
+        o.scan(exclusive=excl)
+        same(g, o)
+
+
+def test_scan_run_tests_case():
+    g = ft.GeArray.from_small_scalars(np.arange(7, dtype=np.uint64))
+    g.scan()
+    want = b"".join(O.scalarmult_base(bytes(sb(v))) for v in [0, 1, 3, 6, 10, 15, 21])
+    assert bytes(g.to_bytes_folded()) == want
+
+
+def seg_lengths(rng, nseg):
+    lens = np.clip(np.round(rng.lognormal(np.log(16), 1.5, size=nseg)), 1, 6000).astype(np.int64)
+    lens[rng.integers(0, nseg, size=nseg // 10)] = 0
+    lens[:3] = [0, 0, 4100]
+    lens[-1] = 0
+    return lens
+
+
+def test_segmented_sum():
+    rng = np.random.default_rng(9)
+    lens = seg_lengths(rng, 400)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    s = scalars(int(off[-1]), 114)
+    g = ft.GeArray.from_scalars(s)
+    o = O.Array.from_scalars(s)
+    same(ft.segmented_sum(off, g), O.segmented_sum(off, o))
+
+
+def test_segmented_sum_chunk_aligned_and_single():
+    s = scalars(16 * 40, 115)
+    g, o = ft.GeArray.from_scalars(s), O.Array.from_scalars(s)
+    for lens in ([16] * 40, [640], [1] * 640, [0, 640, 0], [15, 17] * 20):
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        same(ft.segmented_sum(off, g), O.segmented_sum(off, o))
+
+
+def test_gather_sum():
+    rng = np.random.default_rng(10)
+    nsrc, nrows = 3000, 500
+    deg = rng.poisson(8, size=nrows)
+    deg[5] = 2000
+    deg[6] = 0
+    off = np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+    cols = rng.integers(0, nsrc, size=int(off[-1]), dtype=np.int32)
+    s = scalars(nsrc, 116)
+    g, o = ft.GeArray.from_scalars(s), O.Array.from_scalars(s)
+    same(ft.gather_sum(off, cols, g), O.segmented_sum(off, o, idx=cols))
+    with pytest.raises(ft.FtError) as e:
+        ft.gather_sum(off, np.full_like(cols, nsrc), g)
+    assert e.value.code == ft.FT_ERR_OUT_OF_BOUNDS
+
+
+def test_reduce_isum():
+    res = ft.GeArray.from_small_scalars(np.array([5, 8, 20, 30, 50], dtype=np.uint64))
+    src = ft.GeArray.from_small_scalars(np.array([10, 20, 30], dtype=np.uint64))
+    res.reduce_isum([2, 2, 3], src)
+    want = b"".join(O.scalarmult_base(bytes(sb(v))) for v in [5, 8, 50, 60, 50])
+    assert bytes(res.to_bytes_folded()) == want
+    rng = np.random.default_rng(11)
+    r, s = scalars(300, 117), scalars(2000, 118)
+    keys = rng.integers(0, 300, size=2000).astype(np.uint64)
+    g, o = ft.GeArray.from_scalars(r), O.Array.from_scalars(r)
+    g.reduce_isum(keys, ft.GeArray.from_scalars(s))
+    assert o.reduce_isum(keys, O.Array.from_scalars(s)) == 0
+    same(g, o)
+    with pytest.raises(ft.FtError):
+        g.reduce_isum(np.array([300] * 2000, dtype=np.uint64), ft.GeArray.from_scalars(s))
+
+
+# ------------------------------------------------------------ ElGamal
+
+@pytest.mark.parametrize("bits", [1, 8, 64])
+def test_encrypt_and_rerandomize(bits):
+    n = 1500
+    x = scalars(1, 119)
+    pk = ft.GeArray.from_scalars(x)
+    H = bytes(pk.to_bytes_folded())
+    key = ft.Key(pk, 0)
+    rng = np.random.default_rng(12 + bits)
+    hi = 2**bits
+    m = rng.integers(0, hi if bits < 64 else 2**63, size=n, dtype=np.uint64)
+    if bits == 64:
+        m[0] = 2**64 - 1
+    m[1] = 0
+    r = scalars(n, 120)
+    c1, c2 = ft.encrypt(key, m, r)
+    o1, o2 = O.encrypt(H, m, r)
+    same(c1, o1)
+    same(c2, o2)
+    rho = scalars(n, 121)
+    ft.rerandomize(c1, c2, key, rho)
+    O.rerandomize(o1, o2, H, rho)
+    same(c1, o1)
+    same(c2, o2)
+
+
+# ------------------------------------------------------------ data movement / predicates
+
+def test_subset_range_assign_delete():
+    s = scalars(100, 122)
+    g = ft.GeArray.from_scalars(s)
+    enc = g.to_bytes_folded().reshape(-1, 32)
+    idx = np.array([3, 7, 99, 0, 7], dtype=np.uint64)
+    assert np.array_equal(g.get_subset(idx).to_bytes_folded().reshape(-1, 32), enc[idx.astype(int)])
+    r = g.get_range(5, 50, 7)
+    assert len(r) == 7
+    assert np.array_equal(r.to_bytes_folded().reshape(-1, 32), enc[5:50:7])
+    g2 = ft.GeArray.from_scalars(s)
+    g2.assign_zero_to_subset(np.array([1, 2], dtype=np.uint64))
+    g2.assign_to_range(10, 13, 1, g.get_subset(np.array([50, 51, 52], dtype=np.uint64)))
+    e2 = g2.to_bytes_folded().reshape(-1, 32)
+    assert bytes(e2[1]) == IDENT and bytes(e2[2]) == IDENT
+    assert np.array_equal(e2[10:13], enc[50:53])
+    g2.delete_range(0, 100, 2)
+    assert np.array_equal(g2.to_bytes_folded().reshape(-1, 32)[6:], enc[13::2])
+    g3 = ft.GeArray.from_scalars(s)
+    g3.delete_subset(np.array([0, 5, 5, 99], dtype=np.uint64))
+    keep = [i for i in range(100) if i not in (0, 5, 99)]
+    assert np.array_equal(g3.to_bytes_folded().reshape(-1, 32), enc[keep])
+    g4 = ft.GeArray.from_scalars(s)
+    g4.set_length(120)
+    e4 = g4.to_bytes_folded().reshape(-1, 32)
+    assert np.array_equal(e4[:100], enc) and all(bytes(e4[i]) == IDENT for i in range(100, 120))
+    g4.set_length(30)
+    assert np.array_equal(g4.to_bytes_folded().reshape(-1, 32), enc[:30])
+
+
+def test_predicates():
+    a = ft.GeArray.from_small_scalars(np.array([3, 5, 8, 2], dtype=np.uint64))
+    b = ft.GeArray.from_small_scalars(np.arange(1, 8, dtype=np.uint64))
+    assert list(a.contains(b)) == [1, 1, 0, 1]          # run_tests.cc:157-170
+    f = ft.GeArray.from_small_scalars(np.array([5], dtype=np.uint64))
+    f2 = ft.GeArray.from_small_scalars(np.array([5], dtype=np.uint64))
+    assert list(f.equal(f2)) == [1]                    # run_tests.cc:140-152
+    f2.add(f)
+    assert list(f.equal(f2)) == [0] and list(f.not_equal(f2)) == [1]
+    z = ft.GeArray.from_small_scalars(np.array([0, 1, 0, 9], dtype=np.uint64))
+    assert list(z.index()) == [0, 1, 0, 1]
+    t = ft.GeArray.from_small_scalars(np.array([1, 2, 3], dtype=np.uint64))
+    e = ft.GeArray.from_small_scalars(np.array([7, 8, 9], dtype=np.uint64))
+    t.mux(np.array([1, 0, 5], dtype=np.int64), e)
+    assert bytes(t.to_bytes_folded()) == b"".join(O.scalarmult_base(bytes(sb(v))) for v in [1, 8, 3])
+
+
+# ------------------------------------------------------------ the small reference workload
+
+def test_config0_workload():
+    """Keypair, encrypt two arrays of 65,536 bits, add them, reduce: every
+    intermediate compared byte-exact with the oracle."""
+    n = 65536
+    x = scalars(1, 0xF7C00000, 1)
+    pk = ft.GeArray.from_scalars(x)
+    H = bytes(pk.to_bytes_folded())
+    key = ft.Key(pk)
+    rng = np.random.default_rng(0xF7C00000)
+    mA = rng.integers(0, 2, size=n, dtype=np.uint64)
+    mB = rng.integers(0, 2, size=n, dtype=np.uint64)
+    rA, rB = scalars(n, 0xF7C00000, 2), scalars(n, 0xF7C00000, 3)
+    a1, a2 = ft.encrypt(key, mA, rA, msg_bits=1)
+    b1, b2 = ft.encrypt(key, mB, rB, msg_bits=1)
+    oa1, oa2 = O.encrypt(H, mA, rA)
+    ob1, ob2 = O.encrypt(H, mB, rB)
+    for g, o in ((a1, oa1), (a2, oa2), (b1, ob1), (b2, ob2)):
+        same(g, o)
+    a1.add(b1)
+    a2.add(b2)
+    oa1.add(ob1)
+    oa2.add(ob2)
+    same(a1, oa1)
+    same(a2, oa2)
+    want1, want2 = bytes(oa1.reduce()), bytes(oa2.reduce())
+    a1.reduce()
+    a2.reduce()
+    assert bytes(a1.to_bytes_folded()) == want1
+    assert bytes(a2.to_bytes_folded()) == want2
+    # closed form: the sum decrypts to sum(m) (C2 - x C1 = (sum m) B)
+    xs = int.from_bytes(bytes(x[0]), "little")
+    msum = int(mA.sum() + mB.sum())
+    c1 = edref.decode(want1)
+    c2 = edref.decode(want2)
+    assert edref.add(c2, edref.neg(edref.mul(xs, c1))) == edref.mul(msum, edref.G)
