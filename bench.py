@@ -214,7 +214,7 @@ def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
     """The oracle on a bounded sample of the same step, all host cores."""
     from oracle import oracle as O
     nt = O.threads()
-    samp = max(256, min(n, 1 << 14))
+    samp = max(256, min(n, 1 << 19))
     t = {}
     t0 = time.time()
     O.Array.from_scalars(k_fixed[:samp], nt=nt)

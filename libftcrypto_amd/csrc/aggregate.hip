@@ -10,11 +10,12 @@
 //
 // Segmented sum: level 1 walks the elements; a segment that lies inside one
 // chunk is written to the output directly, one that crosses a chunk edge
-// leaves a partial sum in the chunk's head slot (segment began before the
-// chunk) or tail slot (segment continues after it).  The slots form the
-// element array of level 2, where segment s covers slots
-// [2 c_first + 1, 2 c_last + 1) -- a range computed from the offsets alone --
-// and the same kernel runs again, until one chunk holds everything.
+// leaves a partial sum and its segment id in the chunk's head slot (segment
+// began before the chunk) or tail slot (segment continues after it).  The
+// slots form the sorted, keyed element array of level 2, where segment s
+// covers slots [2 c_first + 1, 2 c_last + 1) -- a range computed from the
+// offsets alone, which decides whether a run of s is complete -- and level 2
+// runs the same chunked walk over the records, until one chunk holds all.
 #include "kernels.h"
 
 namespace ft {
@@ -99,29 +100,35 @@ __device__ __forceinline__ void seg_range(const int64_t *off, size_t s, int leve
   }
 }
 
-// first segment whose range ends after position p (ranges are sorted)
-__device__ __forceinline__ size_t seg_find(const int64_t *off, size_t nseg, int level, int64_t p) {
-  size_t a = 0, b = nseg;
+// first segment in [a, nseg) whose level-1 range ends after position p
+__device__ __forceinline__ size_t seg_find1(const int64_t *off, size_t a, size_t nseg, int64_t p) {
+  size_t b = nseg;
   while (a < b) {
     const size_t mid = (a + b) >> 1;
-    int64_t lo, hi;
-    seg_range(off, mid, level, lo, hi);
-    if (hi <= p) a = mid + 1;
+    if (off[mid + 1] <= p) a = mid + 1;
     else b = mid;
   }
   return a;
 }
 
-__device__ __forceinline__ void seg_flush(PtView out, PtView rec, size_t s, int64_t lo, int64_t hi,
-                                          int64_t clo, int64_t chi, size_t t, const ge_p3 &acc) {
-  if (lo >= clo && hi <= chi) store_p3(out, s, acc);      // complete in this chunk
-  else if (lo < clo) store_p3(rec, 2 * t, acc);           // began before the chunk
-  else store_p3(rec, 2 * t + 1, acc);                     // continues after it
+__device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys, size_t s, int64_t lo,
+                                          int64_t hi, int64_t clo, int64_t chi, size_t t, const ge_p3 &acc) {
+  if (lo >= clo && hi <= chi) {
+    store_p3(out, s, acc);                      // complete in this chunk
+  } else if (lo < clo) {
+    store_p3(rec, 2 * t, acc);                  // began before the chunk
+    keys[2 * t] = (int64_t)s;
+  } else {
+    store_p3(rec, 2 * t + 1, acc);              // continues after it
+    keys[2 * t + 1] = (int64_t)s;
+  }
 }
 
-__global__ void __launch_bounds__(ATPB) k_seg_level(PtView out, PtView rec, PtView elems, size_t n,
-                                                    const int64_t *off, size_t nseg,
-                                                    const int32_t *gather, int level) {
+// Level 1: lane t walks elements [16 t, 16 t + 16) of the input (through the
+// gather indices when given), segment boundaries from the offsets.
+__global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
+                                                     size_t n, const int64_t *off, size_t nseg,
+                                                     const int32_t *gather) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
@@ -130,31 +137,79 @@ __global__ void __launch_bounds__(ATPB) k_seg_level(PtView out, PtView rec, PtVi
   if (rec.w) {
     store_identity(rec, 2 * t);
     store_identity(rec, 2 * t + 1);
+    keys[2 * t] = -1;
+    keys[2 * t + 1] = -1;
   }
-  size_t s = seg_find(off, nseg, level, clo);
+  size_t s = seg_find1(off, 0, nseg, clo);
   int64_t lo = 0, hi = 0;
-  if (s < nseg) seg_range(off, s, level, lo, hi);
+  if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
   ge_p3 acc, x;
   ge_p3_0(acc);
   bool touched = false;
 #pragma unroll 1
   for (int64_t i = clo; i < cend; ++i) {
-    while (s < nseg && hi <= i) {
-      if (touched) seg_flush(out, rec, s, lo, hi, clo, chi, t, acc);
+    if (s < nseg && hi <= i) {
+      if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
       touched = false;
-      ge_p3_0(acc);
-      ++s;
-      if (s < nseg) seg_range(off, s, level, lo, hi);
+      // next segment containing i: a few linear steps, else binary search
+      int steps = 0;
+      do { ++s; ++steps; } while (s < nseg && off[s + 1] <= i && steps < 4);
+      if (s < nseg && off[s + 1] <= i) s = seg_find1(off, s, nseg, i);
+      if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
     }
     if (s < nseg && lo <= i) {
       const size_t j = gather ? (size_t)gather[i] : (size_t)i;
-      load_p3(x, elems, j);
+      load_p3(x, src, j);
       if (touched) ge_p3_add(acc, acc, x);
       else acc = x;
       touched = true;
     }
   }
-  if (touched) seg_flush(out, rec, s, lo, hi, clo, chi, t, acc);
+  if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
+}
+
+// Level L >= 2: lane t walks records [16 t, 16 t + 16) of level L-1, whose
+// segment keys are sorted (-1 marks an unused slot holding the identity);
+// a run is complete here when the segment's level-L range lies in the chunk.
+__global__ void __launch_bounds__(ATPB) k_seg_levelN(PtView out, PtView rec, int64_t *keys, PtView in,
+                                                     const int64_t *in_keys, size_t n, const int64_t *off,
+                                                     int level) {
+  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int64_t clo = (int64_t)(t << CH);
+  if (clo >= (int64_t)n) return;
+  const int64_t chi = clo + CHN;
+  const int64_t cend = chi < (int64_t)n ? chi : (int64_t)n;
+  if (rec.w) {
+    store_identity(rec, 2 * t);
+    store_identity(rec, 2 * t + 1);
+    keys[2 * t] = -1;
+    keys[2 * t + 1] = -1;
+  }
+  int64_t cur = -1;
+  ge_p3 acc, x;
+  ge_p3_0(acc);
+#pragma unroll 1
+  for (int64_t i = clo; i < cend; ++i) {
+    const int64_t k = in_keys[i];
+    if (k < 0) continue;
+    if (k != cur) {
+      if (cur >= 0) {
+        int64_t lo, hi;
+        seg_range(off, (size_t)cur, level, lo, hi);
+        seg_flush(out, rec, keys, (size_t)cur, lo, hi, clo, chi, t, acc);
+      }
+      cur = k;
+      load_p3(acc, in, (size_t)i);
+    } else {
+      load_p3(x, in, (size_t)i);
+      ge_p3_add(acc, acc, x);
+    }
+  }
+  if (cur >= 0) {
+    int64_t lo, hi;
+    seg_range(off, (size_t)cur, level, lo, hi);
+    seg_flush(out, rec, keys, (size_t)cur, lo, hi, clo, chi, t, acc);
+  }
 }
 
 __global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *off, size_t nseg) {
@@ -162,12 +217,16 @@ __global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *o
   if (s < nseg && off[s] == off[s + 1]) store_identity(out, s);
 }
 
-hipError_t launch_seg_level(PtView out, PtView rec, PtView elems, size_t n, const int64_t *off,
-                            size_t nseg, const int32_t *gather, int level, hipStream_t s) {
+hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
+                            size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
+                            hipStream_t s) {
   const size_t T = (n + CHN - 1) >> CH;
   if (!T || !nseg) return hipSuccess;
-  hipLaunchKernelGGL(k_seg_level, dim3((unsigned)((T + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, out, rec,
-                     elems, n, off, nseg, gather, level);
+  const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
+  if (level == 1)
+    hipLaunchKernelGGL(k_seg_level1, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather);
+  else
+    hipLaunchKernelGGL(k_seg_levelN, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level);
   return hipGetLastError();
 }
 

@@ -28,8 +28,8 @@ constexpr int SEG_CHUNK_LOG2 = 4;                     // elements per lane per l
 size_t scan_tiles(size_t n);
 hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s);
 hipError_t launch_scan_apply(PtView v, size_t n, PtView tile_prefix, bool exclusive, hipStream_t s);
-hipError_t launch_seg_level(PtView out, PtView rec_out, PtView elems, size_t n_elems,
-                            const int64_t *off, size_t nseg, const int32_t *gather, int level,
+hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
+                            size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
                             hipStream_t s);
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s);
 
