@@ -6,9 +6,12 @@ N = 2^22 of each kind, every input resident in HBM before timing starts:
   fixed-base      k_i * B                       (N scalar-mults)
   variable-base   k_i * P_i, P_i = s_i * B      (N scalar-mults)
   ct x constant   (a_i C1_i, a_i C2_i), a_i u32 (2N scalar-mults)
-`value` = scalar-mults per second over the whole step.  Point-add rates of
-the aggregation kernels (elementwise ciphertext add, reduce, scan, CSR
-segmented sum) are reported under "detail".
+`value` = scalar-mults per second over the whole step.  The other
+configurations are measured at full size under "detail.configs": config 0
+(encrypt 2 x 65,536 bits, add, reduce), config 2 (reduce 2^26 ciphertexts,
+inclusive scan 2^24, CSR segmented sum 2^24 in 2^19 lognormal segments) and
+config 3 (sparse gather-sum over a 2^22-node / 2^25-edge power-law graph plus
+rerandomisation, replayed as one hipGraph).
 
 Timing uses HIP events recorded on the stream the kernels run on.  The
 roofline object is for the kernel with the largest share of the step.  The
@@ -115,16 +118,6 @@ def run(args):
     x = ft.random_scalars(1, seed, 5)
     key = ft.Key(ft.GeArray.from_scalars(x))
     c1, c2 = ft.encrypt(key, m_ct, r_ct, msg_bits=1)          # setup ciphertexts
-    # aggregation inputs: an independent ciphertext array and CSR segments
-    d1, d2 = ft.encrypt(key, m_ct[::-1].copy(), ft.random_scalars(n, seed, 6), msg_bits=1)
-    lens = np.clip(np.round(rng.lognormal(np.log(16), 1.5, size=max(1, n // 32))), 1, 65536).astype(np.int64)
-    lens = lens[: np.searchsorted(np.cumsum(lens), n)]
-    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
-    d_off = hip.upload(off)
-    nseg, nel = len(lens), int(off[-1])
-    seg_out = ft.GeArray.alloc(nseg)
-    seg_scr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(nel))
-    red_out = ft.GeArray.alloc(1)
     ft.check(L.ftx_synchronize(None))
 
     ops = {
@@ -133,13 +126,6 @@ def run(args):
         "mul_u32": lambda: ft.check(L.ftx_mul_u32_dev(c1.h, c2.h, d_a, stream)),
     }
     mults = {"fixed_base": n, "variable_base": n, "mul_u32": 2 * n}
-    agg = {
-        "ct_add": (lambda: (ft.check(L.ftx_add_dev(c1.h, d1.h, stream)),
-                            ft.check(L.ftx_add_dev(c2.h, d2.h, stream))), 2 * n, 3 * POINT_BYTES),
-        "reduce": (lambda: ft.check(L.ftx_reduce_dev(red_out.h, d1.h, stream)), n - 1, POINT_BYTES),
-        "segsum": (lambda: ft.check(L.ftx_segmented_sum_dev(seg_out.h, d_off, nseg, nel, None, d1.h,
-                                                            seg_scr.h, stream)), nel - nseg, None),
-    }
 
     ev = [hip.event() for _ in range(2)]
 
@@ -178,19 +164,6 @@ def run(args):
         ach = rate * PRODUCTS[k]
         detail[k] = {"ms": round(per[k], 4), "scalar_mults_per_s": rate,
                      "mul32_per_s": ach, "valu_frac": ach / PEAK_MUL32}
-    for k, (f, units, bpu) in agg.items():
-        f()
-        ft.check(L.ftx_synchronize(None))
-        ms = timed(f, 3)
-        rate = units / (ms * 1e-3)
-        d = {"ms": round(ms, 4), "point_adds_per_s": rate,
-             "valu_frac": rate * PRODUCTS["point_add"] / PEAK_MUL32, "n": units}
-        if bpu:
-            d["hbm_GBps"] = rate * bpu / 1e9
-            d["hbm_frac"] = rate * bpu / HBM_PEAK
-        detail[k] = d
-        log("[bench] %-14s %.3f ms  %.3e point-adds/s" % (k, ms, rate))
-
     dom = max(per, key=lambda k: per[k])
     ach = mults[dom] / (per[dom] * 1e-3) * PRODUCTS[dom]
     roofline = {"kernel": dom, "bound": "valu", "achieved": ach / 1e12, "peak": PEAK_MUL32 / 1e12,
@@ -199,6 +172,13 @@ def run(args):
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, k_fixed, k_var, s_var, a_u32)
+    del out_fixed, pts, c1, c2
+    configs = {}
+    if not args.headline_only:
+        configs["cfg0_encrypt_add_reduce_n65536"] = bench_cfg0(ft, hip, key, seed)
+        configs["cfg2_aggregation"] = bench_cfg2(ft, hip, key, seed, args)
+        configs["cfg3_sparse_step_graph"] = bench_cfg3(ft, hip, key, seed, args)
+    detail["configs"] = configs
     return {
         "metric": "scalar-mults/s (fixed-base, variable-base, ciphertext x u32) on 1 MI355X",
         "value": value, "unit": "scalar-mults/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -208,6 +188,126 @@ def run(args):
                    "n": n, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "detail": detail,
     }
+
+
+class Timer:
+    def __init__(self, hip, stream):
+        self.hip, self.stream = hip, stream
+        self.ev = [hip.event(), hip.event()]
+
+    def __call__(self, fn, reps=3):
+        fn()
+        self.hip.L.hipStreamSynchronize(self.stream)
+        self.hip.L.hipEventRecord(self.ev[0], self.stream)
+        for _ in range(reps):
+            fn()
+        self.hip.L.hipEventRecord(self.ev[1], self.stream)
+        return self.hip.elapsed_ms(self.ev[0], self.ev[1]) / reps
+
+
+def adds_entry(ms, adds, bytes_per_add=None):
+    rate = adds / (ms * 1e-3)
+    d = {"ms": round(ms, 4), "point_adds": adds, "point_adds_per_s": rate,
+         "valu_frac": rate * PRODUCTS["point_add"] / PEAK_MUL32}
+    if bytes_per_add:
+        d["hbm_GBps"] = rate * bytes_per_add / 1e9
+        d["hbm_frac"] = rate * bytes_per_add / HBM_PEAK
+    return d
+
+
+def bench_cfg0(ft, hip, key, seed):
+    """Encrypt 2 x 65,536 bits, add the two arrays, reduce both halves."""
+    L = ft.lib()
+    stream = L.ftx_default_stream()
+    T = Timer(hip, stream)
+    n = 65536
+    rng = np.random.default_rng(seed)
+    m = hip.upload(rng.integers(0, 2, size=n, dtype=np.uint64))
+    r = hip.upload(ft.random_scalars(n, seed, 10))
+    a1, a2, b1, b2 = (ft.GeArray.alloc(n) for _ in range(4))
+    out = ft.GeArray.alloc(1)
+    enc_ms = T(lambda: (ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, stream)),
+                        ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, stream))))
+    add_ms = T(lambda: (ft.check(L.ftx_add_dev(a1.h, b1.h, stream)), ft.check(L.ftx_add_dev(a2.h, b2.h, stream))))
+    red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, a1.h, stream)), ft.check(L.ftx_reduce_dev(out.h, a2.h, stream))))
+    return {"encrypt": {"ms": round(enc_ms, 4), "ciphertexts_per_s": 2 * n / (enc_ms * 1e-3),
+                        "valu_frac": 2 * n * (2 * PRODUCTS["fixed_base"]) / (enc_ms * 1e-3) / PEAK_MUL32},
+            "ct_add": adds_entry(add_ms, 2 * n, 3 * POINT_BYTES),
+            "reduce": adds_entry(red_ms, 2 * (n - 1), POINT_BYTES),
+            "step_ms": round(enc_ms + add_ms + red_ms, 4)}
+
+
+def bench_cfg2(ft, hip, key, seed, args):
+    """reduce 2^26 ct, inclusive scan 2^24 ct, segmented sum 2^24 ct / 2^19 segments."""
+    from libftcrypto_amd import workloads as W
+    L = ft.lib()
+    stream = L.ftx_default_stream()
+    T = Timer(hip, stream)
+    rng = np.random.default_rng(seed + 2)
+    nr, ns, nseg = 1 << args.cfg2_reduce_log2, 1 << args.cfg2_log2, 1 << (args.cfg2_log2 - 5)
+    res = {}
+    log("[bench] cfg2: reduce over 2^%d ciphertexts" % args.cfg2_reduce_log2)
+    c1, c2 = ft.GeArray.alloc(nr), ft.GeArray.alloc(nr)
+    m = hip.upload(rng.integers(0, 256, size=nr, dtype=np.uint64))
+    r = hip.upload(ft.random_scalars(nr, seed, 20))
+    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
+    out = ft.GeArray.alloc(1)
+    ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, c1.h, stream)), ft.check(L.ftx_reduce_dev(out.h, c2.h, stream))))
+    res["reduce"] = adds_entry(ms, 2 * (nr - 1), POINT_BYTES)
+    hip.L.hipFree(m); hip.L.hipFree(r)
+    del c1, c2
+    log("[bench] cfg2: scan and segmented sum over 2^%d ciphertexts" % args.cfg2_log2)
+    c1, c2 = ft.GeArray.alloc(ns), ft.GeArray.alloc(ns)
+    m = hip.upload(rng.integers(0, 256, size=ns, dtype=np.uint64))
+    r = hip.upload(ft.random_scalars(ns, seed, 21))
+    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
+    off = W.lognormal_segments(nseg, ns, seed)
+    d_off = hip.upload(off)
+    s1, s2 = ft.GeArray.alloc(nseg), ft.GeArray.alloc(nseg)
+    scr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ns))
+    ms = T(lambda: (ft.check(L.ftx_segmented_sum_dev(s1.h, d_off, nseg, ns, None, c1.h, scr.h, stream)),
+                    ft.check(L.ftx_segmented_sum_dev(s2.h, d_off, nseg, ns, None, c2.h, scr.h, stream))))
+    res["segmented_sum"] = adds_entry(ms, 2 * (ns - nseg), (2 * ns * POINT_BYTES + (nseg + 1) * 8 + 2 * nseg * POINT_BYTES) / (2 * (ns - nseg)))
+    res["segmented_sum"]["segments"] = nseg
+    ms = T(lambda: (ft.check(L.ftx_scan_dev(c1.h, 0, stream)), ft.check(L.ftx_scan_dev(c2.h, 0, stream))))
+    res["scan"] = adds_entry(ms, 2 * (ns - 1), 2 * POINT_BYTES)
+    hip.L.hipFree(m); hip.L.hipFree(r); hip.L.hipFree(d_off)
+    return res
+
+
+def bench_cfg3(ft, hip, key, seed, args):
+    """Gather-sum of in-neighbours + rerandomisation, one hipGraph per step."""
+    from libftcrypto_amd import workloads as W
+    L = ft.lib()
+    nn, ne = 1 << args.cfg3_nodes_log2, 1 << args.cfg3_edges_log2
+    log("[bench] cfg3: graph with 2^%d nodes, 2^%d edges" % (args.cfg3_nodes_log2, args.cfg3_edges_log2))
+    row_off, cols, deg = W.power_law_graph(nn, ne, seed + 3)
+    rng = np.random.default_rng(seed + 3)
+    st = ft.Stream()
+    T = Timer(hip, st.h.value)
+    m = hip.upload((rng.random(nn) < 0.01).astype(np.uint64))
+    r = hip.upload(ft.random_scalars(nn, seed, 30))
+    rho = hip.upload(ft.random_scalars(nn, seed, 31))
+    d_off, d_cols = hip.upload(row_off), hip.upload(cols)
+    c1, c2, o1, o2 = (ft.GeArray.alloc(nn) for _ in range(4))
+    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 1, st.h))
+    scr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ne))
+
+    def step():
+        ft.check(L.ftx_segmented_sum_dev(o1.h, d_off, nn, ne, d_cols, c1.h, scr.h, st.h))
+        ft.check(L.ftx_segmented_sum_dev(o2.h, d_off, nn, ne, d_cols, c2.h, scr.h, st.h))
+        ft.check(L.ftx_rerandomize_dev(o1.h, o2.h, key.h, rho, st.h))
+    st.synchronize()
+    with ft.Graph(st) as g:
+        step()
+    ms = T(lambda: g.launch(), reps=3)
+    eager_ms = T(step, reps=3)
+    nonempty = int((np.diff(row_off) > 0).sum())
+    adds = 2 * (ne - nonempty) + 2 * nn
+    mults = 2 * nn
+    return {"graph_ms_per_step": round(ms, 4), "eager_ms_per_step": round(eager_ms, 4),
+            "point_adds": adds, "fixed_base_mults": mults,
+            "valu_frac": (adds * PRODUCTS["point_add"] + mults * PRODUCTS["fixed_base"]) / (ms * 1e-3) / PEAK_MUL32}
 
 
 def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
@@ -240,6 +340,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=22, help="log2 of the array length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the configurations other than the N=2^22 scalar mults")
+    ap.add_argument("--cfg2-reduce-log2", type=int, default=26)
+    ap.add_argument("--cfg2-log2", type=int, default=24)
+    ap.add_argument("--cfg3-nodes-log2", type=int, default=22)
+    ap.add_argument("--cfg3-edges-log2", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this library's hot path runs on a single device "

@@ -85,6 +85,24 @@ ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, void *stream);
 /* 32-byte canonical compressed encodings into device memory d_out */
 ft_error ftx_encode_dev(void *d_out, const_ft_ge25519_array array, void *stream);
 
+/* ---------------------------------------------------------- streams, graphs */
+
+/* Opaque instantiated HIP graph of captured library calls. */
+struct ftx_graph_s;
+typedef struct ftx_graph_s *ftx_graph;
+
+/* A non-blocking stream at default priority. */
+ft_error ftx_stream_create(void **stream);
+void ftx_stream_destroy(void *stream);
+/* Capture the _dev calls issued on `stream` (not the library stream) between
+ * begin and end into a graph.  Capturable: ftx_from_scalars_dev, ftx_mul_dev,
+ * ftx_mul_u32_dev, ftx_add_dev, ftx_encrypt_dev, ftx_rerandomize_dev,
+ * ftx_encode_dev, and ftx_segmented_sum_dev with caller-provided scratch. */
+ft_error ftx_capture_begin(void *stream);
+ft_error ftx_capture_end(void *stream, ftx_graph *graph);
+ft_error ftx_graph_launch(ftx_graph graph, void *stream);
+void ftx_graph_free(ftx_graph graph);
+
 /* ---------------------------------------------------------- inputs */
 
 /* n scalars uniform mod l: 64 bytes of SplitMix64(seed, stream) output per

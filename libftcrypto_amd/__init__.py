@@ -100,6 +100,12 @@ _SIGS = {
     "ftx_reduce_dev": (i32, [vp, vp, vp]),
     "ftx_scan_dev": (i32, [vp, i32, vp]),
     "ftx_encode_dev": (i32, [vp, vp, vp]),
+    "ftx_stream_create": (i32, [vp]),
+    "ftx_stream_destroy": (None, [vp]),
+    "ftx_capture_begin": (i32, [vp]),
+    "ftx_capture_end": (i32, [vp, vp]),
+    "ftx_graph_launch": (i32, [vp, vp]),
+    "ftx_graph_free": (None, [vp]),
     "ftx_random_scalars": (None, [vp, sz, u64, u64]),
     "ftx_scalar_reduce64": (None, [vp, vp, sz]),
 }
@@ -395,3 +401,49 @@ def scalar_reduce64(wide):
 
 def error_str(code):
     return lib().ft_error_str(code).decode()
+
+
+class Stream:
+    """A HIP stream owned by the library (default priority)."""
+
+    def __init__(self):
+        init()
+        s = ctypes.c_void_p()
+        check(lib().ftx_stream_create(ctypes.byref(s)), "ftx_stream_create")
+        self.h = s
+
+    def synchronize(self):
+        check(lib().ftx_synchronize(self.h), "ftx_synchronize")
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().ftx_stream_destroy(self.h)
+            self.h = None
+
+
+class Graph:
+    """Capture of library _dev calls issued on `stream` (context manager)."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.h = None
+
+    def __enter__(self):
+        check(lib().ftx_capture_begin(self.stream.h), "ftx_capture_begin")
+        return self
+
+    def __exit__(self, et, ev, tb):
+        g = ctypes.c_void_p()
+        rc = lib().ftx_capture_end(self.stream.h, ctypes.byref(g))
+        if et is None:
+            check(rc, "ftx_capture_end")
+            self.h = g
+        return False
+
+    def launch(self, stream=None):
+        check(lib().ftx_graph_launch(self.h, (stream or self.stream).h), "ftx_graph_launch")
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().ftx_graph_free(self.h)
+            self.h = None

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <thread>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -30,6 +32,11 @@ struct ft_ge25519_array_s {
 
 struct ftx_key_s {
   uint32_t *table;  // device comb table of H
+};
+
+struct ftx_graph_s {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
 };
 
 namespace {
@@ -900,6 +907,50 @@ ft_error ftx_add_dev(ft_ge25519_array res, const_ft_ge25519_array summand, void 
   return FT_ERR_NO_ERROR;
 }
 
+// ------------------------------------------------------------ streams and graphs
+
+ft_error ftx_stream_create(void **stream) {
+  hipStream_t st;
+  CK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));   // default priority
+  *stream = st;
+  return FT_ERR_NO_ERROR;
+}
+
+void ftx_stream_destroy(void *stream) {
+  if (stream) (void)hipStreamDestroy((hipStream_t)stream);
+}
+
+ft_error ftx_capture_begin(void *stream) {
+  if (!stream) return FT_ERR_BAD_SIZE;            // the library stream is never captured
+  CK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_capture_end(void *stream, ftx_graph *graph) {
+  auto *g = new (std::nothrow) ftx_graph_s;
+  if (!g) return FT_ERR_OUT_OF_MEMORY;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    delete g;
+    return herr(e);
+  }
+  *graph = g;
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_graph_launch(ftx_graph graph, void *stream) {
+  CK(hipGraphLaunch(graph->exec, S(stream)));
+  return FT_ERR_NO_ERROR;
+}
+
+void ftx_graph_free(ftx_graph graph) {
+  if (!graph) return;
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  delete graph;
+}
+
 size_t ftx_segsum_scratch_points(size_t n_elems) { return seg_scratch_points(n_elems); }
 
 ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg, size_t n_elems,
@@ -970,31 +1021,45 @@ ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, void *stream) {
 
 // ------------------------------------------------------------ inputs (host)
 
+static void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_t)> &fn) {
+  unsigned hw = std::thread::hardware_concurrency();
+  size_t nt = std::min<size_t>(hw ? std::min(hw, 16u) : 1, (n + grain - 1) / grain);
+  if (nt <= 1) { fn(0, n); return; }
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { fn(n * t / nt, n * (t + 1) / nt); });
+  for (auto &x : th) x.join();
+}
+
 void ftx_scalar_reduce64(uint8_t *out, const uint8_t *in, size_t n) {
   const Barrett &b = barrett();
-  for (size_t i = 0; i < n; ++i) {
-    uint32_t x[16], r[8];
-    for (int k = 0; k < 16; ++k)
-      x[k] = (uint32_t)in[64 * i + 4 * k] | ((uint32_t)in[64 * i + 4 * k + 1] << 8) |
-             ((uint32_t)in[64 * i + 4 * k + 2] << 16) | ((uint32_t)in[64 * i + 4 * k + 3] << 24);
-    b.reduce(r, x);
-    words_to_bytes(out + 32 * i, r, 8);
-  }
+  parallel_for(n, 4096, [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+      uint32_t x[16], r[8];
+      for (int k = 0; k < 16; ++k)
+        x[k] = (uint32_t)in[64 * i + 4 * k] | ((uint32_t)in[64 * i + 4 * k + 1] << 8) |
+               ((uint32_t)in[64 * i + 4 * k + 2] << 16) | ((uint32_t)in[64 * i + 4 * k + 3] << 24);
+      b.reduce(r, x);
+      words_to_bytes(out + 32 * i, r, 8);
+    }
+  });
 }
 
 void ftx_random_scalars(uint8_t *out, size_t n, uint64_t seed, uint64_t stream) {
   const Barrett &b = barrett();
   const uint64_t base = stream << 40;
-  for (size_t i = 0; i < n; ++i) {
-    uint32_t x[16], r[8];
-    for (int j = 0; j < 8; ++j) {
-      const uint64_t z = splitmix_at(seed, base + 8 * (uint64_t)i + (uint64_t)j);
-      x[2 * j] = (uint32_t)z;
-      x[2 * j + 1] = (uint32_t)(z >> 32);
+  parallel_for(n, 4096, [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+      uint32_t x[16], r[8];
+      for (int j = 0; j < 8; ++j) {
+        const uint64_t z = splitmix_at(seed, base + 8 * (uint64_t)i + (uint64_t)j);
+        x[2 * j] = (uint32_t)z;
+        x[2 * j + 1] = (uint32_t)(z >> 32);
+      }
+      b.reduce(r, x);
+      words_to_bytes(out + 32 * i, r, 8);
     }
-    b.reduce(r, x);
-    words_to_bytes(out + 32 * i, r, 8);
-  }
+  });
 }
 
 }  // extern "C"
