@@ -178,6 +178,7 @@ def run(args):
         configs["cfg0_encrypt_add_reduce_n65536"] = bench_cfg0(ft, hip, key, seed)
         configs["cfg2_aggregation"] = bench_cfg2(ft, hip, key, seed, args)
         configs["cfg3_sparse_step_graph"] = bench_cfg3(ft, hip, key, seed, args)
+        configs["cfg4_serving"] = bench_cfg4(ft, key, seed, args)
     detail["configs"] = configs
     return {
         "metric": "scalar-mults/s (fixed-base, variable-base, ciphertext x u32) on 1 MI355X",
@@ -310,6 +311,40 @@ def bench_cfg3(ft, hip, key, seed, args):
             "valu_frac": (adds * PRODUCTS["point_add"] + mults * PRODUCTS["fixed_base"]) / (ms * 1e-3) / PEAK_MUL32}
 
 
+def bench_cfg4(ft, key, seed, args):
+    """Mixed request stream through the serving loop vs plain one-shot kernels."""
+    from libftcrypto_amd import serving as SV
+    log("[bench] cfg4: building a %d-request trace" % args.cfg4_requests)
+    ops, size_of = SV.make_trace(args.cfg4_requests, seed + 4, max_live=64)
+    pool, ops = SV.pin_inputs(ops)
+    n_req = sum(1 for op in ops if op[0] != SV.FREE)
+    elems = SV.trace_elements(ops)
+    t1, vec = SV.run_oneshot(key, ops, size_of)
+    del vec
+    log("[bench] cfg4: one-shot %.3f s; served burst" % t1)
+    srv = SV.Server(key, budget_bytes=args.cfg4_budget_gib << 30, n_streams=args.cfg4_streams, borrow=True)
+    t2, lat_b, _ = SV.run_served(srv, ops)
+    st = srv.stats()
+    srv.close()
+    cap = n_req / t1
+    rate = args.cfg4_load * cap
+    log("[bench] cfg4: served burst %.3f s; Poisson at %.0f req/s" % (t2, rate))
+    arr = SV.poisson_arrivals(ops, rate, seed + 44)
+    srv = SV.Server(key, budget_bytes=args.cfg4_budget_gib << 30, n_streams=args.cfg4_streams, borrow=True)
+    t3, lat, _ = SV.run_served(srv, ops, arrivals=arr)
+    srv.close()
+    return {"requests": n_req, "ciphertexts": elems, "mix": "40% encrypt, 30% add, 15% x u32, 15% segsum",
+            "sizes": "lognormal(ln 2^14, 2) in [2^8, 2^22], power-of-two rounded",
+            "oneshot_s": round(t1, 4), "oneshot_req_per_s": cap,
+            "served_saturation_s": round(t2, 4), "served_vs_oneshot_throughput": t1 / t2,
+            "batches_at_saturation": st["batches"],
+            "fixed_load": {"offered_req_per_s": rate, "load_vs_oneshot": args.cfg4_load,
+                           "wall_s": round(t3, 4), "achieved_req_per_s": n_req / t3,
+                           "p50_ms": float(np.percentile(lat, 50)) / 1e3, "p99_ms": float(np.percentile(lat, 99)) / 1e3,
+                           "max_ms": float(lat.max()) / 1e3},
+            "budget_gib": args.cfg4_budget_gib, "streams": args.cfg4_streams}
+
+
 def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
     """The oracle on a bounded sample of the same step, all host cores."""
     from oracle import oracle as O
@@ -345,6 +380,10 @@ def main():
     ap.add_argument("--cfg2-log2", type=int, default=24)
     ap.add_argument("--cfg3-nodes-log2", type=int, default=22)
     ap.add_argument("--cfg3-edges-log2", type=int, default=25)
+    ap.add_argument("--cfg4-requests", type=int, default=1000)
+    ap.add_argument("--cfg4-load", type=float, default=0.7)
+    ap.add_argument("--cfg4-streams", type=int, default=4)
+    ap.add_argument("--cfg4-budget-gib", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this library's hot path runs on a single device "

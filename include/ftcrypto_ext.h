@@ -103,6 +103,40 @@ ft_error ftx_capture_end(void *stream, ftx_graph *graph);
 ft_error ftx_graph_launch(ftx_graph graph, void *stream);
 void ftx_graph_free(ftx_graph graph);
 
+/* ---------------------------------------------------------- serving */
+
+/* A request loop over ciphertext vectors resident in HBM (one public key).
+ * Vectors are named by uint64 handles.  Requests are asynchronous: each
+ * returns a request id; ftx_server_wait blocks until it completed and gives
+ * its latency (submission to observed completion).  Queued requests of one
+ * kind are batched into one kernel launch, batches run on `n_streams`
+ * concurrent streams, and requests that create vectors are admitted only when
+ * the `hbm_budget_bytes` arena can hold them.  Requests on one vector execute
+ * in submission order. */
+struct ftx_server_s;
+typedef struct ftx_server_s *ftx_server;
+
+/* flags: FTX_SERVER_BORROW_INPUTS -- host inputs passed to the submit calls
+ * are read when their batch launches (not copied at submission) and must stay
+ * valid and unchanged until the request completes; pinned (hipHostMalloc)
+ * inputs are then transferred by DMA without a host copy. */
+#define FTX_SERVER_BORROW_INPUTS 1
+ft_error ftx_server_create(ftx_server *server, ftx_key key, size_t hbm_budget_bytes, int n_streams,
+                           size_t max_batch_elems, int flags);
+/* waits for every submitted request, then frees the server */
+void ftx_server_destroy(ftx_server server);
+ft_error ftx_server_encrypt(ftx_server server, const uint64_t *m, const uint8_t *r, size_t n, int msg_bits,
+                            uint64_t *req, uint64_t *handle);
+ft_error ftx_server_add(ftx_server server, uint64_t dst, uint64_t src, uint64_t *req);
+ft_error ftx_server_mul_u32(ftx_server server, uint64_t handle, const uint32_t *k, size_t n, uint64_t *req);
+ft_error ftx_server_segsum(ftx_server server, uint64_t handle, const int64_t *offsets, size_t n_seg,
+                           uint64_t *req, uint64_t *out_handle);
+ft_error ftx_server_wait(ftx_server server, uint64_t req, double *latency_us);
+ft_error ftx_server_release(ftx_server server, uint64_t handle);
+/* folded encodings of both halves of a vector (waits for its pending work) */
+ft_error ftx_server_read(ftx_server server, uint64_t handle, uint8_t *c1_enc, uint8_t *c2_enc);
+ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *batches, uint64_t *completed);
+
 /* ---------------------------------------------------------- inputs */
 
 /* n scalars uniform mod l: 64 bytes of SplitMix64(seed, stream) output per

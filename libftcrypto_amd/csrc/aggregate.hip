@@ -230,6 +230,51 @@ hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems,
   return hipGetLastError();
 }
 
+size_t segsum_records(size_t n) {
+  const size_t t1 = (n + 15) / 16, t2 = (2 * t1 + 15) / 16;
+  return 2 * t1 + 2 * t2;
+}
+
+// Scratch of the segmented sum, as points of one array: level-1 records
+// (2 per 16-element chunk) and level-2 records in plane slots [0, R), their
+// int64 segment keys in plane-0 slots [R, R + R/4).
+size_t segsum_scratch_points(size_t n) {
+  const size_t R = segsum_records(n);
+  return R + R / 4 + 8;
+}
+
+hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
+                         PtView src, PtView scratch, hipStream_t s) {
+  if (nseg == 0) return hipSuccess;
+  hipError_t e = launch_seg_empty(out, d_off, nseg, s);
+  if (e != hipSuccess) return e;
+  const size_t t1 = (n_elems + 15) / 16;
+  const size_t R = segsum_records(n_elems);
+  PtView bufA{scratch.w, scratch.cap};
+  PtView bufB{scratch.w + 2 * t1 * 8, scratch.cap};     // same planes, later element slots
+  int64_t *keysA = reinterpret_cast<int64_t *>(scratch.w + R * 8);
+  int64_t *keysB = keysA + 2 * t1;
+  PtView elems = src;
+  const int64_t *in_keys = nullptr;
+  size_t n = n_elems;
+  int level = 1;
+  bool use_a = true;
+  while (n > 0) {
+    const size_t T = (n + 15) / 16;
+    const bool last = T <= 1;
+    PtView rec = last ? PtView{nullptr, 0} : (use_a ? bufA : bufB);
+    int64_t *keys = use_a ? keysA : keysB;
+    e = launch_seg_level(out, rec, keys, elems, in_keys, n, d_off, nseg, gather, level, s);
+    if (e != hipSuccess || last) return e;
+    elems = rec;
+    in_keys = keys;
+    n = 2 * T;
+    ++level;
+    use_a = !use_a;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s) {
   if (!nseg) return hipSuccess;
   hipLaunchKernelGGL(k_seg_empty, dim3((unsigned)((nseg + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, out, off,

@@ -187,46 +187,11 @@ ft_error scan_rec(PtView v, size_t n, bool exclusive, hipStream_t s) {
   return FT_ERR_NO_ERROR;
 }
 
-// Scratch of the segmented sum, as points of one array: level-1 records
-// (2 per 16-element chunk) and level-2 records in plane slots [0, R), their
-// int64 segment keys in plane-0 slots [R, R + R/4).
-size_t seg_records(size_t n) {
-  const size_t t1 = (n + 15) / 16, t2 = (2 * t1 + 15) / 16;
-  return 2 * t1 + 2 * t2;
-}
-size_t seg_scratch_points(size_t n) {
-  const size_t R = seg_records(n);
-  return R + R / 4 + 8;
-}
+size_t seg_scratch_points(size_t n) { return segsum_scratch_points(n); }
 
 ft_error segsum_run(ft_ge25519_array out, const int64_t *d_off, size_t nseg, size_t n_elems,
                     const int32_t *d_gather, const_ft_ge25519_array src, PtView scratch, hipStream_t s) {
-  if (nseg == 0) return FT_ERR_NO_ERROR;
-  CK(launch_seg_empty(view(out), d_off, nseg, s));
-  const size_t t1 = (n_elems + 15) / 16;
-  const size_t R = seg_records(n_elems);
-  PtView bufA{scratch.w, scratch.cap};
-  PtView bufB{scratch.w + 2 * t1 * 8, scratch.cap};     // same planes, later element slots
-  int64_t *keysA = reinterpret_cast<int64_t *>(scratch.w + R * 8);
-  int64_t *keysB = keysA + 2 * t1;
-  PtView elems = view(src);
-  const int64_t *in_keys = nullptr;
-  size_t n = n_elems;
-  int level = 1;
-  bool use_a = true;
-  while (n > 0) {
-    const size_t T = (n + 15) / 16;
-    const bool last = T <= 1;
-    PtView rec = last ? PtView{nullptr, 0} : (use_a ? bufA : bufB);
-    int64_t *keys = use_a ? keysA : keysB;
-    CK(launch_seg_level(view(out), rec, keys, elems, in_keys, n, d_off, nseg, d_gather, level, s));
-    if (last) break;
-    elems = rec;
-    in_keys = keys;
-    n = 2 * T;
-    ++level;
-    use_a = !use_a;
-  }
+  CK(segsum_launch(view(out), d_off, nseg, n_elems, d_gather, view(src), scratch, s));
   return FT_ERR_NO_ERROR;
 }
 
@@ -344,6 +309,11 @@ void words_to_bytes(uint8_t *o, const uint32_t *w, int nw) {
 int msg_digits(int bits) { return bits <= 0 ? 1 : (bits + 3) / 4 + 1; }
 
 }  // namespace
+
+namespace ft {
+const uint32_t *base_table() { return G.btab; }
+const uint32_t *key_table(const ftx_key_s *key) { return key->table; }
+}  // namespace ft
 
 extern "C" {
 

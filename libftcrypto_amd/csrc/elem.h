@@ -1,0 +1,111 @@
+// Per-element bodies of the elementwise ciphertext kernels, shared by the
+// single-array kernels (kernels.hip) and the batched multi-request kernels of
+// the serving loop (serve.hip).
+#pragma once
+#include "scalarmult.h"
+
+namespace ft {
+
+// a[i] = a[i] +/- b[j]
+template <bool SUB>
+__device__ __forceinline__ void add_one(PtView a, size_t i, PtView b, size_t j) {
+  ge_p3 p, q;
+  load_p3(p, a, i);
+  load_p3(q, b, j);
+  ge_cached c;
+  ge_p3_to_cached(c, q);
+  if (SUB) ge_cached_cneg(c, 1u);
+  ge_p1p1 t;
+  ge_add(t, p, c);
+  ge_p1p1_to_p3(p, t);
+  store_p3(a, i, p);
+}
+
+// 32-bit multipliers: 9 signed digits (the bit length is public), placed at
+// stream positions 55..63 so the window loop runs over nine windows only
+__device__ __forceinline__ void recode_u32(uint32_t e[16], uint32_t k) {
+  int carry = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) e[q] = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    int d = (i < 8 ? (int)((k >> (4 * i)) & 15u) : 0) + carry;
+    if (i < 8) {
+      carry = (d + 8) >> 4;
+      d -= carry << 4;
+    }
+    const int v = i + 55;
+    e[v >> 2] |= ((uint32_t)d & 0xffu) << (8 * (v & 3));
+  }
+}
+
+// a[i] = k a[i] and, when two, b[i] = k b[i]
+__device__ __forceinline__ void mul_u32_one(PtView a, PtView b, int two, size_t i, uint32_t k) {
+  uint32_t e[16];
+  ge_p3 p, h;
+  recode_u32(e, k);
+  load_p3(p, a, i);
+  varbase_windows(h, p, e, 55);
+  store_p3(a, i, h);
+  if (two) {
+    recode_u32(e, k);
+    load_p3(p, b, i);
+    varbase_windows(h, p, e, 55);
+    store_p3(b, i, h);
+  }
+}
+
+// c1[i] = r B, c2[i] = m B + r H (ACC: c1[i] += r B, c2[i] += r H).  The two
+// combs share the four middle doublings of r H with m B; the message uses nd
+// digits (nd = 0 for rerandomisation), a public bound on its bit length.
+// tb, th: LDS comb tables of B and H; r: the lane's 8-word scalar.
+template <bool ACC>
+__device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, const uint32_t *tb,
+                                            const uint32_t *th, uint64_t x, int nd, const uint32_t *r) {
+  uint32_t k[8], e[16], odd[8], even[8];
+  ld_words(k, r);
+  recode16(e, k);
+  // message digits (at most 17): odd ones and even ones, low first
+  uint32_t mo[3] = {0, 0, 0}, me[3] = {0, 0, 0};
+  if (!ACC) {
+    int carry = 0;
+#pragma unroll
+    for (int q = 0; q < 17; ++q) {
+      int d = (q < 16 ? (int)((x >> (4 * q)) & 15u) : 0) + carry;
+      carry = (d + 8) >> 4;
+      d -= carry << 4;
+      if (q == nd - 1) d += carry << 4;       // top digit keeps the final carry
+      const uint32_t db = q < nd ? ((uint32_t)d & 0xffu) : 0u;
+      const int h = q >> 1;
+      if (q & 1) mo[h >> 2] |= db << (8 * (h & 3));
+      else me[h >> 2] |= db << (8 * (h & 3));
+      if (q >= nd - 1) carry = 0;
+    }
+  }
+  ge_p3 a1, a2;
+  // C1 = r B
+  split_odd_even(odd, even, e);
+  ge_p3_0(a1);
+  comb_pass<8>(a1, tb, odd, 32);
+  dbl4(a1);
+  comb_pass<8>(a1, tb, even, 32);
+  // C2 = r H + m B, sharing the four doublings
+  split_odd_even(odd, even, e);
+  ge_p3_0(a2);
+  comb_pass<8>(a2, th, odd, 32);
+  if (!ACC) comb_pass<3>(a2, tb, mo, nd >> 1);
+  dbl4(a2);
+  comb_pass<8>(a2, th, even, 32);
+  if (!ACC) comb_pass<3>(a2, tb, me, (nd + 1) >> 1);
+  if (ACC) {
+    ge_p3 o;
+    load_p3(o, c1, i);
+    ge_p3_add(a1, o, a1);
+    load_p3(o, c2, i);
+    ge_p3_add(a2, o, a2);
+  }
+  store_p3(c1, i, a1);
+  store_p3(c2, i, a2);
+}
+
+}  // namespace ft

@@ -4,9 +4,9 @@
 // limbs of 25.5 bits, 100 int32 x int32 -> int64 products per multiply).
 // Here a field element is v[0..8] with value sum v[i] * 2^(29 i) (a 261-bit
 // span), reduced modulo p only lazily.  A multiply is 81
-// v_mad_u64_u32 (full rate on gfx950) into 17 uint64 column sums, a carry of
-// the high columns, a fold by 2^261 = 1216 (mod p) and a carry of the low
-// columns.  No branches, no data-dependent addresses: every routine is
+// v_mad_u64_u32 (full rate on gfx950) into 17 uint64 column sums, a fold of
+// the high columns by 2^261 = 1216 (mod p) done on their 32-bit halves (16
+// more mads), and one carry pass over the low columns.  No branches, no data-dependent addresses: every routine is
 // constant-time.
 //
 // Bound classes (largest value of any limb):
@@ -71,15 +71,17 @@ FT_HD void fe_carry(fe &r) {
 }
 
 // Reduce 17 column sums (weights 2^(29k)) to class N.
+// A high column k >= 9 is folded without carrying it first: its low word
+// lo weighs 2^(29k) = 2^261 2^(29(k-9)) -> 1216 lo into column k-9, its high
+// word hi weighs 2^32 2^(29k) = 8 2^(29(k+1)) -> 9728 hi into column k-8.
+// Each low column gains < 2^45.5, so the sums stay below 2^64 for every
+// allowed operand class (9 A B <= 2^63.76).  One carry pass then remains.
 FT_HD void fe_reduce_cols(fe &r, uint64_t c[17]) {
-  uint32_t h[8];
 #pragma unroll
-  for (int k = 9; k < 16; ++k) { c[k + 1] += c[k] >> 29; h[k - 9] = (uint32_t)c[k] & M29; }
-  h[7] = (uint32_t)c[16] & M29;
-  uint32_t c17 = (uint32_t)(c[16] >> 29);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) c[k] += (uint64_t)h[k] * FOLD261;
-  c[8] += (uint64_t)c17 * FOLD261;
+  for (int k = 9; k < 17; ++k) {
+    c[k - 9] += (uint64_t)(uint32_t)c[k] * FOLD261;
+    c[k - 8] += (uint64_t)(uint32_t)(c[k] >> 32) * (8u * FOLD261);
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) { c[k + 1] += c[k] >> 29; r.v[k] = (uint32_t)c[k] & M29; }
   r.v[8] = (uint32_t)c[8] & M29;

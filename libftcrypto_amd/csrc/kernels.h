@@ -32,5 +32,12 @@ hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems,
                             size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
                             hipStream_t s);
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s);
+size_t segsum_scratch_points(size_t n_elems);
+hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
+                         PtView src, PtView scratch, hipStream_t s);
+
+// api.hip: device tables owned by the library
+const uint32_t *base_table();
+const uint32_t *key_table(const struct ftx_key_s *key);
 
 }  // namespace ft

@@ -1,0 +1,730 @@
+// Serving loop: continuous batching of encrypt / add / multiply-by-u32 /
+// segmented-sum requests over ciphertext vectors kept resident in HBM.
+//
+// * Memory: one device arena of the configured HBM budget; a request that
+//   creates a vector is admitted only when the arena can hold its output
+//   (first-fit with coalescing).  Admission control is that allocation.
+// * Scheduling: a scheduler thread takes the oldest queued request and every
+//   later queued request of the same kind that does not touch a vector an
+//   earlier skipped request touches (per-vector order is kept), up to a batch
+//   element cap, and launches them as ONE kernel whose lanes find their
+//   request in a descriptor table.  Batches go to the first idle of several
+//   default-priority streams, so batches of different kinds overlap on the
+//   device; a batch touching a vector still in flight on another stream waits
+//   for that stream's event.
+// * Latency: submit time to the scheduler's observation of the batch's
+//   completion event (steady clock).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/ftcrypto_ext.h"
+#include "elem.h"
+#include "kernels.h"
+
+namespace ft {
+
+// ------------------------------------------------------------ batched kernels
+
+struct BItem {
+  PtView a1, a2;        // destination (encrypt output, add dst, mul operand)
+  PtView b1, b2;        // add source
+  const uint64_t *m;    // encrypt messages
+  const uint32_t *r;    // encrypt scalars, 8 words each
+  const uint32_t *k;    // multipliers
+  uint64_t start, n;    // lanes [start, start + n) of the launch
+};
+
+__device__ __forceinline__ int find_item(const BItem *it, int cnt, uint64_t t) {
+  int lo = 0, hi = cnt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (it[mid].start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(TPB) kb_encrypt(const BItem *items, int cnt, uint64_t total,
+                                                  const uint32_t *btab, const uint32_t *htab, int nd) {
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  if (t >= total) return;
+  const BItem &it = items[find_item(items, cnt, t)];
+  const uint64_t i = t - it.start;
+  encrypt_one<false>(it.a1, it.a2, i, tb, th, it.m[i], nd, it.r + i * 8);
+}
+
+__global__ void __launch_bounds__(TPB) kb_add(const BItem *items, int cnt, uint64_t total) {
+  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  if (t >= total) return;
+  const BItem &it = items[find_item(items, cnt, t)];
+  const uint64_t i = t - it.start;
+  add_one<false>(it.a1, i, it.b1, i);
+  add_one<false>(it.a2, i, it.b2, i);
+}
+
+__global__ void __launch_bounds__(TPB) kb_mul_u32(const BItem *items, int cnt, uint64_t total) {
+  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  if (t >= total) return;
+  const BItem &it = items[find_item(items, cnt, t)];
+  const uint64_t i = t - it.start;
+  mul_u32_one(it.a1, it.a2, 1, i, it.k[i]);
+}
+
+}  // namespace ft
+
+// ------------------------------------------------------------ server state
+
+namespace {
+
+using namespace ft;
+using Clock = std::chrono::steady_clock;
+
+enum Kind { K_ENCRYPT = 0, K_ADD = 1, K_MUL = 2, K_SEGSUM = 3 };
+constexpr size_t ALIGN = 256;
+
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  std::map<size_t, size_t> free_;     // offset -> length
+  void init(char *b, size_t s) { base = b; size = s; free_.clear(); free_[0] = s; used = 0; }
+  bool alloc(size_t bytes, size_t *off) {
+    bytes = (bytes + ALIGN - 1) / ALIGN * ALIGN;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, o = it->first;
+        free_.erase(it);
+        if (rest) free_[o + bytes] = rest;
+        used += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + ALIGN - 1) / ALIGN * ALIGN;
+    used -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+    }
+  }
+};
+
+struct Handle {
+  size_t n = 0;             // ciphertexts
+  bool allocated = false;
+  size_t off = 0, bytes = 0;
+  PtView c1{nullptr, 0}, c2{nullptr, 0};
+  int inflight = 0;         // launched batches not yet complete
+  int queued = 0;           // queued requests referencing it
+  bool released = false;
+  bool failed = false;      // its creating request failed
+  int last_slot = -1;       // stream of the last batch that touched it
+  uint64_t last_seq = 0;    // that batch's sequence number
+};
+
+struct Request {
+  uint64_t id = 0;
+  Kind kind = K_ENCRYPT;
+  uint64_t dst = 0, src = 0;   // vectors: dst is written (created for encrypt/segsum)
+  size_t n = 0, nseg = 0;
+  int msg_bits = 0;
+  std::vector<uint8_t> in;     // host inputs owned by the request: m|r, k, or offsets
+  const uint8_t *bp[2] = {nullptr, nullptr};   // borrowed inputs (m, r) / (k) / (offsets)
+  size_t bn[2] = {0, 0};
+  Clock::time_point t_submit, t_done;
+  int state = 0;               // 0 queued, 1 in flight, 2 done
+  ft_error err = FT_ERR_NO_ERROR;
+};
+
+struct Slot {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+  uint64_t seq = 0;            // sequence number of the batch in flight / last done
+  uint64_t done_seq = 0;
+  char *dev = nullptr;         // device staging (inputs, descriptors, scratch)
+  size_t dev_bytes = 0;
+  char *host = nullptr;        // pinned staging
+  size_t host_bytes = 0;
+  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<uint64_t> touched;
+};
+
+}  // namespace
+
+struct ftx_server_s {
+  ftx_key key = nullptr;
+  const uint32_t *btab = nullptr;
+  bool borrow = false;
+  size_t max_batch = 0;
+  Arena arena;
+  void *arena_mem = nullptr;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
+  std::unordered_map<uint64_t, Handle> handles;
+  uint64_t next_req = 1, next_handle = 1;
+  uint64_t n_batches = 0, n_done = 0;
+  bool stop = false;
+  std::thread th;
+  ft_error fatal = FT_ERR_NO_ERROR;
+};
+
+namespace {
+
+// staging buffer growth (slot idle, so nothing of its is in flight)
+bool ensure_staging(Slot &s, size_t dev_need, size_t host_need) {
+  if (dev_need > s.dev_bytes) {
+    if (s.dev) (void)hipFree(s.dev);
+    s.dev_bytes = std::max(dev_need, (size_t)64 << 20);
+    if (hipMalloc(&s.dev, s.dev_bytes) != hipSuccess) { s.dev = nullptr; s.dev_bytes = 0; return false; }
+  }
+  if (host_need > s.host_bytes) {
+    if (s.host) (void)hipHostFree(s.host);
+    s.host_bytes = std::max(host_need, (size_t)64 << 20);
+    if (hipHostMalloc(&s.host, s.host_bytes, 0) != hipSuccess) { s.host = nullptr; s.host_bytes = 0; return false; }
+  }
+  return true;
+}
+
+size_t in_bytes(const Request &r) { return r.bp[0] ? r.bn[0] + r.bn[1] : r.in.size(); }
+const uint8_t *in_ptr(const Request &r, int q) { return r.bp[0] ? r.bp[q] : r.in.data() + (q ? r.bn[0] : 0); }
+
+void maybe_free(ftx_server_s *S, uint64_t h) {
+  auto it = S->handles.find(h);
+  if (it == S->handles.end()) return;
+  Handle &H = it->second;
+  if (H.released && H.inflight == 0 && H.queued == 0) {
+    if (H.allocated) S->arena.release(H.off, H.bytes);
+    S->handles.erase(it);
+  }
+}
+
+// point views of a vector of n ciphertexts at arena offset off
+void place(ftx_server_s *S, Handle &H, size_t off) {
+  H.off = off;
+  H.bytes = H.n * 256;
+  uint32_t *w = reinterpret_cast<uint32_t *>(S->arena.base + off);
+  const size_t cap = H.n ? H.n : 1;
+  H.c1 = PtView{w, cap};
+  H.c2 = PtView{w + 32 * cap, cap};
+  H.allocated = true;
+}
+
+void complete(ftx_server_s *S, Slot &sl) {
+  const auto now = Clock::now();
+  for (auto &r : sl.reqs) {
+    r->state = 2;
+    r->t_done = now;
+    ++S->n_done;
+  }
+  for (uint64_t h : sl.touched) {
+    auto it = S->handles.find(h);
+    if (it != S->handles.end()) {
+      it->second.inflight--;
+      maybe_free(S, h);
+    }
+  }
+  sl.reqs.clear();
+  sl.touched.clear();
+  sl.busy = false;
+  sl.done_seq = sl.seq;
+}
+
+void fail_request(ftx_server_s *S, const std::shared_ptr<Request> &r, ft_error e) {
+  r->err = e;
+  if (r->kind == K_ENCRYPT || r->kind == K_SEGSUM) S->handles[r->dst].failed = true;
+  r->state = 2;
+  r->t_done = Clock::now();
+  ++S->n_done;
+  for (uint64_t h : {r->dst, r->src}) {
+    auto it = S->handles.find(h);
+    if (h && it != S->handles.end()) { it->second.queued--; maybe_free(S, h); }
+  }
+}
+
+// vector `h` still being written by an in-flight batch on another stream?
+void wait_for(ftx_server_s *S, Slot &sl, int slot_idx, const Handle &H) {
+  if (H.inflight > 0 && H.last_slot >= 0 && H.last_slot != slot_idx) {
+    Slot &o = S->slots[H.last_slot];
+    if (o.busy && o.seq == H.last_seq) (void)hipStreamWaitEvent(sl.st, o.ev, 0);
+  }
+}
+
+// 1: ready, 0: waits for a vector still being created, -1: a vector it needs failed
+int request_ready(ftx_server_s *S, const Request &r) {
+  const bool dst_in = (r.kind == K_ADD || r.kind == K_MUL);     // dst must already exist
+  const bool src_in = (r.kind == K_ADD || r.kind == K_SEGSUM);
+  for (int q = 0; q < 2; ++q) {
+    const bool need = q == 0 ? dst_in : src_in;
+    if (!need) continue;
+    auto it = S->handles.find(q == 0 ? r.dst : r.src);
+    if (it == S->handles.end() || it->second.failed) return -1;
+    if (!it->second.allocated) return 0;
+  }
+  return 1;
+}
+
+// launch one batch of same-kind requests on slot idx (lock held)
+void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &batch) {
+  Slot &sl = S->slots[idx];
+  const Kind kind = batch[0]->kind;
+  // staging sizes
+  size_t in_total = 0;
+  for (auto &r : batch) in_total += (in_bytes(*r) + 255) / 256 * 256;
+  const size_t desc = (batch.size() * sizeof(BItem) + 255) / 256 * 256;
+  if (!ensure_staging(sl, in_total + desc, in_total + desc)) {
+    for (auto &r : batch) fail_request(S, r, FT_ERR_OUT_OF_MEMORY);
+    return;
+  }
+  // host staging: inputs then descriptors
+  std::vector<BItem> items(batch.size());
+  size_t off = 0;
+  uint64_t start = 0;
+  int nd = 1;
+  for (size_t j = 0; j < batch.size(); ++j) {
+    Request &r = *batch[j];
+    Handle &D = S->handles[r.dst];
+    BItem &it = items[j];
+    std::memset(&it, 0, sizeof it);
+    it.a1 = D.c1;
+    it.a2 = D.c2;
+    if (kind == K_ADD) {
+      Handle &Sx = S->handles[r.src];
+      it.b1 = Sx.c1;
+      it.b2 = Sx.c2;
+    }
+    char *din = sl.dev + off;
+    if (kind == K_ENCRYPT) {
+      it.m = reinterpret_cast<const uint64_t *>(din);
+      it.r = reinterpret_cast<const uint32_t *>(din + r.n * 8);
+      const int d = r.msg_bits <= 0 ? 1 : (r.msg_bits + 3) / 4 + 1;
+      nd = std::max(nd, d);
+    } else if (kind == K_MUL) {
+      it.k = reinterpret_cast<const uint32_t *>(din);
+    }
+    off += (in_bytes(r) + 255) / 256 * 256;
+    it.start = start;
+    it.n = r.n;
+    start += r.n;
+  }
+  // inputs: one DMA per request (borrowed, ideally pinned, client memory) or
+  // a copy through the slot's pinned staging and one DMA for the batch
+  std::memcpy(sl.host + off, items.data(), items.size() * sizeof(BItem));
+  const BItem *ditems = reinterpret_cast<const BItem *>(sl.dev + off);
+  size_t o2 = 0;
+  for (auto &rp : batch) {
+    const Request &r = *rp;
+    const size_t b0 = r.bp[0] ? r.bn[0] : in_bytes(r), b1 = r.bp[0] ? r.bn[1] : 0;
+    if (S->borrow) {
+      (void)hipMemcpyAsync(sl.dev + o2, in_ptr(r, 0), b0, hipMemcpyHostToDevice, sl.st);
+      if (b1) (void)hipMemcpyAsync(sl.dev + o2 + b0, in_ptr(r, 1), b1, hipMemcpyHostToDevice, sl.st);
+    } else {
+      std::memcpy(sl.host + o2, r.in.data(), r.in.size());
+    }
+    o2 += (in_bytes(r) + 255) / 256 * 256;
+  }
+  // ordering against other streams
+  for (auto &r : batch) {
+    for (uint64_t h : {r->dst, r->src}) {
+      if (!h) continue;
+      auto it = S->handles.find(h);
+      if (it != S->handles.end()) wait_for(S, sl, idx, it->second);
+    }
+  }
+  hipError_t e = S->borrow
+      ? hipMemcpyAsync(sl.dev + off, sl.host + off, items.size() * sizeof(BItem), hipMemcpyHostToDevice, sl.st)
+      : hipMemcpyAsync(sl.dev, sl.host, off + items.size() * sizeof(BItem), hipMemcpyHostToDevice, sl.st);
+  const unsigned grid = (unsigned)((start + TPB - 1) / TPB);
+  if (e == hipSuccess && start > 0) {
+    if (kind == K_ENCRYPT) {
+      hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
+                         key_table(S->key), nd);
+    } else if (kind == K_ADD) {
+      hipLaunchKernelGGL(kb_add, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
+    } else if (kind == K_MUL) {
+      hipLaunchKernelGGL(kb_mul_u32, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && kind == K_SEGSUM) {
+    Request &r = *batch[0];
+    Handle &Sx = S->handles[r.src];
+    Handle &D = S->handles[r.dst];
+    const size_t n_el = (size_t)reinterpret_cast<const int64_t *>(in_ptr(r, 0))[r.nseg];
+    const size_t pts = segsum_scratch_points(n_el);
+    // scratch after the inputs; the slot is idle so it may be reallocated
+    char *scr = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void **>(&scr), pts * 128, sl.st) != hipSuccess) {
+      e = hipErrorOutOfMemory;
+    } else {
+      PtView sv{reinterpret_cast<uint32_t *>(scr), pts};
+      const int64_t *doff = reinterpret_cast<const int64_t *>(sl.dev);
+      e = segsum_launch(D.c1, doff, r.nseg, n_el, nullptr, Sx.c1, sv, sl.st);
+      if (e == hipSuccess) e = segsum_launch(D.c2, doff, r.nseg, n_el, nullptr, Sx.c2, sv, sl.st);
+      (void)hipFreeAsync(scr, sl.st);
+    }
+  }
+  if (e == hipSuccess) e = hipEventRecord(sl.ev, sl.st);
+  if (e != hipSuccess) {
+    S->fatal = FT_ERR_INTERNAL_LIBRARY_ERROR;
+    for (auto &r : batch) fail_request(S, r, FT_ERR_INTERNAL_LIBRARY_ERROR);
+    return;
+  }
+  sl.busy = true;
+  sl.seq++;
+  ++S->n_batches;
+  for (auto &r : batch) {
+    r->state = 1;
+    for (uint64_t h : {r->dst, r->src}) {
+      if (!h) continue;
+      Handle &H = S->handles[h];
+      H.queued--;
+      H.inflight++;
+      H.last_slot = idx;
+      H.last_seq = sl.seq;
+      sl.touched.push_back(h);
+    }
+  }
+  sl.reqs = batch;
+}
+
+void scheduler(ftx_server_s *S) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (!S->stop) {
+    // completions
+    bool any_busy = false;
+    for (auto &sl : S->slots) {
+      if (!sl.busy) continue;
+      if (hipEventQuery(sl.ev) == hipSuccess) complete(S, sl);
+      else any_busy = true;
+    }
+    S->cv_done.notify_all();
+    int idle = -1;
+    for (size_t i = 0; i < S->slots.size(); ++i)
+      if (!S->slots[i].busy) { idle = (int)i; break; }
+    if (S->queue.empty() || idle < 0) {
+      if (any_busy || idle < 0) {
+        lk.unlock();
+        std::this_thread::yield();
+        lk.lock();
+      } else {
+        S->cv_work.wait_for(lk, std::chrono::microseconds(200));
+      }
+      continue;
+    }
+    // form a batch: the oldest ready request and later ready ones of its kind;
+    // a request touching a vector of a skipped request, or of one already in
+    // the batch, stays queued (per-vector order, no intra-batch aliasing)
+    std::vector<std::shared_ptr<Request>> batch;
+    std::unordered_set<uint64_t> blocked, in_batch;
+    size_t elems = 0;
+    Kind kind = K_ENCRYPT;
+    for (auto it = S->queue.begin(); it != S->queue.end();) {
+      Request &r = **it;
+      const bool touches_blocked = (r.dst && (blocked.count(r.dst) || in_batch.count(r.dst))) ||
+                                   (r.src && (blocked.count(r.src) || in_batch.count(r.src)));
+      const int ready = touches_blocked ? 0 : request_ready(S, r);
+      if (ready < 0) {
+        auto rr = *it;
+        it = S->queue.erase(it);
+        fail_request(S, rr, FT_ERR_OUT_OF_BOUNDS);
+        continue;
+      }
+      bool take = false;
+      if (ready > 0) {
+        if (batch.empty()) {
+          take = true;
+        } else if (r.kind == kind && r.kind != K_SEGSUM && elems + r.n <= S->max_batch) {
+          take = true;
+        }
+      }
+      if (take && (r.kind == K_ENCRYPT || r.kind == K_SEGSUM)) {
+        Handle &D = S->handles[r.dst];
+        if (!D.allocated) {
+          size_t off;
+          if (!S->arena.alloc(D.n * 256, &off)) {
+            take = false;                                 // admission: wait for memory
+            bool any_inflight = false;
+            for (auto &sl2 : S->slots) any_inflight |= sl2.busy;
+            if (batch.empty() && !any_inflight && D.n * 256 > S->arena.size) {   // can never fit
+              auto rr = *it;
+              it = S->queue.erase(it);
+              fail_request(S, rr, FT_ERR_OUT_OF_MEMORY);
+              continue;
+            }
+          } else {
+            place(S, D, off);
+          }
+        }
+      }
+      if (take) {
+        if (batch.empty()) kind = r.kind;
+        elems += r.n;
+        if (r.dst) in_batch.insert(r.dst);
+        if (r.src) in_batch.insert(r.src);
+        batch.push_back(*it);
+        it = S->queue.erase(it);
+        if (kind == K_SEGSUM) break;
+      } else {
+        if (r.dst) blocked.insert(r.dst);
+        if (r.src) blocked.insert(r.src);
+        ++it;
+      }
+    }
+    if (batch.empty()) {
+      lk.unlock();
+      std::this_thread::yield();
+      lk.lock();
+      continue;
+    }
+    launch(S, idle, batch);
+  }
+}
+
+ft_error submit(ftx_server_s *S, std::shared_ptr<Request> r, uint64_t *req) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->fatal) return S->fatal;
+  r->id = S->next_req++;
+  r->t_submit = Clock::now();
+  for (uint64_t h : {r->dst, r->src})
+    if (h) S->handles[h].queued++;
+  S->reqs[r->id] = r;
+  S->queue.push_back(r);
+  *req = r->id;
+  S->cv_work.notify_one();
+  return FT_ERR_NO_ERROR;
+}
+
+}  // namespace
+
+extern "C" {
+
+ft_error ftx_server_create(ftx_server *out, ftx_key key, size_t hbm_budget_bytes, int n_streams, size_t max_batch,
+                           int flags) {
+  if (!key || n_streams < 1 || n_streams > 16 || hbm_budget_bytes < 4096) return FT_ERR_BAD_SIZE;
+  auto *S = new (std::nothrow) ftx_server_s;
+  if (!S) return FT_ERR_OUT_OF_MEMORY;
+  S->key = key;
+  S->borrow = (flags & FTX_SERVER_BORROW_INPUTS) != 0;
+  S->btab = base_table();
+  S->max_batch = max_batch ? max_batch : ((size_t)1 << 22);
+  if (hipMalloc(&S->arena_mem, hbm_budget_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete S;
+    return FT_ERR_OUT_OF_MEMORY;
+  }
+  S->arena.init(static_cast<char *>(S->arena_mem), hbm_budget_bytes);
+  S->slots.resize(n_streams);
+  for (auto &sl : S->slots) {
+    if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+      ftx_server_destroy(S);
+      return FT_ERR_INTERNAL_LIBRARY_ERROR;
+    }
+  }
+  S->th = std::thread(scheduler, S);
+  *out = S;
+  return FT_ERR_NO_ERROR;
+}
+
+void ftx_server_destroy(ftx_server S) {
+  if (!S) return;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    // drain: wait for queued and in-flight work
+    S->cv_done.wait(lk, [&] {
+      bool busy = false;
+      for (auto &sl : S->slots) busy |= sl.busy;
+      return S->queue.empty() && !busy;
+    });
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->th.joinable()) S->th.join();
+  for (auto &sl : S->slots) {
+    if (sl.st) (void)hipStreamSynchronize(sl.st);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.st) (void)hipStreamDestroy(sl.st);
+    if (sl.dev) (void)hipFree(sl.dev);
+    if (sl.host) (void)hipHostFree(sl.host);
+  }
+  if (S->arena_mem) (void)hipFree(S->arena_mem);
+  delete S;
+}
+
+ft_error ftx_server_encrypt(ftx_server S, const uint64_t *m, const uint8_t *r, size_t n, int msg_bits,
+                            uint64_t *req, uint64_t *handle) {
+  if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
+  auto q = std::make_shared<Request>();
+  q->kind = K_ENCRYPT;
+  q->n = n;
+  q->msg_bits = msg_bits ? msg_bits : 64;
+  q->bn[0] = n * 8;
+  q->bn[1] = n * 32;
+  if (S->borrow) {
+    q->bp[0] = reinterpret_cast<const uint8_t *>(m);
+    q->bp[1] = r;
+  } else {
+    q->in.resize(n * 8 + n * 32);
+    std::memcpy(q->in.data(), m, n * 8);
+    std::memcpy(q->in.data() + n * 8, r, n * 32);
+  }
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    q->dst = S->next_handle++;
+    S->handles[q->dst].n = n;
+  }
+  *handle = q->dst;
+  return submit(S, q, req);
+}
+
+ft_error ftx_server_add(ftx_server S, uint64_t dst, uint64_t src, uint64_t *req) {
+  auto q = std::make_shared<Request>();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto a = S->handles.find(dst), b = S->handles.find(src);
+    if (a == S->handles.end() || b == S->handles.end() || a->second.released || b->second.released)
+      return FT_ERR_OUT_OF_BOUNDS;
+    if (a->second.n != b->second.n) return FT_ERR_SIZE_MISMATCH;
+    q->n = a->second.n;
+  }
+  q->kind = K_ADD;
+  q->dst = dst;
+  q->src = src;
+  return submit(S, q, req);
+}
+
+ft_error ftx_server_mul_u32(ftx_server S, uint64_t h, const uint32_t *k, size_t n, uint64_t *req) {
+  auto q = std::make_shared<Request>();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto a = S->handles.find(h);
+    if (a == S->handles.end() || a->second.released) return FT_ERR_OUT_OF_BOUNDS;
+    if (a->second.n != n) return FT_ERR_SIZE_MISMATCH;
+  }
+  q->kind = K_MUL;
+  q->n = n;
+  q->dst = h;
+  q->bn[0] = n * 4;
+  if (S->borrow) q->bp[0] = reinterpret_cast<const uint8_t *>(k);
+  else q->in.assign(reinterpret_cast<const uint8_t *>(k), reinterpret_cast<const uint8_t *>(k) + n * 4);
+  return submit(S, q, req);
+}
+
+ft_error ftx_server_segsum(ftx_server S, uint64_t h, const int64_t *offsets, size_t nseg, uint64_t *req,
+                           uint64_t *out_handle) {
+  auto q = std::make_shared<Request>();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto a = S->handles.find(h);
+    if (a == S->handles.end() || a->second.released) return FT_ERR_OUT_OF_BOUNDS;
+    if (offsets[0] < 0) return FT_ERR_OUT_OF_BOUNDS;
+    for (size_t s = 0; s < nseg; ++s)
+      if (offsets[s + 1] < offsets[s]) return FT_ERR_BAD_SIZE;
+    if ((size_t)offsets[nseg] > a->second.n) return FT_ERR_OUT_OF_BOUNDS;
+    q->dst = S->next_handle++;
+    S->handles[q->dst].n = nseg;
+  }
+  q->kind = K_SEGSUM;
+  q->n = nseg;
+  q->nseg = nseg;
+  q->src = h;
+  q->bn[0] = (nseg + 1) * 8;
+  if (S->borrow) q->bp[0] = reinterpret_cast<const uint8_t *>(offsets);
+  else q->in.assign(reinterpret_cast<const uint8_t *>(offsets), reinterpret_cast<const uint8_t *>(offsets + nseg + 1));
+  *out_handle = q->dst;
+  return submit(S, q, req);
+}
+
+ft_error ftx_server_wait(ftx_server S, uint64_t req, double *latency_us) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(req);
+  if (it == S->reqs.end()) return FT_ERR_OUT_OF_BOUNDS;
+  auto r = it->second;
+  S->cv_done.wait(lk, [&] { return r->state == 2; });
+  if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count();
+  S->reqs.erase(req);
+  return r->err;
+}
+
+ft_error ftx_server_release(ftx_server S, uint64_t h) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->handles.find(h);
+  if (it == S->handles.end() || it->second.released) return FT_ERR_OUT_OF_BOUNDS;
+  it->second.released = true;
+  maybe_free(S, h);
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_server_read(ftx_server S, uint64_t h, uint8_t *c1_enc, uint8_t *c2_enc) {
+  PtView v1, v2;
+  size_t n;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->handles.find(h);
+    if (it == S->handles.end()) return FT_ERR_OUT_OF_BOUNDS;
+    S->cv_done.wait(lk, [&] {
+      auto jt = S->handles.find(h);
+      return jt == S->handles.end() || (jt->second.queued == 0 && jt->second.inflight == 0);
+    });
+    it = S->handles.find(h);
+    if (it == S->handles.end() || !it->second.allocated) return FT_ERR_OUT_OF_BOUNDS;
+    it->second.queued++;                 // pin while reading
+    v1 = it->second.c1;
+    v2 = it->second.c2;
+    n = it->second.n;
+  }
+  ft_error err = FT_ERR_NO_ERROR;
+  void *d = nullptr;
+  hipStream_t st = (hipStream_t)ftx_default_stream();
+  if (hipMalloc(&d, n * 32 + 32) != hipSuccess) err = FT_ERR_OUT_OF_MEMORY;
+  if (!err && launch_encode((uint32_t *)d, v1, n, st) != hipSuccess) err = FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (!err && hipMemcpyAsync(c1_enc, d, n * 32, hipMemcpyDeviceToHost, st) != hipSuccess) err = FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (!err && hipStreamSynchronize(st) != hipSuccess) err = FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (!err && launch_encode((uint32_t *)d, v2, n, st) != hipSuccess) err = FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (!err && hipMemcpyAsync(c2_enc, d, n * 32, hipMemcpyDeviceToHost, st) != hipSuccess) err = FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (!err && hipStreamSynchronize(st) != hipSuccess) err = FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (d) (void)hipFree(d);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->handles[h].queued--;
+    maybe_free(S, h);
+  }
+  return err;
+}
+
+ft_error ftx_server_stats(ftx_server S, uint64_t *bytes_in_use, uint64_t *batches, uint64_t *done) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (bytes_in_use) *bytes_in_use = S->arena.used;
+  if (batches) *batches = S->n_batches;
+  if (done) *done = S->n_done;
+  return FT_ERR_NO_ERROR;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+"""The serving loop (ftx_server): every result is the same computed in the
+mixed, batched, multi-stream stream as computed alone by plain one-shot
+kernels, and matches the oracle; admission under a tight HBM budget still
+completes every request; bad requests are refused with the reference's error
+codes."""
+import numpy as np
+import pytest
+
+import libftcrypto_amd as ft
+from libftcrypto_amd import serving as SV
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def key():
+    ft.init()
+    x = ft.random_scalars(1, 4242, 0)
+    pk = ft.GeArray.from_scalars(x)
+    return ft.Key(pk), bytes(pk.to_bytes_folded())
+
+
+def compare_all(server, hmap, vec, size_of, freed):
+    checked = 0
+    for v, h in hmap.items():
+        if v in freed:
+            continue
+        n = size_of[v]
+        a, b = server.read(h, n)
+        assert np.array_equal(a, vec[v][0].to_bytes_folded()), v
+        assert np.array_equal(b, vec[v][1].to_bytes_folded()), v
+        checked += 1
+    return checked
+
+
+@pytest.mark.parametrize("streams,max_batch,budget,max_live",
+                         [(4, 1 << 22, 1 << 30, 10**9), (2, 3000, 1 << 30, 10**9), (3, 1 << 22, 6 << 20, 6)])
+def test_mixed_stream_equals_alone(key, streams, max_batch, budget, max_live):
+    k, H = key
+    ops, size_of = SV.make_trace(120, seed=streams * 7 + max_batch % 97, mu_log2=9, sigma=1.0, lo_log2=8,
+                                 hi_log2=12, max_live=max_live)
+    freed = {op[1] for op in ops if op[0] == SV.FREE}
+    srv = SV.Server(k, budget_bytes=budget, n_streams=streams, max_batch=max_batch)
+    wall, lats, hmap = SV.run_served(srv, ops)
+    assert len(lats) == sum(1 for op in ops if op[0] != SV.FREE)
+    assert (lats > 0).all()
+    _, vec = SV.run_oneshot(k, ops, size_of)
+    assert compare_all(srv, hmap, vec, size_of, freed) == len(hmap) - len(freed)
+    st = srv.stats()
+    assert st["batches"] <= st["completed"]
+    srv.close()
+
+
+def test_releases_and_oracle(key):
+    k, H = key
+    ops, size_of = SV.make_trace(150, seed=77, mu_log2=9, sigma=1.0, lo_log2=8, hi_log2=11, max_live=12)
+    freed = {op[1] for op in ops if op[0] == SV.FREE}
+    assert freed
+    srv = SV.Server(k, budget_bytes=64 << 20, n_streams=4)
+    wall, lats, hmap = SV.run_served(srv, ops)
+    _, vec = SV.run_oneshot(k, ops, size_of)
+    assert compare_all(srv, hmap, vec, size_of, freed) > 0
+    # an encrypt served in the mix equals the oracle's encryption
+    modified = {o[1] for o in ops if o[0] in (SV.ADD, SV.MUL)}
+    enc = [op for op in ops if op[0] == SV.ENC and op[3] not in freed and op[3] not in modified]
+    for op in enc[:3]:
+        a, b = srv.read(hmap[op[3]], len(op[1]))
+        o1, o2 = O.encrypt(H, op[1], op[2])
+        assert np.array_equal(a, o1.to_bytes()) and np.array_equal(b, o2.to_bytes())
+    srv.close()
+
+
+def test_bad_requests(key):
+    k, _ = key
+    srv = SV.Server(k, budget_bytes=1 << 24, n_streams=2)
+    m = np.zeros(300, dtype=np.uint64)
+    r = ft.random_scalars(300, 5, 5)
+    q1, h1 = srv.encrypt(m, r, 1)
+    q2, h2 = srv.encrypt(m[:200].copy(), r[:200].copy(), 1)
+    srv.wait(q1)
+    srv.wait(q2)
+    with pytest.raises(ft.FtError) as e:
+        srv.add(h1, h2)
+    assert e.value.code == ft.FT_ERR_SIZE_MISMATCH
+    with pytest.raises(ft.FtError) as e:
+        srv.add(h1, 999999)
+    assert e.value.code == ft.FT_ERR_OUT_OF_BOUNDS
+    # a vector larger than the whole budget is refused, not queued forever
+    big = 1 << 17
+    q3, h3 = srv.encrypt(np.zeros(big, dtype=np.uint64), ft.random_scalars(big, 6, 6), 1)
+    with pytest.raises(ft.FtError) as e:
+        srv.wait(q3)
+    assert e.value.code == ft.FT_ERR_OUT_OF_MEMORY
+    srv.close()
