@@ -136,6 +136,11 @@ def run(args):
         hip.L.hipEventRecord(ev[1], stream)
         return hip.elapsed_ms(ev[0], ev[1]) / reps
 
+    if args.calibration_copy:
+        # a plain copy kernel of known size (2 x N x 128 B moved) for PMC byte-counter calibration
+        cp = pts.get_range(0, n, 1)
+        ft.check(L.ftx_synchronize(None))
+        del cp
     log("[bench] warmup %d" % args.warmup)
     for _ in range(args.warmup):
         for f in ops.values():
@@ -174,10 +179,14 @@ def run(args):
         cpu = cpu_baseline(n, k_fixed, k_var, s_var, a_u32)
     del out_fixed, pts, c1, c2
     configs = {}
-    if not args.headline_only:
+    sections = set(args.sections.split(",")) if not args.headline_only else set()
+    if "cfg0" in sections:
         configs["cfg0_encrypt_add_reduce_n65536"] = bench_cfg0(ft, hip, key, seed)
+    if "cfg2" in sections:
         configs["cfg2_aggregation"] = bench_cfg2(ft, hip, key, seed, args)
+    if "cfg3" in sections:
         configs["cfg3_sparse_step_graph"] = bench_cfg3(ft, hip, key, seed, args)
+    if "cfg4" in sections:
         configs["cfg4_serving"] = bench_cfg4(ft, key, seed, args)
     detail["configs"] = configs
     return {
@@ -227,10 +236,12 @@ def bench_cfg0(ft, hip, key, seed):
     r = hip.upload(ft.random_scalars(n, seed, 10))
     a1, a2, b1, b2 = (ft.GeArray.alloc(n) for _ in range(4))
     out = ft.GeArray.alloc(1)
+    rscr = ft.GeArray.alloc(L.ftx_reduce_scratch_points(n))
     enc_ms = T(lambda: (ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, stream)),
                         ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, stream))))
     add_ms = T(lambda: (ft.check(L.ftx_add_dev(a1.h, b1.h, stream)), ft.check(L.ftx_add_dev(a2.h, b2.h, stream))))
-    red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, a1.h, stream)), ft.check(L.ftx_reduce_dev(out.h, a2.h, stream))))
+    red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, a1.h, rscr.h, stream)),
+                        ft.check(L.ftx_reduce_dev(out.h, a2.h, rscr.h, stream))))
     return {"encrypt": {"ms": round(enc_ms, 4), "ciphertexts_per_s": 2 * n / (enc_ms * 1e-3),
                         "valu_frac": 2 * n * (2 * PRODUCTS["fixed_base"]) / (enc_ms * 1e-3) / PEAK_MUL32},
             "ct_add": adds_entry(add_ms, 2 * n, 3 * POINT_BYTES),
@@ -253,7 +264,9 @@ def bench_cfg2(ft, hip, key, seed, args):
     r = hip.upload(ft.random_scalars(nr, seed, 20))
     ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
     out = ft.GeArray.alloc(1)
-    ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, c1.h, stream)), ft.check(L.ftx_reduce_dev(out.h, c2.h, stream))))
+    rscr = ft.GeArray.alloc(L.ftx_reduce_scratch_points(nr))
+    ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, c1.h, rscr.h, stream)),
+                    ft.check(L.ftx_reduce_dev(out.h, c2.h, rscr.h, stream))))
     res["reduce"] = adds_entry(ms, 2 * (nr - 1), POINT_BYTES)
     hip.L.hipFree(m); hip.L.hipFree(r)
     del c1, c2
@@ -270,7 +283,8 @@ def bench_cfg2(ft, hip, key, seed, args):
                     ft.check(L.ftx_segmented_sum_dev(s2.h, d_off, nseg, ns, None, c2.h, scr.h, stream))))
     res["segmented_sum"] = adds_entry(ms, 2 * (ns - nseg), (2 * ns * POINT_BYTES + (nseg + 1) * 8 + 2 * nseg * POINT_BYTES) / (2 * (ns - nseg)))
     res["segmented_sum"]["segments"] = nseg
-    ms = T(lambda: (ft.check(L.ftx_scan_dev(c1.h, 0, stream)), ft.check(L.ftx_scan_dev(c2.h, 0, stream))))
+    sscr = ft.GeArray.alloc(L.ftx_scan_scratch_points(ns))
+    ms = T(lambda: (ft.check(L.ftx_scan_dev(c1.h, 0, sscr.h, stream)), ft.check(L.ftx_scan_dev(c2.h, 0, sscr.h, stream))))
     res["scan"] = adds_entry(ms, 2 * (ns - 1), 2 * POINT_BYTES)
     hip.L.hipFree(m); hip.L.hipFree(r); hip.L.hipFree(d_off)
     return res
@@ -376,6 +390,10 @@ def main():
     ap.add_argument("--n", type=int, default=22, help="log2 of the array length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the configurations other than the N=2^22 scalar mults")
+    ap.add_argument("--sections", default="cfg0,cfg2,cfg3,cfg4",
+                    help="configurations measured after the headline step (comma list of cfg0,cfg2,cfg3,cfg4)")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="first run one plain copy kernel over the N points (N x 128 B read and written)")
     ap.add_argument("--cfg2-reduce-log2", type=int, default=26)
     ap.add_argument("--cfg2-log2", type=int, default=24)
     ap.add_argument("--cfg3-nodes-log2", type=int, default=22)

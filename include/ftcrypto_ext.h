@@ -79,9 +79,16 @@ size_t ftx_segsum_scratch_points(size_t n_elems);
 ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg,
                                size_t n_elems, const void *d_gather,
                                const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream);
-/* out[0] = sum of src (out needs 1 element) */
-ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, void *stream);
-ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, void *stream);
+/* out[0] = sum of src (out needs 1 element).  scratch: an array of at least
+ * ftx_reduce_scratch_points(len(src)) points, or NULL (then the call allocates
+ * and synchronises). */
+size_t ftx_reduce_scratch_points(size_t n);
+ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, ft_ge25519_array scratch,
+                        void *stream);
+/* in-place inclusive (exclusive = 0) or exclusive scan; scratch as above with
+ * ftx_scan_scratch_points(len(array)) points */
+size_t ftx_scan_scratch_points(size_t n);
+ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, ft_ge25519_array scratch, void *stream);
 /* 32-byte canonical compressed encodings into device memory d_out */
 ft_error ftx_encode_dev(void *d_out, const_ft_ge25519_array array, void *stream);
 
@@ -97,7 +104,8 @@ void ftx_stream_destroy(void *stream);
 /* Capture the _dev calls issued on `stream` (not the library stream) between
  * begin and end into a graph.  Capturable: ftx_from_scalars_dev, ftx_mul_dev,
  * ftx_mul_u32_dev, ftx_add_dev, ftx_encrypt_dev, ftx_rerandomize_dev,
- * ftx_encode_dev, and ftx_segmented_sum_dev with caller-provided scratch. */
+ * ftx_encode_dev, and ftx_segmented_sum_dev / ftx_reduce_dev / ftx_scan_dev with
+ * caller-provided scratch. */
 ft_error ftx_capture_begin(void *stream);
 ft_error ftx_capture_end(void *stream, ftx_graph *graph);
 ft_error ftx_graph_launch(ftx_graph graph, void *stream);

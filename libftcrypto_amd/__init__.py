@@ -97,8 +97,10 @@ _SIGS = {
     "ftx_rerandomize_dev": (i32, [vp, vp, vp, vp, vp]),
     "ftx_segsum_scratch_points": (sz, [sz]),
     "ftx_segmented_sum_dev": (i32, [vp, vp, sz, sz, vp, vp, vp, vp]),
-    "ftx_reduce_dev": (i32, [vp, vp, vp]),
-    "ftx_scan_dev": (i32, [vp, i32, vp]),
+    "ftx_reduce_scratch_points": (sz, [sz]),
+    "ftx_reduce_dev": (i32, [vp, vp, vp, vp]),
+    "ftx_scan_scratch_points": (sz, [sz]),
+    "ftx_scan_dev": (i32, [vp, i32, vp, vp]),
     "ftx_encode_dev": (i32, [vp, vp, vp]),
     "ftx_stream_create": (i32, [vp]),
     "ftx_stream_destroy": (None, [vp]),

@@ -28,42 +28,91 @@ constexpr int CHN = 1 << CH;
 
 size_t scan_tiles(size_t n) { return (n + CHN - 1) >> CH; }
 
-// sums[t] = sum of elements [16 t, 16 t + 16)
+// sums[t] = sum of elements [16 t, 16 t + 16), read a cache line at a time
 __global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size_t n) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const size_t lo = t << CH;
   if (lo >= n) return;
   const size_t hi = (lo + CHN < n) ? lo + CHN : n;
   ge_p3 acc, x;
-  load_p3(acc, v, lo);
+  ge_p3_0(acc);
 #pragma unroll 1
-  for (size_t i = lo + 1; i < hi; ++i) {
-    load_p3(x, v, i);
-    ge_p3_add(acc, acc, x);
+  for (size_t g = lo; g < hi; g += 4) {
+    PW q0, q1, q2, q3;
+    FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    un_pw(x, q0);
+    if (g == lo) acc = x;
+    else ge_p3_add(acc, acc, x);
+    if (g + 1 < hi) { un_pw(x, q1); ge_p3_add(acc, acc, x); }
+    if (g + 2 < hi) { un_pw(x, q2); ge_p3_add(acc, acc, x); }
+    if (g + 3 < hi) { un_pw(x, q3); ge_p3_add(acc, acc, x); }
   }
   store_p3(sums, t, acc);
 }
 
-// walk chunk t from its exclusive prefix (identity when pre.w is null)
-__global__ void __launch_bounds__(ATPB) k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
+// one scan element: q holds the element on entry and its output on return
+__device__ __forceinline__ void scan_step(ge_p3 &run, PW &q, int exclusive) {
+  ge_p3 x;
+  un_pw(x, q);
+  if (exclusive) pk_pw(q, run);
+  ge_p3_add(run, run, x);
+  if (!exclusive) pk_pw(q, run);
+}
+
+// walk chunk t from its exclusive prefix (identity when pre.w is null); the
+// four outputs of a group are written together, whole lines per coordinate
+__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const size_t lo = t << CH;
   if (lo >= n) return;
   const size_t hi = (lo + CHN < n) ? lo + CHN : n;
-  ge_p3 run, x;
+  ge_p3 run;
   if (pre.w) load_p3(run, pre, t);
   else ge_p3_0(run);
 #pragma unroll 1
-  for (size_t i = lo; i < hi; ++i) {
-    load_p3(x, v, i);
-    if (exclusive) {
-      store_p3(v, i, run);
-      ge_p3_add(run, run, x);
-    } else {
-      ge_p3_add(run, run, x);
-      store_p3(v, i, run);
-    }
+  for (size_t g = lo; g < hi; g += 4) {
+    PW q0, q1, q2, q3;
+    FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    scan_step(run, q0, exclusive);
+    if (g + 1 < hi) scan_step(run, q1, exclusive);
+    if (g + 2 < hi) scan_step(run, q2, exclusive);
+    if (g + 3 < hi) scan_step(run, q3, exclusive);
+    st_pw(v, g, q0);
+    if (g + 1 < hi) st_pw(v, g + 1, q1);
+    if (g + 2 < hi) st_pw(v, g + 2, q2);
+    if (g + 3 < hi) st_pw(v, g + 3, q3);
   }
+}
+
+size_t scan_scratch_points(size_t n) {
+  size_t tot = 0;
+  for (size_t m = scan_tiles(n); m > 1; m = scan_tiles(m)) tot += m;
+  return tot + 1;
+}
+
+// inclusive / exclusive scan of n points in place; chunk sums of every level
+// live in scratch (scan_scratch_points(n) points), no allocation, no sync
+hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  PtView lv[48];
+  size_t ln[48];
+  int L = 0;
+  lv[0] = v;
+  ln[0] = n;
+  size_t used = 0;
+  while (scan_tiles(ln[L]) > 1) {
+    const size_t m = scan_tiles(ln[L]);
+    lv[L + 1] = PtView{scratch.w + used * 8, scratch.cap};
+    ln[L + 1] = m;
+    used += m;
+    hipError_t e = launch_scan_tile_sums(lv[L + 1], lv[L], ln[L], s);
+    if (e != hipSuccess) return e;
+    ++L;
+  }
+  hipError_t e = launch_scan_apply(lv[L], ln[L], PtView{nullptr, 0}, L > 0 ? true : exclusive, s);
+  for (int l = L - 1; l >= 0 && e == hipSuccess; --l)
+    e = launch_scan_apply(lv[l], ln[l], lv[l + 1], l > 0 ? true : exclusive, s);
+  return e;
 }
 
 hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s) {
@@ -146,8 +195,8 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
   ge_p3 acc, x;
   ge_p3_0(acc);
   bool touched = false;
-#pragma unroll 1
-  for (int64_t i = clo; i < cend; ++i) {
+  // advance the segment cursor to element i; true when i lies in segment s
+  auto take = [&](int64_t i) -> bool {
     if (s < nseg && hi <= i) {
       if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
       touched = false;
@@ -157,12 +206,26 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
       if (s < nseg && off[s + 1] <= i) s = seg_find1(off, s, nseg, i);
       if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
     }
-    if (s < nseg && lo <= i) {
-      const size_t j = gather ? (size_t)gather[i] : (size_t)i;
-      load_p3(x, src, j);
-      if (touched) ge_p3_add(acc, acc, x);
-      else acc = x;
-      touched = true;
+    return s < nseg && lo <= i;
+  };
+  auto accum = [&]() {
+    if (touched) ge_p3_add(acc, acc, x);
+    else acc = x;
+    touched = true;
+  };
+  if (gather) {
+#pragma unroll 1
+    for (int64_t i = clo; i < cend; ++i)
+      if (take(i)) { load_p3(x, src, (size_t)gather[i]); accum(); }
+  } else {
+#pragma unroll 1
+    for (int64_t g = clo; g < cend; g += 4) {
+      PW q0, q1, q2, q3;
+      FT_LOAD4(q0, q1, q2, q3, src, (size_t)g, (size_t)cend);
+      if (take(g)) { un_pw(x, q0); accum(); }
+      if (g + 1 < cend && take(g + 1)) { un_pw(x, q1); accum(); }
+      if (g + 2 < cend && take(g + 2)) { un_pw(x, q2); accum(); }
+      if (g + 3 < cend && take(g + 3)) { un_pw(x, q3); accum(); }
     }
   }
   if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);

@@ -157,32 +157,29 @@ struct ArrGuard {
   ft_ge25519_array release() { auto r = a; a = nullptr; return r; }
 };
 
-ft_error reduce_into(ft_ge25519_array out, const_ft_ge25519_array src, hipStream_t s) {
+// out[0] = sum of src; part (reduce_grid(n) points) holds the block sums
+ft_error reduce_run(ft_ge25519_array out, const_ft_ge25519_array src, PtView part, hipStream_t s) {
   const size_t n = src->n;
   if (n == 0) { CK(launch_set_identity(view(out), 0, 1, s)); return FT_ERR_NO_ERROR; }
   const unsigned g1 = reduce_grid(n);
+  CK(launch_reduce_blocks(part, view(src), n, g1, s));
+  CK(launch_reduce_blocks(view(out), part, g1, 1, s));
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error reduce_into(ft_ge25519_array out, const_ft_ge25519_array src, hipStream_t s) {
   ArrGuard part;
-  CKE(alloc_array(&part.a, g1));
-  CK(launch_reduce_blocks(view(part.a), view(src), n, g1, s));
-  CK(launch_reduce_blocks(view(out), view(part.a), g1, 1, s));
+  CKE(alloc_array(&part.a, reduce_grid(src->n)));
+  CKE(reduce_run(out, src, view(part.a), s));
   CK(hipStreamSynchronize(s));
   return FT_ERR_NO_ERROR;
 }
 
 ft_error scan_rec(PtView v, size_t n, bool exclusive, hipStream_t s) {
   if (n == 0) return FT_ERR_NO_ERROR;
-  const size_t T = scan_tiles(n);
-  if (T <= 1) {
-    CK(launch_scan_apply(v, n, PtView{nullptr, 0}, exclusive, s));
-    return FT_ERR_NO_ERROR;
-  }
-  ft_ge25519_array sums;
-  CKE(alloc_array(&sums, T));
-  ArrGuard g;
-  g.a = sums;
-  CK(launch_scan_tile_sums(view(sums), v, n, s));
-  CKE(scan_rec(view(sums), T, true, s));
-  CK(launch_scan_apply(v, n, view(sums), exclusive, s));
+  ArrGuard scr;
+  CKE(alloc_array(&scr.a, scan_scratch_points(n)));
+  CK(scan_launch(v, n, exclusive, view(scr.a), s));
   CK(hipStreamSynchronize(s));
   return FT_ERR_NO_ERROR;
 }
@@ -980,13 +977,21 @@ ft_error ftx_gather_sum(ft_ge25519_array *res, const int64_t *row_off, const int
   return FT_ERR_NO_ERROR;
 }
 
-ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, void *stream) {
+size_t ftx_reduce_scratch_points(size_t n) { return reduce_grid(n); }
+size_t ftx_scan_scratch_points(size_t n) { return scan_scratch_points(n); }
+
+ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream) {
   if (out->n < 1) return FT_ERR_INSUFFICIENT_SPACE;
-  return reduce_into(out, src, S(stream));
+  if (!scratch) return reduce_into(out, src, S(stream));
+  if (scratch->cap < reduce_grid(src->n)) return FT_ERR_INSUFFICIENT_SPACE;
+  return reduce_run(out, src, view(scratch), S(stream));
 }
 
-ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, void *stream) {
-  return scan_rec(view(array), array->n, exclusive != 0, S(stream));
+ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, ft_ge25519_array scratch, void *stream) {
+  if (!scratch) return scan_rec(view(array), array->n, exclusive != 0, S(stream));
+  if (scratch->cap < scan_scratch_points(array->n)) return FT_ERR_INSUFFICIENT_SPACE;
+  CK(scan_launch(view(array), array->n, exclusive != 0, view(scratch), S(stream)));
+  return FT_ERR_NO_ERROR;
 }
 
 // ------------------------------------------------------------ inputs (host)

@@ -26,6 +26,8 @@ hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, 
 // aggregate.hip
 constexpr int SEG_CHUNK_LOG2 = 4;                     // elements per lane per level
 size_t scan_tiles(size_t n);
+size_t scan_scratch_points(size_t n);
+hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s);
 hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s);
 hipError_t launch_scan_apply(PtView v, size_t n, PtView tile_prefix, bool exclusive, hipStream_t s);
 hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,

@@ -82,6 +82,60 @@ __device__ __forceinline__ void copy_point(const PtView d, size_t i, const PtVie
   }
 }
 
+// A point as its packed 32-byte coordinates.  A lane walking consecutive
+// points loads four of them (i0 a multiple of 4: per coordinate one whole
+// 128-byte line) before using any, so no line is fetched twice; `pin_pw` is an
+// empty asm that takes the loaded words as register operands, which makes the
+// compiler issue all 32 loads up front instead of sinking them to their uses.
+struct PW { uint32_t x[8], y[8], z[8], t[8]; };
+
+__device__ __forceinline__ void ld_pw(PW &p, const PtView v, size_t i) {
+  ld_words(p.x, v.w + (0 * v.cap + i) * 8);
+  ld_words(p.y, v.w + (1 * v.cap + i) * 8);
+  ld_words(p.z, v.w + (2 * v.cap + i) * 8);
+  ld_words(p.t, v.w + (3 * v.cap + i) * 8);
+}
+
+__device__ __forceinline__ void un_pw(ge_p3 &r, const PW &p) {
+  fe_from_words(r.X, p.x);
+  fe_from_words(r.Y, p.y);
+  fe_from_words(r.Z, p.z);
+  fe_from_words(r.T, p.t);
+}
+
+__device__ __forceinline__ void st_pw(const PtView v, size_t i, const PW &p) {
+  st_words(v.w + (0 * v.cap + i) * 8, p.x);
+  st_words(v.w + (1 * v.cap + i) * 8, p.y);
+  st_words(v.w + (2 * v.cap + i) * 8, p.z);
+  st_words(v.w + (3 * v.cap + i) * 8, p.t);
+}
+
+__device__ __forceinline__ void pk_pw(PW &p, const ge_p3 &r) {
+  fe_store_words(p.x, r.X);
+  fe_store_words(p.y, r.Y);
+  fe_store_words(p.z, r.Z);
+  fe_store_words(p.t, r.T);
+}
+
+__device__ __forceinline__ void pin_pw(PW &p) {
+  asm volatile("" : "+v"(p.x[0]), "+v"(p.x[1]), "+v"(p.x[2]), "+v"(p.x[3]), "+v"(p.x[4]), "+v"(p.x[5]),
+                    "+v"(p.x[6]), "+v"(p.x[7]), "+v"(p.y[0]), "+v"(p.y[1]), "+v"(p.y[2]), "+v"(p.y[3]),
+                    "+v"(p.y[4]), "+v"(p.y[5]), "+v"(p.y[6]), "+v"(p.y[7]));
+  asm volatile("" : "+v"(p.z[0]), "+v"(p.z[1]), "+v"(p.z[2]), "+v"(p.z[3]), "+v"(p.z[4]), "+v"(p.z[5]),
+                    "+v"(p.z[6]), "+v"(p.z[7]), "+v"(p.t[0]), "+v"(p.t[1]), "+v"(p.t[2]), "+v"(p.t[3]),
+                    "+v"(p.t[4]), "+v"(p.t[5]), "+v"(p.t[6]), "+v"(p.t[7]));
+}
+
+// points g .. g+3 (indices at or past `hi` re-read point g; their values are unused)
+#define FT_LOAD4(q0, q1, q2, q3, v, g, hi)                   \
+  do {                                                       \
+    ld_pw(q0, v, g);                                         \
+    ld_pw(q1, v, ((g) + 1 < (hi)) ? (g) + 1 : (g));          \
+    ld_pw(q2, v, ((g) + 2 < (hi)) ? (g) + 2 : (g));          \
+    ld_pw(q3, v, ((g) + 3 < (hi)) ? (g) + 3 : (g));          \
+    pin_pw(q0); pin_pw(q1); pin_pw(q2); pin_pw(q3);           \
+  } while (0)
+
 // ---------------------------------------------------------- comb entries
 
 __device__ __forceinline__ void comb_entry_store(uint32_t *dst, const ge_precomp &e) {
