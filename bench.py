@@ -49,6 +49,17 @@ PRODUCTS = {
 }
 POINT_BYTES = 128
 
+# HBM bytes per launch at N = 2^22 from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one pass each), scaled by the same counters' reading for a
+# plain copy kernel of known size (x1.993 read, x0.994 write), and the VALU
+# busy fraction of the same run (profiles/scalarmult_n22_pmc_counters.md).
+PMC_N22 = {
+    "fixed_base": {"read": 134.9e6, "write": 533.6e6, "valu_busy": 0.980},
+    "variable_base": {"read": 670.9e6, "write": 533.6e6, "valu_busy": 0.861},
+    "mul_u32": {"read": 1090.8e6, "write": 1067.1e6, "valu_busy": 0.834},
+}
+ALG_BYTES = {"fixed_base": 32 + 128, "variable_base": 32 + 2 * 128, "mul_u32": 4 + 4 * 128}
+
 
 class Hip:
     def __init__(self):
@@ -62,6 +73,7 @@ class Hip:
         L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
         L.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        L.hipStreamWaitEvent.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
 
     def malloc(self, nbytes):
         p = ctypes.c_void_p()
@@ -100,6 +112,14 @@ def run(args):
     stream = L.ftx_default_stream()
     n = 1 << args.n
     seed = 0xF7C00001
+    schedules = None
+    if not args.no_autotune:
+        t0 = time.time()
+        table = ft.autotune(3)
+        schedules = {"picked_by": "ftx_autotune on this GPU", "seconds": round(time.time() - t0, 2),
+                     "classes": {"n<2^14": table[0], "n<2^18": table[1], "n<2^22": table[2], "n>=2^22": table[3]},
+                     "fields": "(seg1, segu, scan: log2 elements per lane; reduce: max blocks)"}
+        log("[bench] autotuned aggregation schedules in %.1fs: %s" % (schedules["seconds"], table))
 
     log("[bench] generating %d x 4 inputs" % n)
     t0 = time.time()
@@ -172,7 +192,13 @@ def run(args):
     dom = max(per, key=lambda k: per[k])
     ach = mults[dom] / (per[dom] * 1e-3) * PRODUCTS[dom]
     roofline = {"kernel": dom, "bound": "valu", "achieved": ach / 1e12, "peak": PEAK_MUL32 / 1e12,
-                "unit": "Tmul32/s", "frac": ach / PEAK_MUL32, "traffic": None}
+                "unit": "Tmul32/s", "frac": ach / PEAK_MUL32, "traffic": None,
+                "algorithmic_bytes": ALG_BYTES[dom] * (mults[dom] // (2 if dom == "mul_u32" else 1))}
+    if n == 1 << 22:
+        pm = PMC_N22[dom]
+        roofline["traffic"] = pm["read"] + pm["write"]
+        roofline["valu_busy"] = pm["valu_busy"]
+        roofline["counters"] = "profiles/scalarmult_n22_pmc_counters.md"
 
     cpu = None
     if not args.no_cpu:
@@ -189,6 +215,7 @@ def run(args):
     if "cfg4" in sections:
         configs["cfg4_serving"] = bench_cfg4(ft, key, seed, args)
     detail["configs"] = configs
+    detail["aggregation_schedules"] = schedules
     return {
         "metric": "scalar-mults/s (fixed-base, variable-base, ciphertext x u32) on 1 MI355X",
         "value": value, "unit": "scalar-mults/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -242,7 +269,13 @@ def bench_cfg0(ft, hip, key, seed):
     add_ms = T(lambda: (ft.check(L.ftx_add_dev(a1.h, b1.h, stream)), ft.check(L.ftx_add_dev(a2.h, b2.h, stream))))
     red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, a1.h, rscr.h, stream)),
                         ft.check(L.ftx_reduce_dev(out.h, a2.h, rscr.h, stream))))
-    return {"encrypt": {"ms": round(enc_ms, 4), "ciphertexts_per_s": 2 * n / (enc_ms * 1e-3),
+    # compression of the outputs (A, B, A+B: 6 arrays of n points), timed apart from the ops
+    enc_out = hip.malloc(n * 32)
+    cmp_ms = T(lambda: [ft.check(L.ftx_encode_dev(enc_out, x.h, stream)) for x in (a1, a2, b1, b2, a1, a2)])
+    hip.L.hipFree(enc_out)
+    return {"compress": {"ms": round(cmp_ms, 4), "points": 6 * n, "points_per_s": 6 * n / (cmp_ms * 1e-3),
+                         "note": "canonical 32-byte encodings, batch inversion; not part of step_ms"},
+            "encrypt": {"ms": round(enc_ms, 4), "ciphertexts_per_s": 2 * n / (enc_ms * 1e-3),
                         "valu_frac": 2 * n * (2 * PRODUCTS["fixed_base"]) / (enc_ms * 1e-3) / PEAK_MUL32},
             "ct_add": adds_entry(add_ms, 2 * n, 3 * POINT_BYTES),
             "reduce": adds_entry(red_ms, 2 * (n - 1), POINT_BYTES),
@@ -250,7 +283,9 @@ def bench_cfg0(ft, hip, key, seed):
 
 
 def bench_cfg2(ft, hip, key, seed, args):
-    """reduce 2^26 ct, inclusive scan 2^24 ct, segmented sum 2^24 ct / 2^19 segments."""
+    """reduce 2^26 ct, inclusive scan 2^24 ct, segmented sum 2^24 ct / 2^19
+    segments: each job alone on one stream, then all three at once on three
+    streams (the inputs of every job resident together)."""
     from libftcrypto_amd import workloads as W
     L = ft.lib()
     stream = L.ftx_default_stream()
@@ -258,35 +293,78 @@ def bench_cfg2(ft, hip, key, seed, args):
     rng = np.random.default_rng(seed + 2)
     nr, ns, nseg = 1 << args.cfg2_reduce_log2, 1 << args.cfg2_log2, 1 << (args.cfg2_log2 - 5)
     res = {}
-    log("[bench] cfg2: reduce over 2^%d ciphertexts" % args.cfg2_reduce_log2)
-    c1, c2 = ft.GeArray.alloc(nr), ft.GeArray.alloc(nr)
-    m = hip.upload(rng.integers(0, 256, size=nr, dtype=np.uint64))
-    r = hip.upload(ft.random_scalars(nr, seed, 20))
-    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
+    log("[bench] cfg2: encrypting 2^%d + 2 x 2^%d ciphertexts" % (args.cfg2_reduce_log2, args.cfg2_log2))
+
+    def make(n, sid):
+        c1, c2 = ft.GeArray.alloc(n), ft.GeArray.alloc(n)
+        m = hip.upload(rng.integers(0, 256, size=n, dtype=np.uint64))
+        r = hip.upload(ft.random_scalars(n, seed, sid))
+        ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
+        ft.check(L.ftx_synchronize(stream))
+        hip.L.hipFree(m)
+        hip.L.hipFree(r)
+        return c1, c2
+
+    r1, r2 = make(nr, 20)
+    g1, g2 = make(ns, 21)          # segmented-sum input
+    q1, q2 = make(ns, 22)          # scanned in place
     out = ft.GeArray.alloc(1)
     rscr = ft.GeArray.alloc(L.ftx_reduce_scratch_points(nr))
-    ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, c1.h, rscr.h, stream)),
-                    ft.check(L.ftx_reduce_dev(out.h, c2.h, rscr.h, stream))))
-    res["reduce"] = adds_entry(ms, 2 * (nr - 1), POINT_BYTES)
-    hip.L.hipFree(m); hip.L.hipFree(r)
-    del c1, c2
-    log("[bench] cfg2: scan and segmented sum over 2^%d ciphertexts" % args.cfg2_log2)
-    c1, c2 = ft.GeArray.alloc(ns), ft.GeArray.alloc(ns)
-    m = hip.upload(rng.integers(0, 256, size=ns, dtype=np.uint64))
-    r = hip.upload(ft.random_scalars(ns, seed, 21))
-    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
     off = W.lognormal_segments(nseg, ns, seed)
     d_off = hip.upload(off)
     s1, s2 = ft.GeArray.alloc(nseg), ft.GeArray.alloc(nseg)
-    scr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ns))
-    ms = T(lambda: (ft.check(L.ftx_segmented_sum_dev(s1.h, d_off, nseg, ns, None, c1.h, scr.h, stream)),
-                    ft.check(L.ftx_segmented_sum_dev(s2.h, d_off, nseg, ns, None, c2.h, scr.h, stream))))
-    res["segmented_sum"] = adds_entry(ms, 2 * (ns - nseg), (2 * ns * POINT_BYTES + (nseg + 1) * 8 + 2 * nseg * POINT_BYTES) / (2 * (ns - nseg)))
-    res["segmented_sum"]["segments"] = nseg
+    gscr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ns))
     sscr = ft.GeArray.alloc(L.ftx_scan_scratch_points(ns))
-    ms = T(lambda: (ft.check(L.ftx_scan_dev(c1.h, 0, sscr.h, stream)), ft.check(L.ftx_scan_dev(c2.h, 0, sscr.h, stream))))
-    res["scan"] = adds_entry(ms, 2 * (ns - 1), 2 * POINT_BYTES)
-    hip.L.hipFree(m); hip.L.hipFree(r); hip.L.hipFree(d_off)
+
+    def job_reduce(st):
+        ft.check(L.ftx_reduce_dev(out.h, r1.h, rscr.h, st))
+        ft.check(L.ftx_reduce_dev(out.h, r2.h, rscr.h, st))
+
+    def job_segsum(st):
+        ft.check(L.ftx_segmented_sum_dev(s1.h, d_off, nseg, ns, None, g1.h, gscr.h, st))
+        ft.check(L.ftx_segmented_sum_dev(s2.h, d_off, nseg, ns, None, g2.h, gscr.h, st))
+
+    def job_scan(st):
+        ft.check(L.ftx_scan_dev(q1.h, 0, sscr.h, st))
+        ft.check(L.ftx_scan_dev(q2.h, 0, sscr.h, st))
+
+    log("[bench] cfg2: each job alone")
+    ms_r = T(lambda: job_reduce(stream))
+    res["reduce"] = adds_entry(ms_r, 2 * (nr - 1), POINT_BYTES)
+    ms_g = T(lambda: job_segsum(stream))
+    seg_bytes = 2 * ns * POINT_BYTES + (nseg + 1) * 8 + 2 * nseg * POINT_BYTES
+    res["segmented_sum"] = adds_entry(ms_g, 2 * (ns - nseg), seg_bytes / (2 * (ns - nseg)))
+    res["segmented_sum"]["segments"] = nseg
+    ms_s = T(lambda: job_scan(stream))
+    res["scan"] = adds_entry(ms_s, 2 * (ns - 1), 2 * POINT_BYTES)
+
+    log("[bench] cfg2: all three at once on three streams")
+    sts = []
+    for _ in range(2):
+        p = ctypes.c_void_p()
+        ft.check(L.ftx_stream_create(ctypes.byref(p)))
+        sts.append(p)
+    ev = [hip.event() for _ in range(4)]
+
+    def together():
+        hip.L.hipEventRecord(ev[2], stream)
+        for st in sts:
+            hip.L.hipStreamWaitEvent(st, ev[2], 0)
+        job_reduce(stream)
+        job_segsum(sts[0])
+        job_scan(sts[1])
+        for st, e in zip(sts, ev[:2]):
+            hip.L.hipEventRecord(e, st)
+            hip.L.hipStreamWaitEvent(stream, e, 0)
+
+    ms_all = T(together)
+    adds = 2 * (nr - 1) + 2 * (ns - nseg) + 2 * (ns - 1)
+    res["all_three_concurrent"] = adds_entry(ms_all, adds)
+    res["all_three_concurrent"]["sum_of_alone_ms"] = round(ms_r + ms_g + ms_s, 4)
+    ft.check(L.ftx_synchronize(None))
+    for st in sts:
+        L.ftx_stream_destroy(st)
+    hip.L.hipFree(d_off)
     return res
 
 
@@ -377,9 +455,21 @@ def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
     t["mul_u32"] = (time.time() - t0) * 2          # two points per ciphertext
     total = sum(t.values())
     rate = 4 * samp / total
+    # one thread, as the reference's serial CPU loops run
+    s1 = max(64, samp >> 6)
+    P = O.Array.from_scalars(s_var[:s1], nt=nt)
+    t0 = time.time()
+    O.Array.from_scalars(k_fixed[:s1], nt=1)
+    t1 = time.time()
+    P.mul(k_var[:s1], nt=1)
+    t2 = time.time()
+    P.mul_u32(a_u32[:s1], nt=1)
+    t3 = time.time()
+    one = (t1 - t0) + (t2 - t1) + 2 * (t3 - t2)
     return {"value": rate, "unit": "scalar-mults/s", "cores": nt, "kind": "port",
             "sample": "%d of each of the N=2^k fixed-base, variable-base and u32 mults (ct = 2 point mults)" % samp,
-            "seconds": round(total, 2)}
+            "seconds": round(total, 2),
+            "one_thread": {"value": 4 * s1 / one, "sample": "%d of each kind" % s1, "seconds": round(one, 2)}}
 
 
 def main():
@@ -389,6 +479,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=22, help="log2 of the array length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default aggregation schedules")
     ap.add_argument("--headline-only", action="store_true", help="skip the configurations other than the N=2^22 scalar mults")
     ap.add_argument("--sections", default="cfg0,cfg2,cfg3,cfg4",
                     help="configurations measured after the headline step (comma list of cfg0,cfg2,cfg3,cfg4)")

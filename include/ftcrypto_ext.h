@@ -145,6 +145,24 @@ ft_error ftx_server_release(ftx_server server, uint64_t handle);
 ft_error ftx_server_read(ftx_server server, uint64_t handle, uint8_t *c1_enc, uint8_t *c2_enc);
 ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *batches, uint64_t *completed);
 
+/* ------------------------------------------------ aggregation schedules */
+
+/* Launch schedules of the reduce, scan and segmented sum, one per size class
+ * of the input length (class 0: n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).
+ * seg1 / segu / scan are log2 elements per lane (4..6) at segmented-sum
+ * level 1, its upper levels and the scan; reduce_blocks (1..4096) bounds the
+ * first reduce pass.  Results are the same under every schedule.
+ * ftx_autotune times every candidate for classes 0..max_class on the device
+ * (class 3 at n = 2^24, about 2.6 GB while it runs), keeps the fastest and
+ * writes the table to path unless it is NULL; ft_crypto_init loads the file
+ * named by the FTX_TUNE_FILE environment variable when set. */
+ft_error ftx_autotune(int max_class, const char *path);
+ft_error ftx_tune_save(const char *path);
+ft_error ftx_tune_load(const char *path);
+ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks);
+ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks);
+int ftx_tune_size_class(size_t n);
+
 /* ---------------------------------------------------------- inputs */
 
 /* n scalars uniform mod l: 64 bytes of SplitMix64(seed, stream) output per

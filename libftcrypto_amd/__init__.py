@@ -108,6 +108,12 @@ _SIGS = {
     "ftx_capture_end": (i32, [vp, vp]),
     "ftx_graph_launch": (i32, [vp, vp]),
     "ftx_graph_free": (None, [vp]),
+    "ftx_autotune": (i32, [i32, ctypes.c_char_p]),
+    "ftx_tune_save": (i32, [ctypes.c_char_p]),
+    "ftx_tune_load": (i32, [ctypes.c_char_p]),
+    "ftx_tune_get": (i32, [i32, vp, vp, vp, vp]),
+    "ftx_tune_set": (i32, [i32, i32, i32, i32, i32]),
+    "ftx_tune_size_class": (i32, [sz]),
     "ftx_random_scalars": (None, [vp, sz, u64, u64]),
     "ftx_scalar_reduce64": (None, [vp, vp, sz]),
 }
@@ -399,6 +405,40 @@ def scalar_reduce64(wide):
     out = np.zeros(n * 32, dtype=np.uint8)
     lib().ftx_scalar_reduce64(_p(out), _p(wide), n)
     return out.reshape(n, 32)
+
+
+def autotune(max_class=3, path=None):
+    """Time every reduce / scan / segmented-sum schedule on the device for
+    size classes 0..max_class, keep the fastest (and save them to path)."""
+    init()
+    check(lib().ftx_autotune(max_class, path.encode() if path else None), "ftx_autotune")
+    return tune_table()
+
+
+def tune_table():
+    """{class: (seg1, segu, scan, reduce_blocks)} of the schedules in use."""
+    out = {}
+    for c in range(4):
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().ftx_tune_get(c, *[ctypes.byref(x) for x in v]), "ftx_tune_get")
+        out[c] = tuple(x.value for x in v)
+    return out
+
+
+def tune_set(size_class, seg1, segu, scan, reduce_blocks):
+    check(lib().ftx_tune_set(size_class, seg1, segu, scan, reduce_blocks), "ftx_tune_set")
+
+
+def tune_load(path):
+    check(lib().ftx_tune_load(path.encode()), "ftx_tune_load")
+
+
+def tune_save(path):
+    check(lib().ftx_tune_save(path.encode()), "ftx_tune_save")
+
+
+def size_class(n):
+    return lib().ftx_tune_size_class(n)
 
 
 def error_str(code):

@@ -1,9 +1,11 @@
 // Scan and CSR segmented-sum kernels for gfx950 (combine = point addition).
 //
-// Both split the input into fixed chunks of 16 consecutive elements, one
+// Both split the input into fixed chunks of 2^CH consecutive elements, one
 // chunk per lane, so the work per lane is the same whatever the segment
 // lengths are (the reference runs one thread per segment, ftcrypto.cu:1116-1127,
-// and a recursive Blelloch scan, :1474-1562).
+// and a recursive Blelloch scan, :1474-1562).  The chunk size of each kernel
+// is a template parameter in [2^4, 2^6]; the schedule used for an input size
+// comes from the tuning table (tune.hip).
 //
 // Scan: level sums of every chunk are scanned recursively (exclusive), then
 // every lane walks its chunk once more starting from its chunk's prefix.
@@ -16,24 +18,24 @@
 // covers slots [2 c_first + 1, 2 c_last + 1) -- a range computed from the
 // offsets alone, which decides whether a run of s is complete -- and level 2
 // runs the same chunked walk over the records, until one chunk holds all.
+// Level 1 and the upper levels may use different chunk sizes.
 #include "kernels.h"
 
 namespace ft {
 
 constexpr int ATPB = 256;
-constexpr int CH = SEG_CHUNK_LOG2;
-constexpr int CHN = 1 << CH;
+
+static size_t tiles(size_t n, int ch) { return (n + ((size_t)1 << ch) - 1) >> ch; }
 
 // ------------------------------------------------------------------ scan
 
-size_t scan_tiles(size_t n) { return (n + CHN - 1) >> CH; }
-
-// sums[t] = sum of elements [16 t, 16 t + 16), read a cache line at a time
+// sums[t] = sum of elements [2^CH t, 2^CH (t + 1)), read a cache line at a time
+template <int CH>
 __global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size_t n) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const size_t lo = t << CH;
   if (lo >= n) return;
-  const size_t hi = (lo + CHN < n) ? lo + CHN : n;
+  const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
   ge_p3 acc, x;
   ge_p3_0(acc);
 #pragma unroll 1
@@ -60,12 +62,15 @@ __device__ __forceinline__ void scan_step(ge_p3 &run, PW &q, int exclusive) {
 }
 
 // walk chunk t from its exclusive prefix (identity when pre.w is null); the
-// four outputs of a group are written together, whole lines per coordinate
-__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
+// four outputs of a group are written together, whole lines per coordinate.
+// Two waves per SIMD (a few spilled words) beat one wave without spills.
+template <int CH>
+__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const size_t lo = t << CH;
   if (lo >= n) return;
-  const size_t hi = (lo + CHN < n) ? lo + CHN : n;
+  const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
   ge_p3 run;
   if (pre.w) load_p3(run, pre, t);
   else ge_p3_0(run);
@@ -84,65 +89,84 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   }
 }
 
+static hipError_t launch_chunk_sums(PtView sums, PtView v, size_t n, int ch, hipStream_t s) {
+  const size_t T = tiles(n, ch);
+  if (!T) return hipSuccess;
+  const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
+  switch (ch) {
+    case 4: hipLaunchKernelGGL(k_chunk_sums<4>, grid, dim3(ATPB), 0, s, sums, v, n); break;
+    case 5: hipLaunchKernelGGL(k_chunk_sums<5>, grid, dim3(ATPB), 0, s, sums, v, n); break;
+    case 6: hipLaunchKernelGGL(k_chunk_sums<6>, grid, dim3(ATPB), 0, s, sums, v, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+static hipError_t launch_chunk_scan(PtView v, size_t n, PtView pre, bool exclusive, int ch, hipStream_t s) {
+  const size_t T = tiles(n, ch);
+  if (!T) return hipSuccess;
+  const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
+  switch (ch) {
+    case 4: hipLaunchKernelGGL(k_chunk_scan<4>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 5: hipLaunchKernelGGL(k_chunk_scan<5>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 6: hipLaunchKernelGGL(k_chunk_scan<6>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// chunk sums of every level, for the smallest chunk (the most sums)
 size_t scan_scratch_points(size_t n) {
   size_t tot = 0;
-  for (size_t m = scan_tiles(n); m > 1; m = scan_tiles(m)) tot += m;
+  for (size_t m = tiles(n, CH_MIN); m > 1; m = tiles(m, CH_MIN)) tot += m;
   return tot + 1;
 }
 
-// inclusive / exclusive scan of n points in place; chunk sums of every level
-// live in scratch (scan_scratch_points(n) points), no allocation, no sync
-hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s) {
+// inclusive / exclusive scan of n points in place with chunks of 2^ch;
+// chunk sums of every level live in scratch, no allocation, no sync
+hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s) {
   if (n == 0) return hipSuccess;
+  if (ch < CH_MIN || ch > CH_MAX) return hipErrorInvalidValue;
   PtView lv[48];
   size_t ln[48];
   int L = 0;
   lv[0] = v;
   ln[0] = n;
   size_t used = 0;
-  while (scan_tiles(ln[L]) > 1) {
-    const size_t m = scan_tiles(ln[L]);
+  while (tiles(ln[L], ch) > 1) {
+    const size_t m = tiles(ln[L], ch);
     lv[L + 1] = PtView{scratch.w + used * 8, scratch.cap};
     ln[L + 1] = m;
     used += m;
-    hipError_t e = launch_scan_tile_sums(lv[L + 1], lv[L], ln[L], s);
+    hipError_t e = launch_chunk_sums(lv[L + 1], lv[L], ln[L], ch, s);
     if (e != hipSuccess) return e;
     ++L;
   }
-  hipError_t e = launch_scan_apply(lv[L], ln[L], PtView{nullptr, 0}, L > 0 ? true : exclusive, s);
+  hipError_t e = launch_chunk_scan(lv[L], ln[L], PtView{nullptr, 0}, L > 0 ? true : exclusive, ch, s);
   for (int l = L - 1; l >= 0 && e == hipSuccess; --l)
-    e = launch_scan_apply(lv[l], ln[l], lv[l + 1], l > 0 ? true : exclusive, s);
+    e = launch_chunk_scan(lv[l], ln[l], lv[l + 1], l > 0 ? true : exclusive, ch, s);
   return e;
 }
 
-hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s) {
-  const size_t T = scan_tiles(n);
-  if (!T) return hipSuccess;
-  hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)((T + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, sums, v, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_apply(PtView v, size_t n, PtView pre, bool exclusive, hipStream_t s) {
-  const size_t T = scan_tiles(n);
-  if (!T) return hipSuccess;
-  hipLaunchKernelGGL(k_chunk_scan, dim3((unsigned)((T + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, v, n, pre,
-                     (int)exclusive);
-  return hipGetLastError();
+hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s) {
+  return scan_launch_ch(v, n, exclusive, scratch, tune_for(n).scan, s);
 }
 
 // --------------------------------------------------------- segmented sum
 
-// range of segment s in the element space of `level` (1 = the input)
-__device__ __forceinline__ void seg_range(const int64_t *off, size_t s, int level, int64_t &lo,
-                                          int64_t &hi) {
+// range of segment s in the element space of `level` (1 = the input); the
+// step from level l to l + 1 uses chunks of 2^ch1 (l = 1) or 2^chu
+__device__ __forceinline__ void seg_range(const int64_t *off, size_t s, int level, int ch1, int chu,
+                                          int64_t &lo, int64_t &hi) {
   lo = off[s];
   hi = off[s + 1];
 #pragma unroll 1
   for (int l = 1; l < level; ++l) {
-    const int64_t c0 = lo >> CH;
-    if (hi > lo && c0 != ((hi - 1) >> CH)) {
+    const int ch = (l == 1) ? ch1 : chu;
+    const int64_t c0 = lo >> ch;
+    if (hi > lo && c0 != ((hi - 1) >> ch)) {
       lo = 2 * c0 + 1;
-      hi = 2 * ((hi - 1) >> CH) + 1;
+      hi = 2 * ((hi - 1) >> ch) + 1;
     } else {
       lo = hi = 2 * c0 + 1;
     }
@@ -173,15 +197,16 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
   }
 }
 
-// Level 1: lane t walks elements [16 t, 16 t + 16) of the input (through the
-// gather indices when given), segment boundaries from the offsets.
+// Level 1: lane t walks elements [2^CH t, 2^CH (t + 1)) of the input (through
+// the gather indices when given), segment boundaries from the offsets.
+template <int CH>
 __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
                                                      size_t n, const int64_t *off, size_t nseg,
                                                      const int32_t *gather) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
-  const int64_t chi = clo + CHN;
+  const int64_t chi = clo + (1 << CH);
   const int64_t cend = chi < (int64_t)n ? chi : (int64_t)n;
   if (rec.w) {
     store_identity(rec, 2 * t);
@@ -190,8 +215,11 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
     keys[2 * t + 1] = -1;
   }
   size_t s = seg_find1(off, 0, nseg, clo);
-  int64_t lo = 0, hi = 0;
+  // [lo, hi) = segment s; nhi = end of segment s + 1, loaded one crossing
+  // ahead so that stepping to the next segment waits on no load
+  int64_t lo = 0, hi = 0, nhi = INT64_MAX;
   if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
+  if (s + 1 < nseg) nhi = off[s + 2];
   ge_p3 acc, x;
   ge_p3_0(acc);
   bool touched = false;
@@ -200,11 +228,14 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
     if (s < nseg && hi <= i) {
       if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
       touched = false;
-      // next segment containing i: a few linear steps, else binary search
-      int steps = 0;
-      do { ++s; ++steps; } while (s < nseg && off[s + 1] <= i && steps < 4);
-      if (s < nseg && off[s + 1] <= i) s = seg_find1(off, s, nseg, i);
-      if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
+      ++s;
+      lo = hi;
+      hi = nhi;
+      if (s < nseg && hi <= i) {                 // empty segments: search
+        s = seg_find1(off, s, nseg, i);
+        if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
+      }
+      nhi = (s + 1 < nseg) ? off[s + 2] : INT64_MAX;
     }
     return s < nseg && lo <= i;
   };
@@ -214,9 +245,16 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
     touched = true;
   };
   if (gather) {
+    // one element ahead: the next source point is in flight during the add
+    PW qc, qn;
+    ld_pw(qc, src, (size_t)gather[clo]);
 #pragma unroll 1
-    for (int64_t i = clo; i < cend; ++i)
-      if (take(i)) { load_p3(x, src, (size_t)gather[i]); accum(); }
+    for (int64_t i = clo; i < cend; ++i) {
+      ld_pw(qn, src, (size_t)gather[(i + 1 < cend) ? i + 1 : i]);
+      pin_pw(qn);
+      if (take(i)) { un_pw(x, qc); accum(); }
+      qc = qn;
+    }
   } else {
 #pragma unroll 1
     for (int64_t g = clo; g < cend; g += 4) {
@@ -231,16 +269,18 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
   if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
 }
 
-// Level L >= 2: lane t walks records [16 t, 16 t + 16) of level L-1, whose
-// segment keys are sorted (-1 marks an unused slot holding the identity);
-// a run is complete here when the segment's level-L range lies in the chunk.
+// Level L >= 2: lane t walks records [2^CH t, 2^CH (t + 1)) of level L-1,
+// whose segment keys are sorted (-1 marks an unused slot holding the
+// identity); a run is complete here when the segment's level-L range lies
+// in the chunk.
+template <int CH>
 __global__ void __launch_bounds__(ATPB) k_seg_levelN(PtView out, PtView rec, int64_t *keys, PtView in,
                                                      const int64_t *in_keys, size_t n, const int64_t *off,
-                                                     int level) {
+                                                     int level, int ch1) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
-  const int64_t chi = clo + CHN;
+  const int64_t chi = clo + (1 << CH);
   const int64_t cend = chi < (int64_t)n ? chi : (int64_t)n;
   if (rec.w) {
     store_identity(rec, 2 * t);
@@ -258,7 +298,7 @@ __global__ void __launch_bounds__(ATPB) k_seg_levelN(PtView out, PtView rec, int
     if (k != cur) {
       if (cur >= 0) {
         int64_t lo, hi;
-        seg_range(off, (size_t)cur, level, lo, hi);
+        seg_range(off, (size_t)cur, level, ch1, CH, lo, hi);
         seg_flush(out, rec, keys, (size_t)cur, lo, hi, clo, chi, t, acc);
       }
       cur = k;
@@ -270,7 +310,7 @@ __global__ void __launch_bounds__(ATPB) k_seg_levelN(PtView out, PtView rec, int
   }
   if (cur >= 0) {
     int64_t lo, hi;
-    seg_range(off, (size_t)cur, level, lo, hi);
+    seg_range(off, (size_t)cur, level, ch1, CH, lo, hi);
     seg_flush(out, rec, keys, (size_t)cur, lo, hi, clo, chi, t, acc);
   }
 }
@@ -280,42 +320,57 @@ __global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *o
   if (s < nseg && off[s] == off[s + 1]) store_identity(out, s);
 }
 
-hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
-                            size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
-                            hipStream_t s) {
-  const size_t T = (n + CHN - 1) >> CH;
+static hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
+                                   size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
+                                   int ch, int ch1, hipStream_t s) {
+  const size_t T = tiles(n, ch);
   if (!T || !nseg) return hipSuccess;
   const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
-  if (level == 1)
-    hipLaunchKernelGGL(k_seg_level1, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather);
-  else
-    hipLaunchKernelGGL(k_seg_levelN, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level);
+  if (level == 1) {
+    switch (ch) {
+      case 4: hipLaunchKernelGGL(k_seg_level1<4>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather); break;
+      case 5: hipLaunchKernelGGL(k_seg_level1<5>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather); break;
+      case 6: hipLaunchKernelGGL(k_seg_level1<6>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (ch) {
+      case 4: hipLaunchKernelGGL(k_seg_levelN<4>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
+      case 5: hipLaunchKernelGGL(k_seg_levelN<5>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
+      case 6: hipLaunchKernelGGL(k_seg_levelN<6>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
   return hipGetLastError();
 }
 
-size_t segsum_records(size_t n) {
-  const size_t t1 = (n + 15) / 16, t2 = (2 * t1 + 15) / 16;
+// records of level 1 (buffer A, reused by levels 3, 5, ...) and level 2
+// (buffer B, reused by levels 4, 6, ...): two per chunk
+static size_t segsum_records(size_t n, int ch1, int chu) {
+  const size_t t1 = tiles(n, ch1), t2 = tiles(2 * t1, chu);
   return 2 * t1 + 2 * t2;
 }
 
-// Scratch of the segmented sum, as points of one array: level-1 records
-// (2 per 16-element chunk) and level-2 records in plane slots [0, R), their
-// int64 segment keys in plane-0 slots [R, R + R/4).
+// Scratch of the segmented sum, as points of one array: records in plane
+// slots [0, R), their int64 segment keys in plane-0 slots [R, R + R/4), R
+// for the smallest chunks (the most records) so that any schedule fits.
 size_t segsum_scratch_points(size_t n) {
-  const size_t R = segsum_records(n);
+  const size_t R = segsum_records(n, CH_MIN, CH_MIN);
   return R + R / 4 + 8;
 }
 
-hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
-                         PtView src, PtView scratch, hipStream_t s) {
+hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
+                            const int32_t *gather, PtView src, PtView scratch, int ch1, int chu,
+                            hipStream_t s) {
   if (nseg == 0) return hipSuccess;
+  if (ch1 < CH_MIN || ch1 > CH_MAX || chu < CH_MIN || chu > CH_MAX) return hipErrorInvalidValue;
   hipError_t e = launch_seg_empty(out, d_off, nseg, s);
   if (e != hipSuccess) return e;
-  const size_t t1 = (n_elems + 15) / 16;
-  const size_t R = segsum_records(n_elems);
+  const size_t t1 = tiles(n_elems, ch1);
+  const size_t Rmax = segsum_records(n_elems, CH_MIN, CH_MIN);
   PtView bufA{scratch.w, scratch.cap};
   PtView bufB{scratch.w + 2 * t1 * 8, scratch.cap};     // same planes, later element slots
-  int64_t *keysA = reinterpret_cast<int64_t *>(scratch.w + R * 8);
+  int64_t *keysA = reinterpret_cast<int64_t *>(scratch.w + Rmax * 8);
   int64_t *keysB = keysA + 2 * t1;
   PtView elems = src;
   const int64_t *in_keys = nullptr;
@@ -323,11 +378,12 @@ hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n
   int level = 1;
   bool use_a = true;
   while (n > 0) {
-    const size_t T = (n + 15) / 16;
+    const int ch = (level == 1) ? ch1 : chu;
+    const size_t T = tiles(n, ch);
     const bool last = T <= 1;
     PtView rec = last ? PtView{nullptr, 0} : (use_a ? bufA : bufB);
     int64_t *keys = use_a ? keysA : keysB;
-    e = launch_seg_level(out, rec, keys, elems, in_keys, n, d_off, nseg, gather, level, s);
+    e = launch_seg_level(out, rec, keys, elems, in_keys, n, d_off, nseg, gather, level, ch, ch1, s);
     if (e != hipSuccess || last) return e;
     elems = rec;
     in_keys = keys;
@@ -336,6 +392,12 @@ hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n
     use_a = !use_a;
   }
   return hipSuccess;
+}
+
+hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
+                         PtView src, PtView scratch, hipStream_t s) {
+  const AggTune t = tune_for(n_elems);
+  return segsum_launch_ch(out, d_off, nseg, n_elems, gather, src, scratch, t.seg1, t.segu, s);
 }
 
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s) {

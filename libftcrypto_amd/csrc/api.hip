@@ -157,19 +157,26 @@ struct ArrGuard {
   ft_ge25519_array release() { auto r = a; a = nullptr; return r; }
 };
 
-// out[0] = sum of src; part (reduce_grid(n) points) holds the block sums
-ft_error reduce_run(ft_ge25519_array out, const_ft_ge25519_array src, PtView part, hipStream_t s) {
+// out[0] = sum of src; part (reduce_scratch(n) points) holds the block sums
+size_t reduce_scratch(size_t n) { return reduce_grid(n, REDUCE_BLOCKS_MAX); }
+
+ft_error reduce_run_blocks(ft_ge25519_array out, const_ft_ge25519_array src, PtView part, unsigned max_blocks,
+                           hipStream_t s) {
   const size_t n = src->n;
   if (n == 0) { CK(launch_set_identity(view(out), 0, 1, s)); return FT_ERR_NO_ERROR; }
-  const unsigned g1 = reduce_grid(n);
+  const unsigned g1 = reduce_grid(n, max_blocks);
   CK(launch_reduce_blocks(part, view(src), n, g1, s));
   CK(launch_reduce_blocks(view(out), part, g1, 1, s));
   return FT_ERR_NO_ERROR;
 }
 
+ft_error reduce_run(ft_ge25519_array out, const_ft_ge25519_array src, PtView part, hipStream_t s) {
+  return reduce_run_blocks(out, src, part, (unsigned)tune_for(src->n).reduce, s);
+}
+
 ft_error reduce_into(ft_ge25519_array out, const_ft_ge25519_array src, hipStream_t s) {
   ArrGuard part;
-  CKE(alloc_array(&part.a, reduce_grid(src->n)));
+  CKE(alloc_array(&part.a, reduce_scratch(src->n)));
   CKE(reduce_run(out, src, view(part.a), s));
   CK(hipStreamSynchronize(s));
   return FT_ERR_NO_ERROR;
@@ -340,6 +347,7 @@ ft_error ft_crypto_init(void) {
   CK(hipMemcpyAsync(G.lwords, L_W, 32, hipMemcpyHostToDevice, G.stream));
   CK(launch_build_comb(G.btab, PtView{nullptr, 0}, 0, true, G.stream));
   CK(hipStreamSynchronize(G.stream));
+  tune_init_from_env();
   G.init = true;
   return FT_ERR_NO_ERROR;
 }
@@ -977,13 +985,13 @@ ft_error ftx_gather_sum(ft_ge25519_array *res, const int64_t *row_off, const int
   return FT_ERR_NO_ERROR;
 }
 
-size_t ftx_reduce_scratch_points(size_t n) { return reduce_grid(n); }
+size_t ftx_reduce_scratch_points(size_t n) { return reduce_scratch(n); }
 size_t ftx_scan_scratch_points(size_t n) { return scan_scratch_points(n); }
 
 ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream) {
   if (out->n < 1) return FT_ERR_INSUFFICIENT_SPACE;
   if (!scratch) return reduce_into(out, src, S(stream));
-  if (scratch->cap < reduce_grid(src->n)) return FT_ERR_INSUFFICIENT_SPACE;
+  if (scratch->cap < reduce_scratch(src->n)) return FT_ERR_INSUFFICIENT_SPACE;
   return reduce_run(out, src, view(scratch), S(stream));
 }
 

@@ -20,23 +20,36 @@ hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint
                           hipStream_t s);
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s);
 hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long long *bad, hipStream_t s);
-unsigned reduce_grid(size_t n);
+unsigned reduce_grid(size_t n, unsigned max_blocks);
 hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, hipStream_t s);
 
+// tune.hip: launch schedules of the aggregation kernels, per size class of n
+// (n < 2^14, < 2^18, < 2^22, larger), picked by measurement (ftx_autotune)
+// or left at the defaults.
+constexpr int TUNE_CLASSES = 4;
+struct AggTune {
+  int seg1;          // log2 elements per lane, segmented-sum level 1
+  int segu;          // log2 records per lane, segmented-sum upper levels
+  int scan;          // log2 elements per lane, every scan level
+  int reduce;        // most blocks of 256 lanes in the first reduce pass
+};
+constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
+constexpr unsigned REDUCE_BLOCKS_MAX = 4096;
+int tune_class(size_t n);
+AggTune tune_for(size_t n);
+void tune_init_from_env();
+
 // aggregate.hip
-constexpr int SEG_CHUNK_LOG2 = 4;                     // elements per lane per level
-size_t scan_tiles(size_t n);
-size_t scan_scratch_points(size_t n);
+size_t scan_scratch_points(size_t n);             // enough for every schedule
 hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s);
-hipError_t launch_scan_tile_sums(PtView sums, PtView v, size_t n, hipStream_t s);
-hipError_t launch_scan_apply(PtView v, size_t n, PtView tile_prefix, bool exclusive, hipStream_t s);
-hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
-                            size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
-                            hipStream_t s);
+hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s);
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s);
-size_t segsum_scratch_points(size_t n_elems);
+size_t segsum_scratch_points(size_t n_elems);     // enough for every schedule
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
                          PtView src, PtView scratch, hipStream_t s);
+hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
+                            const int32_t *gather, PtView src, PtView scratch, int ch1, int chu,
+                            hipStream_t s);
 
 // api.hip: device tables owned by the library
 const uint32_t *base_table();

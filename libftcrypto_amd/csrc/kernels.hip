@@ -357,9 +357,9 @@ hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long l
   FT_LAUNCH_CHECK();
 }
 
-unsigned reduce_grid(size_t n) {
+unsigned reduce_grid(size_t n, unsigned max_blocks) {
   const size_t g = (n + TPB - 1) / TPB;
-  return (unsigned)(g < 2048 ? (g ? g : 1) : 2048);
+  return (unsigned)(g < max_blocks ? (g ? g : 1) : max_blocks);
 }
 
 hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, hipStream_t s) {

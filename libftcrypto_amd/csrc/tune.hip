@@ -1,0 +1,295 @@
+// Launch schedules of the aggregation kernels and the autotuner that picks
+// them on the device at hand.
+//
+// A schedule is chosen per size class of the input length n (n < 2^14,
+// < 2^18, < 2^22, larger):
+//   seg1    log2 elements per lane at segmented-sum level 1 (4..6)
+//   segu    log2 records per lane at the upper segmented-sum levels (4..6)
+//   scan    log2 elements per lane at every scan level (4..6)
+//   reduce  most blocks of 256 lanes in the first reduce pass (512..4096)
+// Every schedule gives the same group element (point addition is
+// associative and commutative), so the choice only moves time.  Scratch
+// sizes are those of the schedule needing the most, so buffers sized by
+// ftx_*_scratch_points stay valid whatever is picked.
+//
+// ftx_autotune times every candidate of every class on synthetic points
+// (s_i B for small s_i, segments of mean length 32) with HIP events, keeps
+// the fastest and can write the table to a text file; ftx_tune_load reads
+// such a file back, and ft_crypto_init loads the file named by the
+// FTX_TUNE_FILE environment variable when it is set.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ftcrypto.h"
+#include "../../include/ftcrypto_ext.h"
+#include "kernels.h"
+
+namespace ft {
+
+namespace {
+
+constexpr AggTune DEFAULT_TUNE = {4, 4, 4, 2048};
+
+std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES];
+std::atomic<bool> T_ready{false};
+
+void tune_defaults() {
+  for (int c = 0; c < TUNE_CLASSES; ++c) {
+    T_seg1[c] = DEFAULT_TUNE.seg1;
+    T_segu[c] = DEFAULT_TUNE.segu;
+    T_scan[c] = DEFAULT_TUNE.scan;
+    T_reduce[c] = DEFAULT_TUNE.reduce;
+  }
+  T_ready = true;
+}
+
+bool valid(const AggTune &t) {
+  auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
+  return ch_ok(t.seg1) && ch_ok(t.segu) && ch_ok(t.scan) && t.reduce >= 1 &&
+         (unsigned)t.reduce <= REDUCE_BLOCKS_MAX;
+}
+
+void tune_store(int c, const AggTune &t) {
+  T_seg1[c] = t.seg1;
+  T_segu[c] = t.segu;
+  T_scan[c] = t.scan;
+  T_reduce[c] = t.reduce;
+}
+
+}  // namespace
+
+int tune_class(size_t n) {
+  if (n < ((size_t)1 << 14)) return 0;
+  if (n < ((size_t)1 << 18)) return 1;
+  if (n < ((size_t)1 << 22)) return 2;
+  return 3;
+}
+
+AggTune tune_for(size_t n) {
+  if (!T_ready) tune_defaults();
+  const int c = tune_class(n);
+  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c]};
+}
+
+}  // namespace ft
+
+using namespace ft;
+
+namespace {
+
+#define TCK(x)                                              \
+  do {                                                      \
+    hipError_t e_ = (x);                                    \
+    if (e_ != hipSuccess) {                                 \
+      (void)hipGetLastError();                              \
+      return e_ == hipErrorOutOfMemory ? FT_ERR_OUT_OF_MEMORY : FT_ERR_INTERNAL_LIBRARY_ERROR; \
+    }                                                       \
+  } while (0)
+
+struct DevMem {
+  void *p = nullptr;
+  ~DevMem() { if (p) (void)hipFree(p); }
+};
+
+struct Events {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~Events() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+struct Stream {
+  hipStream_t s = nullptr;
+  ~Stream() { if (s) (void)hipStreamDestroy(s); }
+};
+
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// average milliseconds of `reps` launches of fn after one warm-up launch
+template <class F>
+ft_error time_ms(float *ms, const Events &ev, hipStream_t s, int reps, F fn) {
+  TCK(fn());
+  TCK(hipEventRecord(ev.a, s));
+  for (int r = 0; r < reps; ++r) TCK(fn());
+  TCK(hipEventRecord(ev.b, s));
+  TCK(hipEventSynchronize(ev.b));
+  float t = 0;
+  TCK(hipEventElapsedTime(&t, ev.a, ev.b));
+  *ms = t / reps;
+  return FT_ERR_NO_ERROR;
+}
+
+}  // namespace
+
+extern "C" {
+
+ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks) {
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  if (!T_ready) tune_defaults();
+  *seg1 = T_seg1[size_class];
+  *segu = T_segu[size_class];
+  *scan = T_scan[size_class];
+  *reduce_blocks = T_reduce[size_class];
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks) {
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  const AggTune t{seg1, segu, scan, reduce_blocks};
+  if (!valid(t)) return FT_ERR_BAD_SIZE;
+  if (!T_ready) tune_defaults();
+  tune_store(size_class, t);
+  return FT_ERR_NO_ERROR;
+}
+
+int ftx_tune_size_class(size_t n) { return tune_class(n); }
+
+// Text format: a comment line, then one line per class:
+// "class seg1 segu scan reduce".
+ft_error ftx_tune_save(const char *path) {
+  FILE *f = std::fopen(path, "w");
+  if (!f) return FT_ERR_INTERNAL_LIBRARY_ERROR;
+  if (!T_ready) tune_defaults();
+  std::fprintf(f, "# libftcrypto_amd aggregation schedules: class seg1 segu scan reduce\n");
+  for (int c = 0; c < TUNE_CLASSES; ++c)
+    std::fprintf(f, "%d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c], (int)T_reduce[c]);
+  return std::fclose(f) == 0 ? FT_ERR_NO_ERROR : FT_ERR_INTERNAL_LIBRARY_ERROR;
+}
+
+ft_error ftx_tune_load(const char *path) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return FT_ERR_OUT_OF_BOUNDS;
+  AggTune t[TUNE_CLASSES];
+  bool seen[TUNE_CLASSES] = {};
+  char line[256];
+  ft_error err = FT_ERR_NO_ERROR;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    int c, a, b, d, r;
+    if (std::sscanf(line, "%d %d %d %d %d", &c, &a, &b, &d, &r) != 5 || c < 0 || c >= TUNE_CLASSES) {
+      err = FT_ERR_BAD_SIZE;
+      break;
+    }
+    t[c] = AggTune{a, b, d, r};
+    if (!valid(t[c])) { err = FT_ERR_BAD_SIZE; break; }
+    seen[c] = true;
+  }
+  std::fclose(f);
+  if (err != FT_ERR_NO_ERROR) return err;
+  for (int c = 0; c < TUNE_CLASSES; ++c)
+    if (!seen[c]) return FT_ERR_BAD_SIZE;
+  if (!T_ready) tune_defaults();
+  for (int c = 0; c < TUNE_CLASSES; ++c) tune_store(c, t[c]);
+  return FT_ERR_NO_ERROR;
+}
+
+// Measure every candidate schedule of every size class (classes up to
+// max_class, 0..3) and keep the fastest; write the table to path when it is
+// not NULL.  Needs ft_crypto_init; allocates up to ~2.6 GB of device memory
+// while it runs (class 3 is timed at n = 2^24).
+ft_error ftx_autotune(int max_class, const char *path) {
+  if (max_class < 0 || max_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  if (!base_table()) return FT_ERR_INSUFFICIENT_DEVICE_OR_DRIVER;
+  if (!T_ready) tune_defaults();
+  static const int LOGN[TUNE_CLASSES] = {12, 16, 20, 24};
+  const size_t nmax = (size_t)1 << LOGN[max_class];
+  Stream st;
+  TCK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+  Events ev;
+  TCK(hipEventCreate(&ev.a));
+  TCK(hipEventCreate(&ev.b));
+  // points s_i B, s_i = i + 1; scratch for the segmented sum, scan and reduce
+  DevMem pts, cpy, out, scr, sc;
+  const size_t seg_pts = segsum_scratch_points(nmax);
+  const size_t scan_pts = scan_scratch_points(nmax);
+  const size_t red_pts = REDUCE_BLOCKS_MAX;
+  const size_t scr_pts = std::max(seg_pts, std::max(scan_pts, red_pts));
+  TCK(hipMalloc(&pts.p, nmax * 128));
+  TCK(hipMalloc(&cpy.p, nmax * 128));
+  TCK(hipMalloc(&scr.p, scr_pts * 128));
+  TCK(hipMalloc(&sc.p, nmax * 8));
+  std::vector<uint64_t> small(nmax);
+  for (size_t i = 0; i < nmax; ++i) small[i] = i + 1;
+  TCK(hipMemcpyAsync(sc.p, small.data(), nmax * 8, hipMemcpyHostToDevice, st.s));
+  const PtView P{(uint32_t *)pts.p, nmax};
+  TCK(launch_fixed_base(P, sc.p, true, nmax, base_table(), st.s));
+  TCK(hipStreamSynchronize(st.s));
+  // segment lengths 1 + floor(-31 ln u): mean 32, a long tail
+  std::vector<int64_t> off;
+  off.reserve(nmax / 16);
+  off.push_back(0);
+  for (uint64_t k = 0; off.back() < (int64_t)nmax; ++k) {
+    const double u = ((mix64(0xF7C0AB00ull + k) >> 11) + 1) * (1.0 / 9007199254740993.0);
+    int64_t len = 1 + (int64_t)std::floor(-31.0 * std::log(u));
+    off.push_back(std::min<int64_t>(off.back() + len, (int64_t)nmax));
+  }
+  TCK(hipMalloc(&out.p, off.size() * 128));          // at least one slot per segment
+  TCK(hipMemcpyAsync(cpy.p, pts.p, nmax * 128, hipMemcpyDeviceToDevice, st.s));
+  const PtView C{(uint32_t *)cpy.p, nmax};
+  const PtView S{(uint32_t *)scr.p, scr_pts};
+  for (int c = 0; c <= max_class; ++c) {
+    const size_t n = (size_t)1 << LOGN[c];
+    const int reps = c < 2 ? 10 : 3;
+    // segments covering [0, n): the prefix of the offsets, last one clipped
+    size_t nseg = 0;
+    while (nseg + 1 < off.size() && off[nseg] < (int64_t)n) ++nseg;
+    std::vector<int64_t> offc(off.begin(), off.begin() + nseg + 1);
+    offc[nseg] = (int64_t)n;
+    DevMem doffc;
+    TCK(hipMalloc(&doffc.p, offc.size() * 8));
+    TCK(hipMemcpyAsync(doffc.p, offc.data(), offc.size() * 8, hipMemcpyHostToDevice, st.s));
+    const PtView O{(uint32_t *)out.p, off.size()};
+    AggTune best = tune_for(n);
+    float bt = INFINITY, t = 0;
+    for (int a = CH_MIN; a <= CH_MAX; ++a)
+      for (int b = CH_MIN; b <= CH_MAX; ++b) {
+        ft_error e = time_ms(&t, ev, st.s, reps, [&] {
+          return segsum_launch_ch(O, (const int64_t *)doffc.p, nseg, n, nullptr, P, S, a, b, st.s);
+        });
+        if (e != FT_ERR_NO_ERROR) return e;
+        if (t < bt) { bt = t; best.seg1 = a; best.segu = b; }
+      }
+    bt = INFINITY;
+    for (int a = CH_MIN; a <= CH_MAX; ++a) {
+      // in place on a copy of the points (any points time the same)
+      ft_error e = time_ms(&t, ev, st.s, reps, [&] { return scan_launch_ch(C, n, false, S, a, st.s); });
+      if (e != FT_ERR_NO_ERROR) return e;
+      if (t < bt) { bt = t; best.scan = a; }
+    }
+    bt = INFINITY;
+    for (unsigned blocks = 512; blocks <= REDUCE_BLOCKS_MAX; blocks *= 2) {
+      ft_error e = time_ms(&t, ev, st.s, reps, [&] {
+        const unsigned g1 = reduce_grid(n, blocks);
+        hipError_t x = launch_reduce_blocks(S, P, n, g1, st.s);
+        if (x != hipSuccess) return x;
+        return launch_reduce_blocks(O, S, g1, 1, st.s);
+      });
+      if (e != FT_ERR_NO_ERROR) return e;
+      if (t < bt) { bt = t; best.reduce = (int)blocks; }
+    }
+    tune_store(c, best);
+  }
+  TCK(hipStreamSynchronize(st.s));
+  return path ? ftx_tune_save(path) : FT_ERR_NO_ERROR;
+}
+
+}  // extern "C"
+
+namespace ft {
+void tune_init_from_env() {
+  const char *p = std::getenv("FTX_TUNE_FILE");
+  if (p && *p) (void)ftx_tune_load(p);
+}
+}  // namespace ft

@@ -1,0 +1,112 @@
+"""Every aggregation schedule the tuner may pick gives the oracle's result.
+
+The reduce, scan and segmented sum run with each candidate chunk size and
+reduce grid set for the size class of the input, and the outputs are compared
+byte for byte (canonical encodings) with the oracle; the autotuner's own run
+must leave results unchanged.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import libftcrypto_amd as ft
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    ft.init()
+    saved = ft.tune_table()
+    yield
+    for c, v in saved.items():
+        ft.tune_set(c, *v)
+
+
+def same(a, o):
+    assert np.array_equal(a.to_bytes_folded(), o.to_bytes())
+
+
+def seg_case(n, seed):
+    rng = np.random.default_rng(seed)
+    lens = np.clip(np.round(rng.lognormal(np.log(16), 1.5, size=n)), 1, 900).astype(np.int64)
+    lens[rng.integers(0, len(lens), size=len(lens) // 9)] = 0
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    off = off[off <= n]
+    if off[-1] != n:
+        off = np.concatenate([off, [n]])
+    return off
+
+
+@pytest.mark.parametrize("n", [5000, 70000])
+def test_every_segsum_schedule(n):
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    s = ft.random_scalars(n, 4100 + n, 0)
+    g, o = ft.GeArray.from_scalars(s), O.Array.from_scalars(s)
+    off = seg_case(n, n)
+    want = O.segmented_sum(off, o).to_bytes()
+    rng = np.random.default_rng(n)
+    cols = rng.integers(0, n, size=n, dtype=np.int32)
+    want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
+    for ch1, chu in itertools.product((4, 5, 6), (4, 5, 6)):
+        ft.tune_set(c, ch1, chu, base[2], base[3])
+        got = ft.segmented_sum(off, g)
+        assert np.array_equal(got.to_bytes_folded(), want), (ch1, chu)
+        got = ft.gather_sum(off, cols, g)
+        assert np.array_equal(got.to_bytes_folded(), want_g), ("gather", ch1, chu)
+    ft.tune_set(c, *base)
+
+
+@pytest.mark.parametrize("n", [1, 63, 5000, 70000])
+def test_every_scan_schedule(n):
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    s = ft.random_scalars(n, 4200 + n, 0)
+    o_inc, o_exc = O.Array.from_scalars(s), O.Array.from_scalars(s)
+    o_inc.scan()
+    o_exc.scan(exclusive=True)
+    for ch in (4, 5, 6):
+        ft.tune_set(c, base[0], base[1], ch, base[3])
+        g = ft.GeArray.from_scalars(s)
+        g.scan()
+        same(g, o_inc)
+        g = ft.GeArray.from_scalars(s)
+        g.prescan()
+        same(g, o_exc)
+    ft.tune_set(c, *base)
+
+
+@pytest.mark.parametrize("n", [1, 300, 70000])
+def test_every_reduce_schedule(n):
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    s = ft.random_scalars(n, 4300 + n, 0)
+    want = bytes(O.Array.from_scalars(s).reduce())
+    for blocks in (1, 3, 512, 1024, 2048, 4096):
+        ft.tune_set(c, base[0], base[1], base[2], blocks)
+        g = ft.GeArray.from_scalars(s)
+        g.reduce()
+        assert bytes(g.to_bytes_folded()) == want, blocks
+    ft.tune_set(c, *base)
+
+
+def test_autotune_keeps_results(tmp_path):
+    path = str(tmp_path / "tune.txt")
+    table = ft.autotune(max_class=2, path=path)
+    assert os.path.exists(path)
+    for c, (a, b, sc, r) in table.items():
+        assert 4 <= a <= 6 and 4 <= b <= 6 and 4 <= sc <= 6 and 1 <= r <= 4096
+    ft.tune_load(path)
+    assert ft.tune_table() == table
+    n = 20000
+    s = ft.random_scalars(n, 4400, 0)
+    g, o = ft.GeArray.from_scalars(s), O.Array.from_scalars(s)
+    off = seg_case(n, 5)
+    same(ft.segmented_sum(off, g), O.segmented_sum(off, o))
+    g.scan()
+    o.scan()
+    same(g, o)

@@ -1,0 +1,55 @@
+"""The schedule table's host API (no device work): set / get, size classes,
+save / load round trip, and rejection of out-of-range or malformed input."""
+import pytest
+
+import libftcrypto_amd as ft
+
+
+@pytest.fixture(autouse=True)
+def _keep():
+    saved = ft.tune_table()
+    yield
+    for c, v in saved.items():
+        ft.tune_set(c, *v)
+
+
+def test_size_classes():
+    assert [ft.size_class(n) for n in (0, 1, 2**14 - 1, 2**14, 2**18 - 1, 2**18, 2**22 - 1, 2**22, 2**30)] == \
+        [0, 0, 0, 1, 1, 2, 2, 3, 3]
+
+
+def test_set_get_save_load(tmp_path):
+    ft.tune_set(0, 5, 6, 4, 512)
+    ft.tune_set(3, 6, 4, 5, 4096)
+    t = ft.tune_table()
+    assert t[0] == (5, 6, 4, 512) and t[3] == (6, 4, 5, 4096)
+    p = str(tmp_path / "t.txt")
+    ft.tune_save(p)
+    ft.tune_set(0, 4, 4, 4, 2048)
+    ft.tune_load(p)
+    assert ft.tune_table() == t
+
+
+@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048), (0, 4, 7, 4, 2048), (0, 4, 4, 4, 0), (0, 4, 4, 4, 4097)])
+def test_set_rejects_out_of_range(bad):
+    before = ft.tune_table()
+    with pytest.raises(ft.FtError) as e:
+        ft.tune_set(*bad)
+    assert e.value.code == ft.FT_ERR_BAD_SIZE
+    assert ft.tune_table() == before
+    with pytest.raises(ft.FtError) as e:
+        ft.tune_set(4, 4, 4, 4, 2048)
+    assert e.value.code == ft.FT_ERR_OUT_OF_BOUNDS
+
+
+@pytest.mark.parametrize("text", ["0 4 4 4 2048\n1 4 4 4 2048\n2 4 4 4 2048\n",
+                                  "0 4 4 4 2048\n1 4 4 4 2048\n2 4 4 4 2048\n3 4 9 4 2048\n",
+                                  "garbage\n"])
+def test_load_rejects_malformed(tmp_path, text):
+    before = ft.tune_table()
+    p = tmp_path / "bad.txt"
+    p.write_text(text)
+    with pytest.raises(ft.FtError) as e:
+        ft.tune_load(str(p))
+    assert e.value.code == ft.FT_ERR_BAD_SIZE
+    assert ft.tune_table() == before
