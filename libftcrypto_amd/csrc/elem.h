@@ -62,11 +62,13 @@ __device__ __forceinline__ void mul_u32_one(PtView a, PtView b, int two, size_t 
 template <bool ACC>
 __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, const uint32_t *tb,
                                             const uint32_t *th, uint64_t x, int nd, const uint32_t *r) {
-  uint32_t k[8], e[16], odd[8], even[8];
+  uint32_t k[8], e[16], odd[11], even[11];
   ld_words(k, r);
   recode16(e, k);
-  // message digits (at most 17): odd ones and even ones, low first
-  uint32_t mo[3] = {0, 0, 0}, me[3] = {0, 0, 0};
+  // streams of the odd and even digits: r's 32 each in words 0..7, then the
+  // message's (at most 17 digits: 8 odd, 9 even) in words 8..10
+#pragma unroll
+  for (int w = 8; w < 11; ++w) { odd[w] = 0; even[w] = 0; }
   if (!ACC) {
     int carry = 0;
 #pragma unroll
@@ -77,26 +79,23 @@ __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, cons
       if (q == nd - 1) d += carry << 4;       // top digit keeps the final carry
       const uint32_t db = q < nd ? ((uint32_t)d & 0xffu) : 0u;
       const int h = q >> 1;
-      if (q & 1) mo[h >> 2] |= db << (8 * (h & 3));
-      else me[h >> 2] |= db << (8 * (h & 3));
+      if (q & 1) odd[8 + (h >> 2)] |= db << (8 * (h & 3));
+      else even[8 + (h >> 2)] |= db << (8 * (h & 3));
       if (q >= nd - 1) carry = 0;
     }
   }
+  const int mo = ACC ? 0 : nd >> 1, me = ACC ? 0 : (nd + 1) >> 1;
   ge_p3 a1, a2;
   // C1 = r B
-  split_odd_even(odd, even, e);
+  uint32_t o1[8], e1[8];
+  split_odd_even(o1, e1, e);
   ge_p3_0(a1);
-  comb_pass<8>(a1, tb, odd, 32);
-  dbl4(a1);
-  comb_pass<8>(a1, tb, even, 32);
-  // C2 = r H + m B, sharing the four doublings
+  comb_run<8>(a1, tb, tb, o1, e1, 32, 32);
+  // C2 = r H + m B, sharing the four doublings: rows of H for r's digits,
+  // rows of B for the message's
   split_odd_even(odd, even, e);
   ge_p3_0(a2);
-  comb_pass<8>(a2, th, odd, 32);
-  if (!ACC) comb_pass<3>(a2, tb, mo, nd >> 1);
-  dbl4(a2);
-  comb_pass<8>(a2, th, even, 32);
-  if (!ACC) comb_pass<3>(a2, tb, me, (nd + 1) >> 1);
+  comb_run<11>(a2, th, tb, odd, even, 32 + mo, 32 + me);
   if (ACC) {
     ge_p3 o;
     load_p3(o, c1, i);

@@ -74,9 +74,7 @@ __global__ void __launch_bounds__(TPB) k_fixed_base(PtView out, const void *sc, 
   split_odd_even(odd, even, e);
   ge_p3 h;
   ge_p3_0(h);
-  comb_pass<8>(h, tab, odd, 32);
-  dbl4(h);
-  comb_pass<8>(h, tab, even, 32);
+  comb_run<8>(h, tab, tab, odd, even, 32, 32);
   store_p3(out, i, h);
 }
 

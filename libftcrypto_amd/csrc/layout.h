@@ -126,6 +126,20 @@ __device__ __forceinline__ void pin_pw(PW &p) {
                     "+v"(p.t[4]), "+v"(p.t[5]), "+v"(p.t[6]), "+v"(p.t[7]));
 }
 
+// an empty asm taking a field element's limbs as register operands: a
+// scheduling fence that keeps work on the limbs from moving across it
+__device__ __forceinline__ void pin_fe(fe &a) {
+  asm volatile("" : "+v"(a.v[0]), "+v"(a.v[1]), "+v"(a.v[2]), "+v"(a.v[3]), "+v"(a.v[4]), "+v"(a.v[5]),
+                    "+v"(a.v[6]), "+v"(a.v[7]), "+v"(a.v[8]));
+}
+
+__device__ __forceinline__ void pin_p3(ge_p3 &p) {
+  pin_fe(p.X);
+  pin_fe(p.Y);
+  pin_fe(p.Z);
+  pin_fe(p.T);
+}
+
 // points g .. g+3 (indices at or past `hi` re-read point g; their values are unused)
 #define FT_LOAD4(q0, q1, q2, q3, v, g, hi)                   \
   do {                                                       \

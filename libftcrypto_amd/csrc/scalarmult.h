@@ -89,6 +89,39 @@ __device__ __forceinline__ void comb_pass(ge_p3 &h, const uint32_t *tab, uint32_
   }
 }
 
+__device__ __forceinline__ void dbl4(ge_p3 &h);
+
+// Comb in one loop: n0 windows of stream s, four doublings, n1 windows of
+// stream s1.  Window w of a phase (w counted from 0 in each phase) uses row
+// w of ta for w < 32 and row w - 32 of tb after that (the message digits of
+// an encryption, over the table of B).  One loop instead of a loop per
+// phase lets the compiler emit the window body once.
+template <int NW>
+__device__ __forceinline__ void comb_run(ge_p3 &h, const uint32_t *ta, const uint32_t *tb, uint32_t s[NW],
+                                         const uint32_t s1[NW], int n0, int n1) {
+  ge_precomp t;
+  ge_p1p1 r;
+  // keep the trip count opaque: with a known count the compiler reshapes
+  // the loop and emits a slower body (an extra partial product and about
+  // 170 register moves per window)
+  asm volatile("" : "+s"(n0), "+s"(n1));
+#pragma unroll 1
+  for (int q = 0; q < n0 + n1; ++q) {
+    if (q == n0) {
+      dbl4(h);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) s[k] = s1[k];
+    }
+    const int w = q < n0 ? q : q - n0;
+    const int b = pop_lo<NW>(s, w);
+    const uint32_t *row = w < 32 ? ta + w * 8 * COMB_ENTRY_WORDS : tb + (w - 32) * 8 * COMB_ENTRY_WORDS;
+    comb_select(t, row, b);
+    ge_madd(r, h, t);
+    ge_p1p1_to_p3(h, r);
+  }
+  pin_p3(h);
+}
+
 __device__ __forceinline__ void dbl4(ge_p3 &h) {
   ge_p1p1 r;
   ge_p2 s;
@@ -168,6 +201,7 @@ __device__ __forceinline__ void varbase_windows(ge_p3 &h, const ge_p3 &p, uint32
     ge_p2_dbl(r, s);
     ge_p1p1_to_p3(h, r);
   }
+  pin_p3(h);
   cached_select(c, tab, pop_hi(e, vlo));
   ge_add(r, h, c);
   ge_p1p1_to_p3(h, r);
