@@ -183,7 +183,21 @@ def run(args):
     total_mults = sum(mults.values())
     value = total_mults / (step_ms * 1e-3)
 
-    detail = {}
+    # the reference-compatible calls take host buffers: the same mults through
+    # ft_array_from_scalars / ft_mul, PCIe copy of the scalars and the final
+    # synchronisation included (wall clock)
+    ft.check(L.ftx_synchronize(None))
+    t0 = time.time()
+    hb = ft.GeArray.from_scalars(k_fixed)
+    t_fixed_host = time.time() - t0
+    t0 = time.time()
+    hb.mul(k_var)
+    t_var_host = time.time() - t0
+    del hb
+    host_rates = {"fixed_base_mults_per_s": n / t_fixed_host, "variable_base_mults_per_s": n / t_var_host,
+                  "note": "ft_array_from_scalars / ft_mul with host scalar buffers, PCIe and sync included"}
+
+    detail = {"host_buffer_calls": host_rates}
     for k in ops:
         rate = mults[k] / (per[k] * 1e-3)
         ach = rate * PRODUCTS[k]
@@ -253,7 +267,8 @@ def adds_entry(ms, adds, bytes_per_add=None):
 
 
 def bench_cfg0(ft, hip, key, seed):
-    """Encrypt 2 x 65,536 bits, add the two arrays, reduce both halves."""
+    """Encrypt 2 x 65,536 bits, add the two arrays, reduce both halves of the
+    sum: on one stream, and with the independent halves on two streams."""
     L = ft.lib()
     stream = L.ftx_default_stream()
     T = Timer(hip, stream)
@@ -262,24 +277,58 @@ def bench_cfg0(ft, hip, key, seed):
     m = hip.upload(rng.integers(0, 2, size=n, dtype=np.uint64))
     r = hip.upload(ft.random_scalars(n, seed, 10))
     a1, a2, b1, b2 = (ft.GeArray.alloc(n) for _ in range(4))
-    out = ft.GeArray.alloc(1)
-    rscr = ft.GeArray.alloc(L.ftx_reduce_scratch_points(n))
+    out1, out2 = ft.GeArray.alloc(1), ft.GeArray.alloc(1)
+    rscr1 = ft.GeArray.alloc(L.ftx_reduce_scratch_points(n))
+    rscr2 = ft.GeArray.alloc(L.ftx_reduce_scratch_points(n))
     enc_ms = T(lambda: (ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, stream)),
                         ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, stream))))
     add_ms = T(lambda: (ft.check(L.ftx_add_dev(a1.h, b1.h, stream)), ft.check(L.ftx_add_dev(a2.h, b2.h, stream))))
-    red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out.h, a1.h, rscr.h, stream)),
-                        ft.check(L.ftx_reduce_dev(out.h, a2.h, rscr.h, stream))))
+    red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out1.h, a1.h, rscr1.h, stream)),
+                        ft.check(L.ftx_reduce_dev(out2.h, a2.h, rscr2.h, stream))))
     # compression of the outputs (A, B, A+B: 6 arrays of n points), timed apart from the ops
     enc_out = hip.malloc(n * 32)
     cmp_ms = T(lambda: [ft.check(L.ftx_encode_dev(enc_out, x.h, stream)) for x in (a1, a2, b1, b2, a1, a2)])
     hip.L.hipFree(enc_out)
+    # the same step with A's and B's work (then the two halves' reduces) on two streams
+    p = ctypes.c_void_p()
+    ft.check(L.ftx_stream_create(ctypes.byref(p)))
+    s2 = p
+    ev = [hip.event() for _ in range(2)]
+
+    def fork():
+        hip.L.hipEventRecord(ev[0], stream)
+        hip.L.hipStreamWaitEvent(s2, ev[0], 0)
+
+    def join():
+        hip.L.hipEventRecord(ev[1], s2)
+        hip.L.hipStreamWaitEvent(stream, ev[1], 0)
+
+    def step2():
+        fork()
+        ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, stream))
+        ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, s2))
+        join()
+        fork()
+        ft.check(L.ftx_add_dev(a1.h, b1.h, stream))
+        ft.check(L.ftx_add_dev(a2.h, b2.h, s2))
+        ft.check(L.ftx_reduce_dev(out1.h, a1.h, rscr1.h, stream))
+        ft.check(L.ftx_reduce_dev(out2.h, a2.h, rscr2.h, s2))
+        join()
+
+    two_ms = T(step2)
+    ft.check(L.ftx_synchronize(None))
+    L.ftx_stream_destroy(s2)
+    hip.L.hipFree(m)
+    hip.L.hipFree(r)
     return {"compress": {"ms": round(cmp_ms, 4), "points": 6 * n, "points_per_s": 6 * n / (cmp_ms * 1e-3),
                          "note": "canonical 32-byte encodings, batch inversion; not part of step_ms"},
             "encrypt": {"ms": round(enc_ms, 4), "ciphertexts_per_s": 2 * n / (enc_ms * 1e-3),
                         "valu_frac": 2 * n * (2 * PRODUCTS["fixed_base"]) / (enc_ms * 1e-3) / PEAK_MUL32},
             "ct_add": adds_entry(add_ms, 2 * n, 3 * POINT_BYTES),
             "reduce": adds_entry(red_ms, 2 * (n - 1), POINT_BYTES),
-            "step_ms": round(enc_ms + add_ms + red_ms, 4)}
+            "step_ms_one_stream": round(enc_ms + add_ms + red_ms, 4),
+            "step_ms_two_streams": round(two_ms, 4),
+            "ciphertexts_per_s_two_streams": 2 * n / (two_ms * 1e-3)}
 
 
 def bench_cfg2(ft, hip, key, seed, args):
