@@ -25,6 +25,11 @@ CFLAGS = [
 ]
 
 
+# per-source extra flags: the variable-base kernel (one wave per SIMD) is
+# scheduled for instruction-level parallelism
+EXTRA = {"varbase.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+
+
 def _deps():
     hdrs = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(HERE, "..", "include", "*.h"))
     return max((os.path.getmtime(h) for h in hdrs), default=0)
@@ -34,7 +39,7 @@ def _compile(src, force):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(src), _deps()):
         return obj
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + CFLAGS + EXTRA.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s" % (src, r.stderr[-4000:]))

@@ -78,23 +78,6 @@ __global__ void __launch_bounds__(TPB) k_fixed_base(PtView out, const void *sc, 
   store_p3(out, i, h);
 }
 
-// ------------------------------------------------------ variable base
-
-// pts[i] = k_i * pts[i]; scalar stride 0 applies one scalar to every element
-// (ft_scale, ftcrypto.cu:966-983), stride 8 words is ft_mul (:986-1011).
-__global__ void __launch_bounds__(TPB) k_varbase(PtView pts, const uint32_t *sc, size_t sc_stride,
-                                                 size_t n) {
-  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
-  if (i >= n) return;
-  uint32_t k[8], e[16];
-  ld_words(k, sc + i * sc_stride);
-  recode16(e, k);
-  ge_p3 p, h;
-  load_p3(p, pts, i);
-  varbase_windows(h, p, e, 0);
-  store_p3(pts, i, h);
-}
-
 // a[i] = k_i * a[i] (and b[i] = k_i * b[i] when two): homomorphic
 // multiplication of a ciphertext by a 32-bit constant.
 __global__ void __launch_bounds__(TPB) k_mul_u32(PtView a, PtView b, int two, const uint32_t *k,
@@ -309,12 +292,6 @@ hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, con
   if (!n) return hipSuccess;
   if (u64) hipLaunchKernelGGL(k_fixed_base<1>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table);
   else hipLaunchKernelGGL(k_fixed_base<0>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table);
-  FT_LAUNCH_CHECK();
-}
-
-hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s) {
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_varbase, dim3(grid_for(n)), dim3(TPB), 0, s, pts, sc, sc_stride_words, n);
   FT_LAUNCH_CHECK();
 }
 
