@@ -39,18 +39,42 @@ __device__ __forceinline__ void recode_u32(uint32_t e[16], uint32_t k) {
   }
 }
 
-// a[i] = k a[i] and, when two, b[i] = k b[i]
+// 32-bit multipliers in radix 8: 11 signed digits in [-4, 4] (the top one
+// covers bits 30..32 plus the carry, at most 4), at stream positions 53..63
+__device__ __forceinline__ void recode_u32_r8(uint32_t e[16], uint32_t k) {
+  int carry = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) e[q] = 0;
+#pragma unroll
+  for (int i = 0; i < 11; ++i) {
+    int d = (int)((i < 10 ? (k >> (3 * i)) : (k >> 30)) & 7u) + carry;
+    if (i < 10) {
+      carry = (d + 4) >> 3;
+      d -= carry << 3;
+    }
+    const int v = i + 53;
+    e[v >> 2] |= ((uint32_t)d & 0xffu) << (8 * (v & 3));
+  }
+}
+
+// a[i] = k a[i] and, when two, b[i] = k b[i]; W = 4 (radix 16, 8-entry
+// table, 9 windows) or W = 3 (radix 8, 4-entry table, 11 windows)
+template <int W>
 __device__ __forceinline__ void mul_u32_one(PtView a, PtView b, int two, size_t i, uint32_t k) {
   uint32_t e[16];
   ge_p3 p, h;
-  recode_u32(e, k);
+  if (W == 4) recode_u32(e, k);
+  else recode_u32_r8(e, k);
   load_p3(p, a, i);
-  varbase_windows(h, p, e, 55);
+  if (W == 4) varbase_windows(h, p, e, 55);
+  else varbase_windows8(h, p, e, 53);
   store_p3(a, i, h);
   if (two) {
-    recode_u32(e, k);
+    if (W == 4) recode_u32(e, k);
+    else recode_u32_r8(e, k);
     load_p3(p, b, i);
-    varbase_windows(h, p, e, 55);
+    if (W == 4) varbase_windows(h, p, e, 55);
+    else varbase_windows8(h, p, e, 53);
     store_p3(b, i, h);
   }
 }

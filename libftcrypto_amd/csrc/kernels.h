@@ -14,6 +14,7 @@ hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, con
                              hipStream_t s);
 hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s);
 hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s);
+hipError_t launch_mul_u32_w(PtView a, PtView b, bool two, const uint32_t *k, size_t n, int w, hipStream_t s);
 hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
@@ -23,7 +24,8 @@ hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long l
 unsigned reduce_grid(size_t n, unsigned max_blocks);
 hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, hipStream_t s);
 
-// tune.hip: launch schedules of the aggregation kernels, per size class of n
+// tune.hip: launch schedules of the aggregation kernels and the window width
+// of the 32-bit-multiplier kernels, per size class of n
 // (n < 2^14, < 2^18, < 2^22, larger), picked by measurement (ftx_autotune)
 // or left at the defaults.
 constexpr int TUNE_CLASSES = 4;
@@ -32,6 +34,7 @@ struct AggTune {
   int segu;          // log2 records per lane, segmented-sum upper levels
   int scan;          // log2 elements per lane, every scan level
   int reduce;        // most blocks of 256 lanes in the first reduce pass
+  int u32w;          // window width of the 32-bit-multiplier kernels (3 or 4)
 };
 constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
 constexpr unsigned REDUCE_BLOCKS_MAX = 4096;

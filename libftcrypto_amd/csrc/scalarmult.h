@@ -208,6 +208,65 @@ __device__ __forceinline__ void varbase_windows(ge_p3 &h, const ge_p3 &p, uint32
 }
 
 
+// Radix-8 window loop for public short multipliers: h = k * p over digits
+// vlo..63 of a high-first stream of signed radix-8 digits in [-4, 4], with a
+// 4-entry cached table (1P..4P) and three doublings per window.  Only the
+// 32-bit-multiplier kernels use it; 256-bit scalars keep radix 16, whose
+// top-digit rule the reference defines.
+__device__ __forceinline__ void build_cached4(ge_cached tab[4], const ge_p3 &p) {
+  ge_p3 m1, m2, m3;
+  ge_p1p1 t;
+  ge_p3_to_cached(tab[0], p);
+  ge_p3_dbl(t, p);  ge_p1p1_to_p3(m1, t);            // 2P
+  ge_p3_to_cached(tab[1], m1);
+  ge_add(t, p, tab[1]); ge_p1p1_to_p3(m2, t);        // 3P
+  ge_p3_dbl(t, m1); ge_p1p1_to_p3(m3, t);            // 4P
+  ge_p3_to_cached(tab[2], m2);
+  ge_p3_to_cached(tab[3], m3);
+}
+
+__device__ __forceinline__ void cached_select4(ge_cached &t, const ge_cached tab[4], int b) {
+  const uint32_t neg = (uint32_t)b >> 31;
+  const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
+  ge_cached_0(t);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      t.YpX.v[l] = m ? tab[j].YpX.v[l] : t.YpX.v[l];
+      t.YmX.v[l] = m ? tab[j].YmX.v[l] : t.YmX.v[l];
+      t.Z.v[l] = m ? tab[j].Z.v[l] : t.Z.v[l];
+      t.T2d.v[l] = m ? tab[j].T2d.v[l] : t.T2d.v[l];
+    }
+  }
+  ge_cached_cneg(t, neg);
+}
+
+__device__ __forceinline__ void varbase_windows8(ge_p3 &h, const ge_p3 &p, uint32_t e[16], int vlo) {
+  ge_cached tab[4];
+  build_cached4(tab, p);
+  ge_p3_0(h);
+  ge_cached c;
+  ge_p1p1 r;
+  ge_p2 s;
+  asm volatile("" : "+s"(vlo));
+#pragma unroll 1
+  for (int v = 63; v > vlo; --v) {
+    cached_select4(c, tab, pop_hi(e, v));
+    ge_add(r, h, c);
+    ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s);
+    ge_p1p1_to_p3(h, r);
+  }
+  pin_p3(h);
+  cached_select4(c, tab, pop_hi(e, vlo));
+  ge_add(r, h, c);
+  ge_p1p1_to_p3(h, r);
+}
+
 __device__ __forceinline__ void recode16_words(uint32_t e[16], const uint32_t k[8]) { recode16(e, k); }
 
 __device__ __forceinline__ void varbase_mul(ge_p3 &h, const ge_p3 &p, uint32_t e[16]) {

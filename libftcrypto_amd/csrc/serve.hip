@@ -80,12 +80,22 @@ __global__ void __launch_bounds__(TPB) kb_add(const BItem *items, int cnt, uint6
   add_one<false>(it.a2, i, it.b2, i);
 }
 
-__global__ void __launch_bounds__(TPB) kb_mul_u32(const BItem *items, int cnt, uint64_t total) {
+__global__ void __launch_bounds__(TPB) kb_mul_u32_r16(const BItem *items, int cnt, uint64_t total) {
   const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
   if (t >= total) return;
   const BItem &it = items[find_item(items, cnt, t)];
   const uint64_t i = t - it.start;
-  mul_u32_one(it.a1, it.a2, 1, i, it.k[i]);
+  mul_u32_one<4>(it.a1, it.a2, 1, i, it.k[i]);
+}
+
+// radix 8 at two waves per SIMD, as k_mul_u32_r8
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+kb_mul_u32_r8(const BItem *items, int cnt, uint64_t total) {
+  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  if (t >= total) return;
+  const BItem &it = items[find_item(items, cnt, t)];
+  const uint64_t i = t - it.start;
+  mul_u32_one<3>(it.a1, it.a2, 1, i, it.k[i]);
 }
 
 }  // namespace ft
@@ -368,7 +378,10 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
     } else if (kind == K_ADD) {
       hipLaunchKernelGGL(kb_add, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     } else if (kind == K_MUL) {
-      hipLaunchKernelGGL(kb_mul_u32, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
+      if (tune_for(start).u32w == 3)
+        hipLaunchKernelGGL(kb_mul_u32_r8, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
+      else
+        hipLaunchKernelGGL(kb_mul_u32_r16, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     }
     e = hipGetLastError();
   }

@@ -7,6 +7,8 @@
 //   segu    log2 records per lane at the upper segmented-sum levels (4..6)
 //   scan    log2 elements per lane at every scan level (4..6)
 //   reduce  most blocks of 256 lanes in the first reduce pass (512..4096)
+//   u32w    window width of ciphertext x 32-bit constant: 4 (radix 16,
+//           8-entry table, 9 windows) or 3 (radix 8, 4-entry table, 11)
 // Every schedule gives the same group element (point addition is
 // associative and commutative), so the choice only moves time.  Scratch
 // sizes are those of the schedule needing the most, so buffers sized by
@@ -35,9 +37,10 @@ namespace ft {
 
 namespace {
 
-constexpr AggTune DEFAULT_TUNE = {4, 4, 4, 2048};
+constexpr AggTune DEFAULT_TUNE = {4, 4, 4, 2048, 3};
 
-std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES];
+std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
+    T_u32w[TUNE_CLASSES];
 std::atomic<bool> T_ready{false};
 
 void tune_defaults() {
@@ -46,6 +49,7 @@ void tune_defaults() {
     T_segu[c] = DEFAULT_TUNE.segu;
     T_scan[c] = DEFAULT_TUNE.scan;
     T_reduce[c] = DEFAULT_TUNE.reduce;
+    T_u32w[c] = DEFAULT_TUNE.u32w;
   }
   T_ready = true;
 }
@@ -53,7 +57,7 @@ void tune_defaults() {
 bool valid(const AggTune &t) {
   auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
   return ch_ok(t.seg1) && ch_ok(t.segu) && ch_ok(t.scan) && t.reduce >= 1 &&
-         (unsigned)t.reduce <= REDUCE_BLOCKS_MAX;
+         (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4);
 }
 
 void tune_store(int c, const AggTune &t) {
@@ -61,6 +65,7 @@ void tune_store(int c, const AggTune &t) {
   T_segu[c] = t.segu;
   T_scan[c] = t.scan;
   T_reduce[c] = t.reduce;
+  T_u32w[c] = t.u32w;
 }
 
 }  // namespace
@@ -75,7 +80,7 @@ int tune_class(size_t n) {
 AggTune tune_for(size_t n) {
   if (!T_ready) tune_defaults();
   const int c = tune_class(n);
-  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c]};
+  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c]};
 }
 
 }  // namespace ft
@@ -135,19 +140,20 @@ ft_error time_ms(float *ms, const Events &ev, hipStream_t s, int reps, F fn) {
 
 extern "C" {
 
-ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks) {
+ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks, int *u32_window) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
   if (!T_ready) tune_defaults();
   *seg1 = T_seg1[size_class];
   *segu = T_segu[size_class];
   *scan = T_scan[size_class];
   *reduce_blocks = T_reduce[size_class];
+  *u32_window = T_u32w[size_class];
   return FT_ERR_NO_ERROR;
 }
 
-ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks) {
+ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
-  const AggTune t{seg1, segu, scan, reduce_blocks};
+  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window};
   if (!valid(t)) return FT_ERR_BAD_SIZE;
   if (!T_ready) tune_defaults();
   tune_store(size_class, t);
@@ -157,14 +163,15 @@ ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_b
 int ftx_tune_size_class(size_t n) { return tune_class(n); }
 
 // Text format: a comment line, then one line per class:
-// "class seg1 segu scan reduce".
+// "class seg1 segu scan reduce u32w".
 ft_error ftx_tune_save(const char *path) {
   FILE *f = std::fopen(path, "w");
   if (!f) return FT_ERR_INTERNAL_LIBRARY_ERROR;
   if (!T_ready) tune_defaults();
-  std::fprintf(f, "# libftcrypto_amd aggregation schedules: class seg1 segu scan reduce\n");
+  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w\n");
   for (int c = 0; c < TUNE_CLASSES; ++c)
-    std::fprintf(f, "%d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c], (int)T_reduce[c]);
+    std::fprintf(f, "%d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c], (int)T_reduce[c],
+                 (int)T_u32w[c]);
   return std::fclose(f) == 0 ? FT_ERR_NO_ERROR : FT_ERR_INTERNAL_LIBRARY_ERROR;
 }
 
@@ -177,12 +184,12 @@ ft_error ftx_tune_load(const char *path) {
   ft_error err = FT_ERR_NO_ERROR;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#' || line[0] == '\n') continue;
-    int c, a, b, d, r;
-    if (std::sscanf(line, "%d %d %d %d %d", &c, &a, &b, &d, &r) != 5 || c < 0 || c >= TUNE_CLASSES) {
+    int c, a, b, d, r, w;
+    if (std::sscanf(line, "%d %d %d %d %d %d", &c, &a, &b, &d, &r, &w) != 6 || c < 0 || c >= TUNE_CLASSES) {
       err = FT_ERR_BAD_SIZE;
       break;
     }
-    t[c] = AggTune{a, b, d, r};
+    t[c] = AggTune{a, b, d, r, w};
     if (!valid(t[c])) { err = FT_ERR_BAD_SIZE; break; }
     seen[c] = true;
   }
@@ -278,6 +285,17 @@ ft_error ftx_autotune(int max_class, const char *path) {
       });
       if (e != FT_ERR_NO_ERROR) return e;
       if (t < bt) { bt = t; best.reduce = (int)blocks; }
+    }
+    // ciphertext x u32 over the points as both halves (in place; any
+    // points time the same), window width 3 against 4
+    bt = INFINITY;
+    for (int w = 3; w <= 4; ++w) {
+      const size_t nm = std::min(n, (size_t)1 << 20);       // mults are costly: time at most 2^20
+      ft_error e = time_ms(&t, ev, st.s, c < 2 ? 3 : 1, [&] {
+        return launch_mul_u32_w(C, C, false, (const uint32_t *)sc.p, nm, w, st.s);
+      });
+      if (e != FT_ERR_NO_ERROR) return e;
+      if (t < bt) { bt = t; best.u32w = w; }
     }
     tune_store(c, best);
   }

@@ -53,7 +53,7 @@ def test_every_segsum_schedule(n):
     cols = rng.integers(0, n, size=n, dtype=np.int32)
     want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
     for ch1, chu in itertools.product((4, 5, 6), (4, 5, 6)):
-        ft.tune_set(c, ch1, chu, base[2], base[3])
+        ft.tune_set(c, ch1, chu, base[2], base[3], base[4])
         got = ft.segmented_sum(off, g)
         assert np.array_equal(got.to_bytes_folded(), want), (ch1, chu)
         got = ft.gather_sum(off, cols, g)
@@ -70,7 +70,7 @@ def test_every_scan_schedule(n):
     o_inc.scan()
     o_exc.scan(exclusive=True)
     for ch in (4, 5, 6):
-        ft.tune_set(c, base[0], base[1], ch, base[3])
+        ft.tune_set(c, base[0], base[1], ch, base[3], base[4])
         g = ft.GeArray.from_scalars(s)
         g.scan()
         same(g, o_inc)
@@ -87,10 +87,33 @@ def test_every_reduce_schedule(n):
     s = ft.random_scalars(n, 4300 + n, 0)
     want = bytes(O.Array.from_scalars(s).reduce())
     for blocks in (1, 3, 512, 1024, 2048, 4096):
-        ft.tune_set(c, base[0], base[1], base[2], blocks)
+        ft.tune_set(c, base[0], base[1], base[2], blocks, base[4])
         g = ft.GeArray.from_scalars(s)
         g.reduce()
         assert bytes(g.to_bytes_folded()) == want, blocks
+    ft.tune_set(c, *base)
+
+
+@pytest.mark.parametrize("n", [1, 1000, 20000])
+def test_every_u32_window(n):
+    """ciphertext x u32 with radix-8 and radix-16 windows against the oracle,
+    multipliers including 0, 1, 7, 8, 2^31, 2^32 - 1 and random"""
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    s1, s2 = ft.random_scalars(n, 4500 + n, 1), ft.random_scalars(n, 4500 + n, 2)
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    edge = np.array([0, 1, 7, 8, 2**31, 2**32 - 1, 4, 5, 0x924924924 & 0xFFFFFFFF, 0xB6DB6DB6], dtype=np.uint32)
+    a[:min(n, len(edge))] = edge[:min(n, len(edge))]
+    o1, o2 = O.Array.from_scalars(s1), O.Array.from_scalars(s2)
+    o1.mul_u32(a)
+    o2.mul_u32(a)
+    for w in (3, 4):
+        ft.tune_set(c, base[0], base[1], base[2], base[3], w)
+        g1, g2 = ft.GeArray.from_scalars(s1), ft.GeArray.from_scalars(s2)
+        g1.mul_u32(a, g2)
+        same(g1, o1)
+        same(g2, o2)
     ft.tune_set(c, *base)
 
 
@@ -98,8 +121,8 @@ def test_autotune_keeps_results(tmp_path):
     path = str(tmp_path / "tune.txt")
     table = ft.autotune(max_class=2, path=path)
     assert os.path.exists(path)
-    for c, (a, b, sc, r) in table.items():
-        assert 4 <= a <= 6 and 4 <= b <= 6 and 4 <= sc <= 6 and 1 <= r <= 4096
+    for c, (a, b, sc, r, w) in table.items():
+        assert 4 <= a <= 6 and 4 <= b <= 6 and 4 <= sc <= 6 and 1 <= r <= 4096 and w in (3, 4)
     ft.tune_load(path)
     assert ft.tune_table() == table
     n = 20000
