@@ -147,13 +147,14 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
 
 /* ------------------------------------------------ aggregation schedules */
 
-/* Launch schedules of the reduce, scan and segmented sum and the window width
- * of ciphertext x u32, one per size class of the input length (class 0:
- * n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).  seg1 / segu / scan are log2
+/* Launch schedules of the reduce, scan and segmented sum and the window widths
+ * of the variable-base mults, one per size class of the input length (class
+ * 0: n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).  seg1 / segu / scan are log2
  * elements per lane (4..6) at segmented-sum level 1, its upper levels and the
- * scan; reduce_blocks (1..4096) bounds the first reduce pass; u32_window is 4
- * (radix-16 digits, 8-entry table) or 3 (radix 8, 4 entries).  Results are
- * the same under every schedule.
+ * scan; reduce_blocks (1..4096) bounds the first reduce pass; u32_window
+ * (ciphertext x u32) and varbase_window (ft_mul, ft_scale) are 4 (radix-16
+ * digits, 8-entry table) or 3 (radix 8, 4 entries).  Results are the same
+ * under every schedule.
  * ftx_autotune times every candidate for classes 0..max_class on the device
  * (class 3 at n = 2^24, about 2.6 GB while it runs), keeps the fastest and
  * writes the table to path unless it is NULL; ft_crypto_init loads the file
@@ -161,8 +162,10 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
 ft_error ftx_autotune(int max_class, const char *path);
 ft_error ftx_tune_save(const char *path);
 ft_error ftx_tune_load(const char *path);
-ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks, int *u32_window);
-ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window);
+ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks, int *u32_window,
+                      int *varbase_window);
+ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window,
+                      int varbase_window);
 int ftx_tune_size_class(size_t n);
 
 /* ---------------------------------------------------------- inputs */

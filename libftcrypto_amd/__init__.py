@@ -111,8 +111,8 @@ _SIGS = {
     "ftx_autotune": (i32, [i32, ctypes.c_char_p]),
     "ftx_tune_save": (i32, [ctypes.c_char_p]),
     "ftx_tune_load": (i32, [ctypes.c_char_p]),
-    "ftx_tune_get": (i32, [i32, vp, vp, vp, vp, vp]),
-    "ftx_tune_set": (i32, [i32, i32, i32, i32, i32, i32]),
+    "ftx_tune_get": (i32, [i32, vp, vp, vp, vp, vp, vp]),
+    "ftx_tune_set": (i32, [i32, i32, i32, i32, i32, i32, i32]),
     "ftx_tune_size_class": (i32, [sz]),
     "ftx_random_scalars": (None, [vp, sz, u64, u64]),
     "ftx_scalar_reduce64": (None, [vp, vp, sz]),
@@ -408,26 +408,28 @@ def scalar_reduce64(wide):
 
 
 def autotune(max_class=3, path=None):
-    """Time every reduce / scan / segmented-sum schedule and both ciphertext x
-    u32 window widths on the device for size classes 0..max_class, keep the
-    fastest (and save them to path)."""
+    """Time every reduce / scan / segmented-sum schedule and both window
+    widths of the variable-base kernels on the device for size classes
+    0..max_class, keep the fastest (and save them to path)."""
     init()
     check(lib().ftx_autotune(max_class, path.encode() if path else None), "ftx_autotune")
     return tune_table()
 
 
 def tune_table():
-    """{class: (seg1, segu, scan, reduce_blocks, u32_window)} of the schedules in use."""
+    """{class: (seg1, segu, scan, reduce_blocks, u32_window, varbase_window)}
+    of the schedules in use."""
     out = {}
     for c in range(4):
-        v = [ctypes.c_int() for _ in range(5)]
+        v = [ctypes.c_int() for _ in range(6)]
         check(lib().ftx_tune_get(c, *[ctypes.byref(x) for x in v]), "ftx_tune_get")
         out[c] = tuple(x.value for x in v)
     return out
 
 
-def tune_set(size_class, seg1, segu, scan, reduce_blocks, u32_window=4):
-    check(lib().ftx_tune_set(size_class, seg1, segu, scan, reduce_blocks, u32_window), "ftx_tune_set")
+def tune_set(size_class, seg1, segu, scan, reduce_blocks, u32_window=3, varbase_window=3):
+    check(lib().ftx_tune_set(size_class, seg1, segu, scan, reduce_blocks, u32_window, varbase_window),
+          "ftx_tune_set")
 
 
 def tune_load(path):

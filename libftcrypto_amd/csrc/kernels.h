@@ -13,6 +13,8 @@ hipError_t launch_neg(PtView v, size_t n, hipStream_t s);
 hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table,
                              hipStream_t s);
 hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s);
+hipError_t launch_varbase_w(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, int w,
+                            hipStream_t s);
 hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s);
 hipError_t launch_mul_u32_w(PtView a, PtView b, bool two, const uint32_t *k, size_t n, int w, hipStream_t s);
 hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
@@ -24,8 +26,8 @@ hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long l
 unsigned reduce_grid(size_t n, unsigned max_blocks);
 hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, hipStream_t s);
 
-// tune.hip: launch schedules of the aggregation kernels and the window width
-// of the 32-bit-multiplier kernels, per size class of n
+// tune.hip: launch schedules of the aggregation kernels and the window widths
+// of the variable-base kernels, per size class of n
 // (n < 2^14, < 2^18, < 2^22, larger), picked by measurement (ftx_autotune)
 // or left at the defaults.
 constexpr int TUNE_CLASSES = 4;
@@ -35,6 +37,7 @@ struct AggTune {
   int scan;          // log2 elements per lane, every scan level
   int reduce;        // most blocks of 256 lanes in the first reduce pass
   int u32w;          // window width of the 32-bit-multiplier kernels (3 or 4)
+  int vbw;           // window width of the 256-bit variable-base kernel (3 or 4)
 };
 constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
 constexpr unsigned REDUCE_BLOCKS_MAX = 4096;

@@ -267,6 +267,108 @@ __device__ __forceinline__ void varbase_windows8(ge_p3 &h, const ge_p3 &p, uint3
   ge_p1p1_to_p3(h, r);
 }
 
+// ------------------------------------------ radix 8 for 256-bit scalars
+//
+// The reference's radix-16 recoding (sodium_ge.cu:693-708) gives a top digit
+// d63 in [0, 16]; a top digit above 8 selects no table entry, so the
+// reference computes (k - d63 2^252) P for such k (only k >= 2^255, outside
+// the documented precondition).  scalar_effective rewrites k to exactly that
+// multiple, as a magnitude and a sign, without a branch; the radix-8 loop
+// then gives the reference's result for every 32-byte input.
+
+// |k - [d63 > 8] d63 2^252| into m, its sign (1 = negative) returned
+__device__ __forceinline__ uint32_t scalar_effective(uint32_t m[8], const uint32_t k[8]) {
+  uint32_t e[16];
+  recode16(e, k);
+  const int d63 = (int)(int8_t)(uint8_t)(e[15] >> 24);
+  const uint32_t big = (uint32_t)(8 - d63) >> 31;              // d63 > 8
+  const uint32_t sub = (0u - big) & (uint32_t)d63;             // 0..16
+#pragma unroll
+  for (int i = 0; i < 7; ++i) m[i] = k[i];
+  const uint64_t t = (uint64_t)k[7] - ((uint64_t)sub << 28);   // 2^252 = bit 28 of word 7
+  m[7] = (uint32_t)t;
+  const uint32_t neg = (uint32_t)(t >> 63);                    // borrow out of the top word
+  const uint32_t mask = 0u - neg;
+  uint64_t c = neg;                                            // two's complement negate when negative
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint64_t x = (uint64_t)(m[i] ^ mask) + c;
+    m[i] = (uint32_t)x;
+    c = x >> 32;
+  }
+  return neg;
+}
+
+// signed radix-8 digits of m < 2^256: 86 digits in [-4, 4] (the top one
+// covers bit 255 plus the carry, at most 2) as 4-bit two's-complement
+// nibbles; digit v sits at nibble position v + 2 (word (v+2) >> 3) so that
+// digit 85 is the top nibble of word 10
+__device__ __forceinline__ void recode8(uint32_t e[11], const uint32_t m[8]) {
+  int carry = 0;
+#pragma unroll
+  for (int q = 0; q < 11; ++q) e[q] = 0;
+#pragma unroll
+  for (int v = 0; v < 86; ++v) {
+    const int bit = 3 * v, w = bit >> 5, sh = bit & 31;
+    uint32_t x = m[w] >> sh;
+    if (sh > 29 && w + 1 < 8) x |= m[w + 1] << (32 - sh);
+    int d = (int)(x & 7u) + carry;
+    if (v < 85) {
+      carry = (d + 4) >> 3;
+      d -= carry << 3;
+    }
+    const int p = v + 2;
+    e[p >> 3] |= ((uint32_t)d & 0xfu) << (4 * (p & 7));
+  }
+}
+
+// pop the top nibble digit (sign-extended) of an 11-word high-first stream;
+// p = its position
+__device__ __forceinline__ int pop_hi4(uint32_t s[11], int p) {
+  const int b = ((int)s[10]) >> 28;
+  s[10] <<= 4;
+  if ((p & 7) == 0) {
+#pragma unroll
+    for (int k = 10; k > 0; --k) s[k] = s[k - 1];
+    s[0] = 0;
+  }
+  return b;
+}
+
+// h = k p for a 256-bit scalar in radix 8 (85 windows of an add and three
+// doublings, then the last add), with the reference's result for k >= 2^255
+__device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const uint32_t k[8]) {
+  uint32_t m[8], e[11];
+  const uint32_t neg = scalar_effective(m, k);
+  recode8(e, m);
+  ge_p3 p = p_in, np;
+  ge_p3_neg(np, p);
+  fe_cmov(p.X, np.X, neg);
+  fe_cmov(p.T, np.T, neg);
+  ge_cached tab[4];
+  build_cached4(tab, p);
+  ge_p3_0(h);
+  ge_cached c;
+  ge_p1p1 r;
+  ge_p2 s;
+  int plo = 2;
+  asm volatile("" : "+s"(plo));
+#pragma unroll 1
+  for (int q = 87; q > plo; --q) {
+    cached_select4(c, tab, pop_hi4(e, q));
+    ge_add(r, h, c);
+    ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
+    ge_p2_dbl(r, s);
+    ge_p1p1_to_p3(h, r);
+  }
+  pin_p3(h);
+  cached_select4(c, tab, pop_hi4(e, plo));
+  ge_add(r, h, c);
+  ge_p1p1_to_p3(h, r);
+}
+
 __device__ __forceinline__ void recode16_words(uint32_t e[16], const uint32_t k[8]) { recode16(e, k); }
 
 __device__ __forceinline__ void varbase_mul(ge_p3 &h, const ge_p3 &p, uint32_t e[16]) {

@@ -9,6 +9,8 @@
 //   reduce  most blocks of 256 lanes in the first reduce pass (512..4096)
 //   u32w    window width of ciphertext x 32-bit constant: 4 (radix 16,
 //           8-entry table, 9 windows) or 3 (radix 8, 4-entry table, 11)
+//   vbw     window width of the 256-bit variable-base mult: 4 (radix 16,
+//           64 windows) or 3 (radix 8, 86 windows)
 // Every schedule gives the same group element (point addition is
 // associative and commutative), so the choice only moves time.  Scratch
 // sizes are those of the schedule needing the most, so buffers sized by
@@ -37,10 +39,10 @@ namespace ft {
 
 namespace {
 
-constexpr AggTune DEFAULT_TUNE = {4, 4, 4, 2048, 3};
+constexpr AggTune DEFAULT_TUNE = {4, 4, 4, 2048, 3, 3};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
-    T_u32w[TUNE_CLASSES];
+    T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES];
 std::atomic<bool> T_ready{false};
 
 void tune_defaults() {
@@ -50,6 +52,7 @@ void tune_defaults() {
     T_scan[c] = DEFAULT_TUNE.scan;
     T_reduce[c] = DEFAULT_TUNE.reduce;
     T_u32w[c] = DEFAULT_TUNE.u32w;
+    T_vbw[c] = DEFAULT_TUNE.vbw;
   }
   T_ready = true;
 }
@@ -57,7 +60,8 @@ void tune_defaults() {
 bool valid(const AggTune &t) {
   auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
   return ch_ok(t.seg1) && ch_ok(t.segu) && ch_ok(t.scan) && t.reduce >= 1 &&
-         (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4);
+         (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4) &&
+         (t.vbw == 3 || t.vbw == 4);
 }
 
 void tune_store(int c, const AggTune &t) {
@@ -66,6 +70,7 @@ void tune_store(int c, const AggTune &t) {
   T_scan[c] = t.scan;
   T_reduce[c] = t.reduce;
   T_u32w[c] = t.u32w;
+  T_vbw[c] = t.vbw;
 }
 
 }  // namespace
@@ -80,7 +85,7 @@ int tune_class(size_t n) {
 AggTune tune_for(size_t n) {
   if (!T_ready) tune_defaults();
   const int c = tune_class(n);
-  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c]};
+  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c], T_vbw[c]};
 }
 
 }  // namespace ft
@@ -140,7 +145,8 @@ ft_error time_ms(float *ms, const Events &ev, hipStream_t s, int reps, F fn) {
 
 extern "C" {
 
-ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks, int *u32_window) {
+ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *reduce_blocks, int *u32_window,
+                      int *varbase_window) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
   if (!T_ready) tune_defaults();
   *seg1 = T_seg1[size_class];
@@ -148,12 +154,14 @@ ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *redu
   *scan = T_scan[size_class];
   *reduce_blocks = T_reduce[size_class];
   *u32_window = T_u32w[size_class];
+  *varbase_window = T_vbw[size_class];
   return FT_ERR_NO_ERROR;
 }
 
-ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window) {
+ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window,
+                      int varbase_window) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
-  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window};
+  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window, varbase_window};
   if (!valid(t)) return FT_ERR_BAD_SIZE;
   if (!T_ready) tune_defaults();
   tune_store(size_class, t);
@@ -163,15 +171,15 @@ ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_b
 int ftx_tune_size_class(size_t n) { return tune_class(n); }
 
 // Text format: a comment line, then one line per class:
-// "class seg1 segu scan reduce u32w".
+// "class seg1 segu scan reduce u32w vbw".
 ft_error ftx_tune_save(const char *path) {
   FILE *f = std::fopen(path, "w");
   if (!f) return FT_ERR_INTERNAL_LIBRARY_ERROR;
   if (!T_ready) tune_defaults();
-  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w\n");
+  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w vbw\n");
   for (int c = 0; c < TUNE_CLASSES; ++c)
-    std::fprintf(f, "%d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c], (int)T_reduce[c],
-                 (int)T_u32w[c]);
+    std::fprintf(f, "%d %d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c], (int)T_reduce[c],
+                 (int)T_u32w[c], (int)T_vbw[c]);
   return std::fclose(f) == 0 ? FT_ERR_NO_ERROR : FT_ERR_INTERNAL_LIBRARY_ERROR;
 }
 
@@ -184,12 +192,13 @@ ft_error ftx_tune_load(const char *path) {
   ft_error err = FT_ERR_NO_ERROR;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#' || line[0] == '\n') continue;
-    int c, a, b, d, r, w;
-    if (std::sscanf(line, "%d %d %d %d %d %d", &c, &a, &b, &d, &r, &w) != 6 || c < 0 || c >= TUNE_CLASSES) {
+    int c, a, b, d, r, w, v;
+    if (std::sscanf(line, "%d %d %d %d %d %d %d", &c, &a, &b, &d, &r, &w, &v) != 7 || c < 0 ||
+        c >= TUNE_CLASSES) {
       err = FT_ERR_BAD_SIZE;
       break;
     }
-    t[c] = AggTune{a, b, d, r, w};
+    t[c] = AggTune{a, b, d, r, w, v};
     if (!valid(t[c])) { err = FT_ERR_BAD_SIZE; break; }
     seen[c] = true;
   }
@@ -296,6 +305,18 @@ ft_error ftx_autotune(int max_class, const char *path) {
       });
       if (e != FT_ERR_NO_ERROR) return e;
       if (t < bt) { bt = t; best.u32w = w; }
+    }
+    // variable-base mults in place on the copy, radix 8 against radix 16;
+    // one 8-word scalar for every lane (stride 0: sc holds 8 bytes per
+    // element, not 32), which times the same as distinct scalars
+    bt = INFINITY;
+    for (int w = 3; w <= 4; ++w) {
+      const size_t nm = std::min(n, (size_t)1 << 19);
+      ft_error e = time_ms(&t, ev, st.s, c < 2 ? 3 : 1, [&] {
+        return launch_varbase_w(C, (const uint32_t *)sc.p, 0, nm, w, st.s);
+      });
+      if (e != FT_ERR_NO_ERROR) return e;
+      if (t < bt) { bt = t; best.vbw = w; }
     }
     tune_store(c, best);
   }

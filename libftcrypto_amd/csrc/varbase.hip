@@ -1,7 +1,8 @@
-// Variable-base scalar multiplication kernel.  It is compiled in a file of
-// its own because it runs at one wave per SIMD (its 8-entry point table lives
-// in registers) and gains from the instruction scheduler's max-ILP strategy,
-// which the other kernels do not (build.py passes the option for this file).
+// Variable-base scalar multiplication kernels, radix 16 and radix 8.  They
+// are compiled in a file of their own because the radix-16 kernel runs at one
+// wave per SIMD (its 8-entry point table lives in registers) and gains from
+// the instruction scheduler's max-ILP strategy, which the other kernels do not
+// (build.py passes the option for this file).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -26,10 +27,35 @@ __global__ void __launch_bounds__(TPB) k_varbase(PtView pts, const uint32_t *sc,
   store_p3(pts, i, h);
 }
 
-hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s) {
+// the same in radix 8: a 4-entry table, two waves per SIMD (a few spilled
+// words), the reference's result for every scalar (varbase_mul8)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_varbase_r8(PtView pts, const uint32_t *sc, size_t sc_stride, size_t n) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k[8];
+  ld_words(k, sc + i * sc_stride);
+  ge_p3 p, h;
+  load_p3(p, pts, i);
+  varbase_mul8(h, p, k);
+  store_p3(pts, i, h);
+}
+
+hipError_t launch_varbase_w(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, int w,
+                            hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_varbase, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, s, pts, sc, sc_stride_words, n);
+  const dim3 grid((unsigned)((n + TPB - 1) / TPB));
+  if (w == 3)
+    hipLaunchKernelGGL(k_varbase_r8, grid, dim3(TPB), 0, s, pts, sc, sc_stride_words, n);
+  else if (w == 4)
+    hipLaunchKernelGGL(k_varbase, grid, dim3(TPB), 0, s, pts, sc, sc_stride_words, n);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s) {
+  return launch_varbase_w(pts, sc, sc_stride_words, n, tune_for(n).vbw, s);
 }
 
 }  // namespace ft

@@ -53,7 +53,7 @@ def test_every_segsum_schedule(n):
     cols = rng.integers(0, n, size=n, dtype=np.int32)
     want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
     for ch1, chu in itertools.product((4, 5, 6), (4, 5, 6)):
-        ft.tune_set(c, ch1, chu, base[2], base[3], base[4])
+        ft.tune_set(c, ch1, chu, *base[2:])
         got = ft.segmented_sum(off, g)
         assert np.array_equal(got.to_bytes_folded(), want), (ch1, chu)
         got = ft.gather_sum(off, cols, g)
@@ -70,7 +70,7 @@ def test_every_scan_schedule(n):
     o_inc.scan()
     o_exc.scan(exclusive=True)
     for ch in (4, 5, 6):
-        ft.tune_set(c, base[0], base[1], ch, base[3], base[4])
+        ft.tune_set(c, base[0], base[1], ch, *base[3:])
         g = ft.GeArray.from_scalars(s)
         g.scan()
         same(g, o_inc)
@@ -87,7 +87,7 @@ def test_every_reduce_schedule(n):
     s = ft.random_scalars(n, 4300 + n, 0)
     want = bytes(O.Array.from_scalars(s).reduce())
     for blocks in (1, 3, 512, 1024, 2048, 4096):
-        ft.tune_set(c, base[0], base[1], base[2], blocks, base[4])
+        ft.tune_set(c, *base[:3], blocks, *base[4:])
         g = ft.GeArray.from_scalars(s)
         g.reduce()
         assert bytes(g.to_bytes_folded()) == want, blocks
@@ -109,7 +109,7 @@ def test_every_u32_window(n):
     o1.mul_u32(a)
     o2.mul_u32(a)
     for w in (3, 4):
-        ft.tune_set(c, base[0], base[1], base[2], base[3], w)
+        ft.tune_set(c, *base[:4], w, base[5])
         g1, g2 = ft.GeArray.from_scalars(s1), ft.GeArray.from_scalars(s2)
         g1.mul_u32(a, g2)
         same(g1, o1)
@@ -117,12 +117,49 @@ def test_every_u32_window(n):
     ft.tune_set(c, *base)
 
 
+def varbase_edge_scalars():
+    L = 2**252 + 27742317777372353535851937790883648493
+    ks = [0, 1, 2, 3, 4, 5, 7, 8, 9, L - 1, L, 2**252, 2**253 - 1, 2**255 - 1, 2**255, 2**256 - 1,
+          2**255 + 2**252 * 3, 15 * 2**252, 9 * 2**252 + 5, 8 * 2**252 + 2**251, int("8" * 64, 16),
+          int("7" * 64, 16), int("9" * 64, 16), int("4" * 64, 16), int("c" * 64, 16), int("f0" * 32, 16)]
+    return np.stack([np.frombuffer(int.to_bytes(k, 32, "little"), dtype=np.uint8) for k in ks])
+
+
+@pytest.mark.parametrize("n", [30, 3000])
+def test_every_varbase_window(n):
+    """k P (ft_mul) and one k for all P (ft_scale) in radix 8 and radix 16
+    against the oracle, incl. scalars >= 2^255 whose top radix-16 digit
+    exceeds 8 (the reference's cmov8 then selects the identity)"""
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    s = ft.random_scalars(n, 4600 + n, 1)
+    k = ft.random_scalars(n, 4600 + n, 2)
+    e = varbase_edge_scalars()
+    k[:len(e)] = e
+    o = O.Array.from_scalars(s)
+    o.mul(k)
+    for sc in (e[13], e[15], e[18]):
+        os_ = O.Array.from_scalars(s)
+        os_.scale(sc)
+        for w in (3, 4):
+            ft.tune_set(c, *base[:5], w)
+            g = ft.GeArray.from_scalars(s)
+            g.scale(sc)
+            same(g, os_)
+    for w in (3, 4):
+        ft.tune_set(c, *base[:5], w)
+        g = ft.GeArray.from_scalars(s)
+        g.mul(k)
+        same(g, o)
+    ft.tune_set(c, *base)
+
+
 def test_autotune_keeps_results(tmp_path):
     path = str(tmp_path / "tune.txt")
     table = ft.autotune(max_class=2, path=path)
     assert os.path.exists(path)
-    for c, (a, b, sc, r, w) in table.items():
-        assert 4 <= a <= 6 and 4 <= b <= 6 and 4 <= sc <= 6 and 1 <= r <= 4096 and w in (3, 4)
+    for c, (a, b, sc, r, w, v) in table.items():
+        assert 4 <= a <= 6 and 4 <= b <= 6 and 4 <= sc <= 6 and 1 <= r <= 4096 and w in (3, 4) and v in (3, 4)
     ft.tune_load(path)
     assert ft.tune_table() == table
     n = 20000

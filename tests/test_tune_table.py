@@ -1,6 +1,6 @@
 """The schedule table's host API (no device work): set / get, size classes,
 save / load round trip, and rejection of out-of-range or malformed input
-(including a file in the older five-column format)."""
+(including a file in an older, shorter format)."""
 import pytest
 
 import libftcrypto_amd as ft
@@ -20,19 +20,20 @@ def test_size_classes():
 
 
 def test_set_get_save_load(tmp_path):
-    ft.tune_set(0, 5, 6, 4, 512, 3)
-    ft.tune_set(3, 6, 4, 5, 4096, 4)
+    ft.tune_set(0, 5, 6, 4, 512, 3, 4)
+    ft.tune_set(3, 6, 4, 5, 4096, 4, 3)
     t = ft.tune_table()
-    assert t[0] == (5, 6, 4, 512, 3) and t[3] == (6, 4, 5, 4096, 4)
+    assert t[0] == (5, 6, 4, 512, 3, 4) and t[3] == (6, 4, 5, 4096, 4, 3)
     p = str(tmp_path / "t.txt")
     ft.tune_save(p)
-    ft.tune_set(0, 4, 4, 4, 2048, 4)
+    ft.tune_set(0, 4, 4, 4, 2048, 4, 4)
     ft.tune_load(p)
     assert ft.tune_table() == t
 
 
-@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048, 4), (0, 4, 7, 4, 2048, 4), (0, 4, 4, 4, 0, 4),
-                                 (0, 4, 4, 4, 4097, 4), (0, 4, 4, 4, 2048, 5), (0, 4, 4, 4, 2048, 2)])
+@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048, 4, 4), (0, 4, 7, 4, 2048, 4, 4), (0, 4, 4, 4, 0, 4, 4),
+                                 (0, 4, 4, 4, 4097, 4, 4), (0, 4, 4, 4, 2048, 5, 4), (0, 4, 4, 4, 2048, 2, 4),
+                                 (0, 4, 4, 4, 2048, 4, 5)])
 def test_set_rejects_out_of_range(bad):
     before = ft.tune_table()
     with pytest.raises(ft.FtError) as e:
@@ -40,12 +41,12 @@ def test_set_rejects_out_of_range(bad):
     assert e.value.code == ft.FT_ERR_BAD_SIZE
     assert ft.tune_table() == before
     with pytest.raises(ft.FtError) as e:
-        ft.tune_set(4, 4, 4, 4, 2048, 4)
+        ft.tune_set(4, 4, 4, 4, 2048, 4, 4)
     assert e.value.code == ft.FT_ERR_OUT_OF_BOUNDS
 
 
-@pytest.mark.parametrize("text", ["0 4 4 4 2048 4\n1 4 4 4 2048 4\n2 4 4 4 2048 4\n",
-                                  "0 4 4 4 2048 4\n1 4 4 4 2048 4\n2 4 4 4 2048 4\n3 4 9 4 2048 4\n",
+@pytest.mark.parametrize("text", ["0 4 4 4 2048 4 4\n1 4 4 4 2048 4 4\n2 4 4 4 2048 4 4\n",
+                                  "0 4 4 4 2048 4 4\n1 4 4 4 2048 4 4\n2 4 4 4 2048 4 4\n3 4 9 4 2048 4 4\n",
                                   "0 4 4 4 2048\n1 4 4 4 2048\n2 4 4 4 2048\n3 4 4 4 2048\n",
                                   "garbage\n"])
 def test_load_rejects_malformed(tmp_path, text):
