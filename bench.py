@@ -52,11 +52,13 @@ POINT_BYTES = 128
 # HBM bytes per launch at N = 2^22 from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (one pass each), scaled by the same counters' reading for a
 # plain copy kernel of known size (x1.993 read, x0.994 write), and the VALU
-# busy fraction of the same run (profiles/scalarmult_n22_pmc_counters.md).
+# busy fraction of the same run (profiles/scalarmult_n22_pmc_counters.md),
+# for the default (radix-8) variable-base kernels; their traffic includes
+# register spills.
 PMC_N22 = {
-    "fixed_base": {"read": 134.7e6, "write": 533.6e6, "valu_busy": 1.0},
-    "variable_base": {"read": 670.8e6, "write": 533.6e6, "valu_busy": 0.835},
-    "mul_u32": {"read": 1090.6e6, "write": 1067.3e6, "valu_busy": 0.817},
+    "fixed_base": {"read": 135.0e6, "write": 533.6e6, "valu_busy": 1.0},
+    "variable_base": {"read": 1750.9e6, "write": 1899.6e6, "valu_busy": 0.982},
+    "mul_u32": {"read": 1311.6e6, "write": 1434.0e6, "valu_busy": 0.929},
 }
 ALG_BYTES = {"fixed_base": 32 + 128, "variable_base": 32 + 2 * 128, "mul_u32": 4 + 4 * 128}
 
