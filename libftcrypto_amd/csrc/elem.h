@@ -86,11 +86,29 @@ __device__ __forceinline__ void mul_u32_one(PtView a, PtView b, int two, size_t 
 template <bool ACC>
 __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, const uint32_t *tb,
                                             const uint32_t *th, uint64_t x, int nd, const uint32_t *r) {
-  uint32_t k[8], e[16], odd[11], even[11];
+  uint32_t k[8];
   ld_words(k, r);
+  // C1 = r B, finished and stored before C2 starts so that only one comb's
+  // digit streams and accumulator are live at a time
+  {
+    uint32_t e[16], o1[8], e1[8];
+    recode16(e, k);
+    split_odd_even(o1, e1, e);
+    ge_p3 a1;
+    ge_p3_0(a1);
+    comb_run<8>(a1, tb, tb, o1, e1, 32, 32);
+    if (ACC) {
+      ge_p3 o;
+      load_p3(o, c1, i);
+      ge_p3_add(a1, o, a1);
+    }
+    store_p3(c1, i, a1);
+  }
+  // C2 = r H + m B, sharing the four doublings: rows of H for r's digits
+  // (stream words 0..7), rows of B for the message's (at most 17 digits:
+  // 8 odd, 9 even, in words 8..10)
+  uint32_t e[16], odd[11], even[11];
   recode16(e, k);
-  // streams of the odd and even digits: r's 32 each in words 0..7, then the
-  // message's (at most 17 digits: 8 odd, 9 even) in words 8..10
 #pragma unroll
   for (int w = 8; w < 11; ++w) { odd[w] = 0; even[w] = 0; }
   if (!ACC) {
@@ -109,25 +127,15 @@ __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, cons
     }
   }
   const int mo = ACC ? 0 : nd >> 1, me = ACC ? 0 : (nd + 1) >> 1;
-  ge_p3 a1, a2;
-  // C1 = r B
-  uint32_t o1[8], e1[8];
-  split_odd_even(o1, e1, e);
-  ge_p3_0(a1);
-  comb_run<8>(a1, tb, tb, o1, e1, 32, 32);
-  // C2 = r H + m B, sharing the four doublings: rows of H for r's digits,
-  // rows of B for the message's
   split_odd_even(odd, even, e);
+  ge_p3 a2;
   ge_p3_0(a2);
   comb_run<11>(a2, th, tb, odd, even, 32 + mo, 32 + me);
   if (ACC) {
     ge_p3 o;
-    load_p3(o, c1, i);
-    ge_p3_add(a1, o, a1);
     load_p3(o, c2, i);
     ge_p3_add(a2, o, a2);
   }
-  store_p3(c1, i, a1);
   store_p3(c2, i, a2);
 }
 

@@ -132,18 +132,31 @@ __global__ void __launch_bounds__(TPB) k_build_comb(uint32_t *table, PtView src,
 // rerandomisation).  The comb runs for both tables at once so the four
 // middle doublings of r H are shared with m B; the message uses ND digits
 // (ND = 0 for rerandomisation), a public bound on its bit length.
-template <bool ACC>
-__global__ void __launch_bounds__(TPB) k_encrypt(PtView c1, PtView c2, const uint32_t *btab,
-                                                 const uint32_t *htab, const uint64_t *m, int nd,
-                                                 const uint32_t *r, size_t n) {
+template <bool ACC, int BS>
+__device__ __forceinline__ void encrypt_block(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
+                                              const uint64_t *m, int nd, const uint32_t *r, size_t n) {
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
   lds_load_table(th, htab);
   __syncthreads();
-  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  const size_t i = blockIdx.x * (size_t)BS + threadIdx.x;
   if (i >= n) return;
   encrypt_one<ACC>(c1, c2, i, tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
+}
+
+template <bool ACC>
+__global__ void __launch_bounds__(TPB) k_encrypt(PtView c1, PtView c2, const uint32_t *btab,
+                                                 const uint32_t *htab, const uint64_t *m, int nd,
+                                                 const uint32_t *r, size_t n) {
+  encrypt_block<ACC, TPB>(c1, c2, btab, htab, m, nd, r, n);
+}
+
+template <bool ACC>
+__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_encrypt_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
+               const uint32_t *r, size_t n) {
+  encrypt_block<ACC, ETPB>(c1, c2, btab, htab, m, nd, r, n);
 }
 
 // ------------------------------------------------------ encode / decode
@@ -329,10 +342,19 @@ hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
                           hipStream_t s) {
   if (!n) return hipSuccess;
-  if (accumulate)
-    hipLaunchKernelGGL(k_encrypt<true>, dim3(grid_for(n)), dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
-  else
-    hipLaunchKernelGGL(k_encrypt<false>, dim3(grid_for(n)), dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  if (n >= ENC_WIDE_MIN) {
+    const dim3 grid(grid_for(n, ETPB));
+    if (accumulate)
+      hipLaunchKernelGGL(k_encrypt_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+    else
+      hipLaunchKernelGGL(k_encrypt_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  } else {
+    const dim3 grid(grid_for(n));
+    if (accumulate)
+      hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+    else
+      hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  }
   FT_LAUNCH_CHECK();
 }
 

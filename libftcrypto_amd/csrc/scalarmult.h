@@ -7,6 +7,14 @@
 namespace ft {
 
 constexpr int TPB = 256;
+// Encryption kernels hold two 28 KB comb tables in LDS per block.  For large
+// batches, blocks of 512 lanes limited to 128 registers (4 spilled words) run
+// two blocks per CU, four waves per SIMD: 6 % faster than 256-lane blocks at
+// two waves (384-lane blocks, three waves uneven over the SIMDs, were slower
+// than both).  Below ENC_WIDE_MIN elements there are too few 512-lane blocks
+// to fill 256 CUs, and 256-lane blocks are used.
+constexpr int ETPB = 512;
+constexpr size_t ENC_WIDE_MIN = (size_t)ETPB * 512;
 
 // ------------------------------------------------------------ digits
 //
@@ -136,7 +144,7 @@ __device__ __forceinline__ void dbl4(ge_p3 &h) {
 }
 
 __device__ __forceinline__ void lds_load_table(uint32_t *lds, const uint32_t *g) {
-  for (int i = threadIdx.x; i < COMB_TABLE_WORDS / 4; i += TPB)
+  for (int i = threadIdx.x; i < COMB_TABLE_WORDS / 4; i += blockDim.x)
     reinterpret_cast<uint4 *>(lds)[i] = reinterpret_cast<const uint4 *>(g)[i];
 }
 

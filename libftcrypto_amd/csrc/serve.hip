@@ -57,18 +57,30 @@ __device__ __forceinline__ int find_item(const BItem *it, int cnt, uint64_t t) {
   return lo;
 }
 
-__global__ void __launch_bounds__(TPB) kb_encrypt(const BItem *items, int cnt, uint64_t total,
-                                                  const uint32_t *btab, const uint32_t *htab, int nd) {
+template <int BS>
+__device__ __forceinline__ void kb_encrypt_body(const BItem *items, int cnt, uint64_t total, const uint32_t *btab,
+                                                const uint32_t *htab, int nd) {
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
   lds_load_table(th, htab);
   __syncthreads();
-  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t = blockIdx.x * (uint64_t)BS + threadIdx.x;
   if (t >= total) return;
   const BItem &it = items[find_item(items, cnt, t)];
   const uint64_t i = t - it.start;
   encrypt_one<false>(it.a1, it.a2, i, tb, th, it.m[i], nd, it.r + i * 8);
+}
+
+__global__ void __launch_bounds__(TPB) kb_encrypt(const BItem *items, int cnt, uint64_t total,
+                                                  const uint32_t *btab, const uint32_t *htab, int nd) {
+  kb_encrypt_body<TPB>(items, cnt, total, btab, htab, nd);
+}
+
+// large batches: 512-lane blocks at four waves per SIMD (as k_encrypt_wide)
+__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+kb_encrypt_wide(const BItem *items, int cnt, uint64_t total, const uint32_t *btab, const uint32_t *htab, int nd) {
+  kb_encrypt_body<ETPB>(items, cnt, total, btab, htab, nd);
 }
 
 __global__ void __launch_bounds__(TPB) kb_add(const BItem *items, int cnt, uint64_t total) {
@@ -373,8 +385,12 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
   const unsigned grid = (unsigned)((start + TPB - 1) / TPB);
   if (e == hipSuccess && start > 0) {
     if (kind == K_ENCRYPT) {
-      hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
-                         key_table(S->key), nd);
+      if (start >= ENC_WIDE_MIN)
+        hipLaunchKernelGGL(kb_encrypt_wide, dim3((unsigned)((start + ETPB - 1) / ETPB)), dim3(ETPB), 0, sl.st,
+                           ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
+      else
+        hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
+                           key_table(S->key), nd);
     } else if (kind == K_ADD) {
       hipLaunchKernelGGL(kb_add, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     } else if (kind == K_MUL) {
