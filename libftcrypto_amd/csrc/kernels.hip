@@ -152,6 +152,22 @@ __global__ void __launch_bounds__(TPB) k_encrypt(PtView c1, PtView c2, const uin
   encrypt_block<ACC, TPB>(c1, c2, btab, htab, m, nd, r, n);
 }
 
+// small batches: two lanes per ciphertext (encrypt_half)
+template <bool ACC>
+__global__ void __launch_bounds__(TPB) k_encrypt_split(PtView c1, PtView c2, const uint32_t *btab,
+                                                       const uint32_t *htab, const uint64_t *m, int nd,
+                                                       const uint32_t *r, size_t n) {
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const size_t t = blockIdx.x * (size_t)TPB + threadIdx.x;
+  const size_t i = t >> 1;
+  if (i >= n) return;
+  encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
+}
+
 template <bool ACC>
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_encrypt_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
@@ -348,12 +364,18 @@ hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint
       hipLaunchKernelGGL(k_encrypt_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
     else
       hipLaunchKernelGGL(k_encrypt_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
-  } else {
+  } else if (n >= ENC_SPLIT_MAX) {
     const dim3 grid(grid_for(n));
     if (accumulate)
       hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
     else
       hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  } else {
+    const dim3 grid(grid_for(2 * n));
+    if (accumulate)
+      hipLaunchKernelGGL(k_encrypt_split<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+    else
+      hipLaunchKernelGGL(k_encrypt_split<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
   }
   FT_LAUNCH_CHECK();
 }

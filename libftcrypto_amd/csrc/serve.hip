@@ -77,6 +77,22 @@ __global__ void __launch_bounds__(TPB) kb_encrypt(const BItem *items, int cnt, u
   kb_encrypt_body<TPB>(items, cnt, total, btab, htab, nd);
 }
 
+// small batches: two lanes per ciphertext (as k_encrypt_split)
+__global__ void __launch_bounds__(TPB) kb_encrypt_split(const BItem *items, int cnt, uint64_t total,
+                                                        const uint32_t *btab, const uint32_t *htab, int nd) {
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const uint64_t t2 = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t = t2 >> 1;
+  if (t >= total) return;
+  const BItem &it = items[find_item(items, cnt, t)];
+  const uint64_t i = t - it.start;
+  encrypt_half<false>(it.a1, it.a2, i, (uint32_t)(t2 & 1), tb, th, it.m[i], nd, it.r + i * 8);
+}
+
 // large batches: 512-lane blocks at four waves per SIMD (as k_encrypt_wide)
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 kb_encrypt_wide(const BItem *items, int cnt, uint64_t total, const uint32_t *btab, const uint32_t *htab, int nd) {
@@ -388,9 +404,12 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
       if (start >= ENC_WIDE_MIN)
         hipLaunchKernelGGL(kb_encrypt_wide, dim3((unsigned)((start + ETPB - 1) / ETPB)), dim3(ETPB), 0, sl.st,
                            ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
-      else
+      else if (start >= ENC_SPLIT_MAX)
         hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
                            key_table(S->key), nd);
+      else
+        hipLaunchKernelGGL(kb_encrypt_split, dim3((unsigned)((2 * start + TPB - 1) / TPB)), dim3(TPB), 0, sl.st,
+                           ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
     } else if (kind == K_ADD) {
       hipLaunchKernelGGL(kb_add, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     } else if (kind == K_MUL) {
