@@ -57,26 +57,19 @@ __device__ __forceinline__ void recode_u32_r8(uint32_t e[16], uint32_t k) {
   }
 }
 
-// a[i] = k a[i] and, when two, b[i] = k b[i]; W = 4 (radix 16, 8-entry
-// table, 9 windows) or W = 3 (radix 8, 4-entry table, 11 windows)
+// v[i] = k v[i] for a 32-bit k; W = 4 (radix 16, 8-entry table, 9 windows)
+// or W = 3 (radix 8, 4-entry table, 11 windows).  The kernels give each point
+// of a ciphertext its own lane.
 template <int W>
-__device__ __forceinline__ void mul_u32_one(PtView a, PtView b, int two, size_t i, uint32_t k) {
+__device__ __forceinline__ void mul_u32_point(PtView v, size_t i, uint32_t k) {
   uint32_t e[16];
   ge_p3 p, h;
   if (W == 4) recode_u32(e, k);
   else recode_u32_r8(e, k);
-  load_p3(p, a, i);
+  load_p3(p, v, i);
   if (W == 4) varbase_windows(h, p, e, 55);
   else varbase_windows8(h, p, e, 53);
-  store_p3(a, i, h);
-  if (two) {
-    if (W == 4) recode_u32(e, k);
-    else recode_u32_r8(e, k);
-    load_p3(p, b, i);
-    if (W == 4) varbase_windows(h, p, e, 55);
-    else varbase_windows8(h, p, e, 53);
-    store_p3(b, i, h);
-  }
+  store_p3(v, i, h);
 }
 
 // c1[i] = r B, c2[i] = m B + r H (ACC: c1[i] += r B, c2[i] += r H).  The two

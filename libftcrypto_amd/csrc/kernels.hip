@@ -83,18 +83,21 @@ __global__ void __launch_bounds__(TPB) k_fixed_base(PtView out, const void *sc, 
 // 8-entry table in registers at one wave per SIMD; radix 8's 4-entry table
 // leaves room for two waves per SIMD (a few spilled words), which measured
 // 23 % faster than radix 16 at 2^22 ciphertexts.
+// lanes [0, n) multiply a, lanes [n, 2n) multiply b (when two)
 __global__ void __launch_bounds__(TPB) k_mul_u32_r16(PtView a, PtView b, int two, const uint32_t *k,
                                                      size_t n) {
-  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
-  if (i >= n) return;
-  mul_u32_one<4>(a, b, two, i, k[i]);
+  const size_t t = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (t >= (two ? 2 * n : n)) return;
+  const size_t i = t < n ? t : t - n;
+  mul_u32_point<4>(t < n ? a : b, i, k[i]);
 }
 
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_mul_u32_r8(PtView a, PtView b, int two, const uint32_t *k, size_t n) {
-  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
-  if (i >= n) return;
-  mul_u32_one<3>(a, b, two, i, k[i]);
+  const size_t t = blockIdx.x * (size_t)TPB + threadIdx.x;
+  if (t >= (two ? 2 * n : n)) return;
+  const size_t i = t < n ? t : t - n;
+  mul_u32_point<3>(t < n ? a : b, i, k[i]);
 }
 
 // ------------------------------------------------------ comb table build
@@ -336,10 +339,11 @@ hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, con
 
 hipError_t launch_mul_u32_w(PtView a, PtView b, bool two, const uint32_t *k, size_t n, int w, hipStream_t s) {
   if (!n) return hipSuccess;
+  const dim3 grid(grid_for(two ? 2 * n : n));
   if (w == 3)
-    hipLaunchKernelGGL(k_mul_u32_r8, dim3(grid_for(n)), dim3(TPB), 0, s, a, b, (int)two, k, n);
+    hipLaunchKernelGGL(k_mul_u32_r8, grid, dim3(TPB), 0, s, a, b, (int)two, k, n);
   else if (w == 4)
-    hipLaunchKernelGGL(k_mul_u32_r16, dim3(grid_for(n)), dim3(TPB), 0, s, a, b, (int)two, k, n);
+    hipLaunchKernelGGL(k_mul_u32_r16, grid, dim3(TPB), 0, s, a, b, (int)two, k, n);
   else
     return hipErrorInvalidValue;
   FT_LAUNCH_CHECK();

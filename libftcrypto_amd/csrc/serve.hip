@@ -108,22 +108,25 @@ __global__ void __launch_bounds__(TPB) kb_add(const BItem *items, int cnt, uint6
   add_one<false>(it.a2, i, it.b2, i);
 }
 
+// lane 2 t + h multiplies half h of ciphertext t of the batch
 __global__ void __launch_bounds__(TPB) kb_mul_u32_r16(const BItem *items, int cnt, uint64_t total) {
-  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t2 = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t = t2 >> 1;
   if (t >= total) return;
   const BItem &it = items[find_item(items, cnt, t)];
   const uint64_t i = t - it.start;
-  mul_u32_one<4>(it.a1, it.a2, 1, i, it.k[i]);
+  mul_u32_point<4>((t2 & 1) ? it.a2 : it.a1, i, it.k[i]);
 }
 
 // radix 8 at two waves per SIMD, as k_mul_u32_r8
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 kb_mul_u32_r8(const BItem *items, int cnt, uint64_t total) {
-  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t2 = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t = t2 >> 1;
   if (t >= total) return;
   const BItem &it = items[find_item(items, cnt, t)];
   const uint64_t i = t - it.start;
-  mul_u32_one<3>(it.a1, it.a2, 1, i, it.k[i]);
+  mul_u32_point<3>((t2 & 1) ? it.a2 : it.a1, i, it.k[i]);
 }
 
 }  // namespace ft
@@ -413,10 +416,11 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
     } else if (kind == K_ADD) {
       hipLaunchKernelGGL(kb_add, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     } else if (kind == K_MUL) {
+      const dim3 grid2((unsigned)((2 * start + TPB - 1) / TPB));
       if (tune_for(start).u32w == 3)
-        hipLaunchKernelGGL(kb_mul_u32_r8, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
+        hipLaunchKernelGGL(kb_mul_u32_r8, grid2, dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
       else
-        hipLaunchKernelGGL(kb_mul_u32_r16, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
+        hipLaunchKernelGGL(kb_mul_u32_r16, grid2, dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     }
     e = hipGetLastError();
   }
