@@ -520,7 +520,36 @@ def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
     return {"value": rate, "unit": "scalar-mults/s", "cores": nt, "kind": "port",
             "sample": "%d of each of the N=2^k fixed-base, variable-base and u32 mults (ct = 2 point mults)" % samp,
             "seconds": round(total, 2),
-            "one_thread": {"value": 4 * s1 / one, "sample": "%d of each kind" % s1, "seconds": round(one, 2)}}
+            "one_thread": {"value": 4 * s1 / one, "sample": "%d of each kind" % s1, "seconds": round(one, 2)},
+            "libsodium_one_thread": libsodium_rates(k_fixed, k_var)}
+
+
+def libsodium_rates(k_fixed, k_var, count=2048):
+    """libsodium 1.0.18's public API on one host thread (the reference CPU
+    path's arithmetic library); every call decodes and encodes 32-byte points,
+    so these are per-operation rates, not array throughput."""
+    lib = None
+    for path in ("/opt/conda/lib/libsodium.so.23", "/opt/conda/lib/libsodium.so"):
+        try:
+            lib = ctypes.CDLL(path)
+            break
+        except OSError:
+            continue
+    if lib is None or lib.sodium_init() < 0:
+        return None
+    out = ctypes.create_string_buffer(32)
+    ks = [bytes(k_fixed[i]) for i in range(count)]
+    kv = [bytes(k_var[i]) for i in range(count)]
+    t0 = time.time()
+    for k in ks:
+        lib.crypto_scalarmult_ed25519_base_noclamp(out, k)
+    t1 = time.time()
+    pt = bytes(out.raw)
+    for k in kv:
+        lib.crypto_scalarmult_ed25519_noclamp(out, k, pt)
+    t2 = time.time()
+    return {"fixed_base_per_s": count / (t1 - t0), "variable_base_per_s": count / (t2 - t1),
+            "sample": "%d of each" % count, "note": "per call: decode and encode of 32-byte points included"}
 
 
 def main():
