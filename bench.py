@@ -435,21 +435,29 @@ def bench_cfg3(ft, hip, key, seed, args):
     d_off, d_cols = hip.upload(row_off), hip.upload(cols)
     c1, c2, o1, o2 = (ft.GeArray.alloc(nn) for _ in range(4))
     ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 1, st.h))
-    scr = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ne))
+    # scratch with room for the point-contiguous copy of the source
+    scr = ft.GeArray.alloc(L.ftx_gather_scratch_points(ne, nn))
+    scr_planes = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ne))
 
-    def step():
-        ft.check(L.ftx_segmented_sum_dev(o1.h, d_off, nn, ne, d_cols, c1.h, scr.h, st.h))
-        ft.check(L.ftx_segmented_sum_dev(o2.h, d_off, nn, ne, d_cols, c2.h, scr.h, st.h))
+    def step(sc=scr):
+        ft.check(L.ftx_segmented_sum_dev(o1.h, d_off, nn, ne, d_cols, c1.h, sc.h, st.h))
+        ft.check(L.ftx_segmented_sum_dev(o2.h, d_off, nn, ne, d_cols, c2.h, sc.h, st.h))
         ft.check(L.ftx_rerandomize_dev(o1.h, o2.h, key.h, rho, st.h))
     st.synchronize()
     with ft.Graph(st) as g:
         step()
     ms = T(lambda: g.launch(), reps=3)
     eager_ms = T(step, reps=3)
+    planes_ms = T(lambda: step(scr_planes), reps=3)
+    gather_ms = T(lambda: ft.check(L.ftx_segmented_sum_dev(o1.h, d_off, nn, ne, d_cols, c1.h, scr.h, st.h)), reps=3)
+    gather_planes_ms = T(lambda: ft.check(L.ftx_segmented_sum_dev(o1.h, d_off, nn, ne, d_cols, c1.h,
+                                                                   scr_planes.h, st.h)), reps=3)
     nonempty = int((np.diff(row_off) > 0).sum())
     adds = 2 * (ne - nonempty) + 2 * nn
     mults = 2 * nn
     return {"graph_ms_per_step": round(ms, 4), "eager_ms_per_step": round(eager_ms, 4),
+            "eager_ms_per_step_gather_from_planes": round(planes_ms, 4),
+            "gather_sum_ms": round(gather_ms, 4), "gather_sum_from_planes_ms": round(gather_planes_ms, 4),
             "point_adds": adds, "fixed_base_mults": mults,
             "valu_frac": (adds * PRODUCTS["point_add"] + mults * PRODUCTS["fixed_base"]) / (ms * 1e-3) / PEAK_MUL32}
 

@@ -74,8 +74,12 @@ ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
 ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                              const void *d_r, void *stream);
 /* out must have n_seg elements; d_gather may be NULL (plain CSR); d_scratch
- * must hold ftx_segsum_scratch_points(n_elems) points (NULL: library-owned). */
+ * must hold ftx_segsum_scratch_points(n_elems) points (NULL: library-owned).
+ * With d_gather, a scratch of ftx_gather_scratch_points(n_elems, len(src))
+ * points also takes a point-contiguous copy of src, from which each gathered
+ * point is one 128-byte line instead of four (faster random gathers). */
 size_t ftx_segsum_scratch_points(size_t n_elems);
+size_t ftx_gather_scratch_points(size_t n_elems, size_t n_src);
 ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg,
                                size_t n_elems, const void *d_gather,
                                const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream);

@@ -199,10 +199,12 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
 
 // Level 1: lane t walks elements [2^CH t, 2^CH (t + 1)) of the input (through
 // the gather indices when given), segment boundaries from the offsets.
-template <int CH>
+// MODE 0: elements in order; 1: gathered from the planes of src; 2:
+// gathered from the point-contiguous copy `aos`
+template <int CH, int MODE>
 __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
                                                      size_t n, const int64_t *off, size_t nseg,
-                                                     const int32_t *gather) {
+                                                     const int32_t *gather, PtView aos, size_t aos_base) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
@@ -244,16 +246,44 @@ __global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int
     else acc = x;
     touched = true;
   };
-  if (gather) {
-    // one element ahead: the next source point is in flight during the add
-    PW qc, qn;
-    ld_pw(qc, src, (size_t)gather[clo]);
+  if (MODE != 0) {
+    // four gathered points in flight per group: their indices (in one 16-byte
+    // load when the group is whole and the index array 16-byte aligned; the
+    // chunk start is a multiple of 16), then all their lines; from the
+    // point-contiguous copy when there is one (one line per point)
+    const bool vec = ((uintptr_t)gather & 15) == 0;
 #pragma unroll 1
-    for (int64_t i = clo; i < cend; ++i) {
-      ld_pw(qn, src, (size_t)gather[(i + 1 < cend) ? i + 1 : i]);
-      pin_pw(qn);
-      if (take(i)) { un_pw(x, qc); accum(); }
-      qc = qn;
+    for (int64_t g = clo; g < cend; g += 4) {
+      int4 ix;
+      if (vec && g + 4 <= cend) {
+        ix = *reinterpret_cast<const int4 *>(gather + g);
+      } else {
+        ix.x = gather[g];
+        ix.y = g + 1 < cend ? gather[g + 1] : ix.x;
+        ix.z = g + 2 < cend ? gather[g + 2] : ix.x;
+        ix.w = g + 3 < cend ? gather[g + 3] : ix.x;
+      }
+      const size_t j0 = (size_t)ix.x;
+      const size_t j1 = (size_t)(g + 1 < cend ? ix.y : ix.x);
+      const size_t j2 = (size_t)(g + 2 < cend ? ix.z : ix.x);
+      const size_t j3 = (size_t)(g + 3 < cend ? ix.w : ix.x);
+      PW q0, q1, q2, q3;
+      if (MODE == 2) {
+        ld_pw_aos(q0, aos, aos_base, j0);
+        ld_pw_aos(q1, aos, aos_base, j1);
+        ld_pw_aos(q2, aos, aos_base, j2);
+        ld_pw_aos(q3, aos, aos_base, j3);
+      } else {
+        ld_pw(q0, src, j0);
+        ld_pw(q1, src, j1);
+        ld_pw(q2, src, j2);
+        ld_pw(q3, src, j3);
+      }
+      pin_pw(q0); pin_pw(q1); pin_pw(q2); pin_pw(q3);
+      if (take(g)) { un_pw(x, q0); accum(); }
+      if (g + 1 < cend && take(g + 1)) { un_pw(x, q1); accum(); }
+      if (g + 2 < cend && take(g + 2)) { un_pw(x, q2); accum(); }
+      if (g + 3 < cend && take(g + 3)) { un_pw(x, q3); accum(); }
     }
   } else {
 #pragma unroll 1
@@ -315,6 +345,19 @@ __global__ void __launch_bounds__(ATPB) k_seg_levelN(PtView out, PtView rec, int
   }
 }
 
+// point-contiguous copy of src into the scratch planes' tails (aos_slot)
+__global__ void __launch_bounds__(ATPB) k_to_aos(PtView aos, size_t aos_base, PtView src, size_t n) {
+  const size_t j = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  if (j >= n) return;
+  PW q;
+  ld_pw(q, src, j);
+  uint32_t *d = aos_point(aos, aos_base, j);
+  st_words(d, q.x);
+  st_words(d + 8, q.y);
+  st_words(d + 16, q.z);
+  st_words(d + 24, q.t);
+}
+
 __global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *off, size_t nseg) {
   const size_t s = blockIdx.x * (size_t)ATPB + threadIdx.x;
   if (s < nseg && off[s] == off[s + 1]) store_identity(out, s);
@@ -322,17 +365,28 @@ __global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *o
 
 static hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
                                    size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
-                                   int ch, int ch1, hipStream_t s) {
+                                   int ch, int ch1, PtView aos, size_t aos_base, hipStream_t s) {
   const size_t T = tiles(n, ch);
   if (!T || !nseg) return hipSuccess;
   const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
   if (level == 1) {
-    switch (ch) {
-      case 4: hipLaunchKernelGGL(k_seg_level1<4>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather); break;
-      case 5: hipLaunchKernelGGL(k_seg_level1<5>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather); break;
-      case 6: hipLaunchKernelGGL(k_seg_level1<6>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather); break;
+    const int mode = !gather ? 0 : (aos.w ? 2 : 1);
+#define FT_SEG1(C, M)                                                                                    \
+  hipLaunchKernelGGL((k_seg_level1<C, M>), grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather, \
+                     aos, aos_base)
+    switch (ch * 4 + mode) {
+      case 16: FT_SEG1(4, 0); break;
+      case 17: FT_SEG1(4, 1); break;
+      case 18: FT_SEG1(4, 2); break;
+      case 20: FT_SEG1(5, 0); break;
+      case 21: FT_SEG1(5, 1); break;
+      case 22: FT_SEG1(5, 2); break;
+      case 24: FT_SEG1(6, 0); break;
+      case 25: FT_SEG1(6, 1); break;
+      case 26: FT_SEG1(6, 2); break;
       default: return hipErrorInvalidValue;
     }
+#undef FT_SEG1
   } else {
     switch (ch) {
       case 4: hipLaunchKernelGGL(k_seg_levelN<4>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
@@ -359,13 +413,29 @@ size_t segsum_scratch_points(size_t n) {
   return R + R / 4 + 8;
 }
 
+// the point-contiguous source copy starts at plane slot segsum_scratch_points(n)
+// (after the records and, in plane 0, their keys); plane q holds points
+// q, q + 4, ... at four slots each: 4 ceil(n_src / 4) slots per plane
+size_t segsum_gather_scratch_points(size_t n_elems, size_t n_src) {
+  return segsum_scratch_points(n_elems) + (n_src + 3) / 4 * 4;
+}
+
 hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
-                            const int32_t *gather, PtView src, PtView scratch, int ch1, int chu,
+                            const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int chu,
                             hipStream_t s) {
   if (nseg == 0) return hipSuccess;
   if (ch1 < CH_MIN || ch1 > CH_MAX || chu < CH_MIN || chu > CH_MAX) return hipErrorInvalidValue;
   hipError_t e = launch_seg_empty(out, d_off, nseg, s);
   if (e != hipSuccess) return e;
+  PtView aos{nullptr, 0};
+  const size_t aos_base = segsum_scratch_points(n_elems);
+  if (gather && n_src && scratch.cap >= segsum_gather_scratch_points(n_elems, n_src)) {
+    aos = scratch;
+    hipLaunchKernelGGL(k_to_aos, dim3((unsigned)((n_src + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, aos, aos_base,
+                       src, n_src);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const size_t t1 = tiles(n_elems, ch1);
   const size_t Rmax = segsum_records(n_elems, CH_MIN, CH_MIN);
   PtView bufA{scratch.w, scratch.cap};
@@ -383,7 +453,7 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
     const bool last = T <= 1;
     PtView rec = last ? PtView{nullptr, 0} : (use_a ? bufA : bufB);
     int64_t *keys = use_a ? keysA : keysB;
-    e = launch_seg_level(out, rec, keys, elems, in_keys, n, d_off, nseg, gather, level, ch, ch1, s);
+    e = launch_seg_level(out, rec, keys, elems, in_keys, n, d_off, nseg, gather, level, ch, ch1, aos, aos_base, s);
     if (e != hipSuccess || last) return e;
     elems = rec;
     in_keys = keys;
@@ -395,9 +465,9 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
 }
 
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
-                         PtView src, PtView scratch, hipStream_t s) {
+                         PtView src, size_t n_src, PtView scratch, hipStream_t s) {
   const AggTune t = tune_for(n_elems);
-  return segsum_launch_ch(out, d_off, nseg, n_elems, gather, src, scratch, t.seg1, t.segu, s);
+  return segsum_launch_ch(out, d_off, nseg, n_elems, gather, src, n_src, scratch, t.seg1, t.segu, s);
 }
 
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s) {

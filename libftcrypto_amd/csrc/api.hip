@@ -195,7 +195,7 @@ size_t seg_scratch_points(size_t n) { return segsum_scratch_points(n); }
 
 ft_error segsum_run(ft_ge25519_array out, const int64_t *d_off, size_t nseg, size_t n_elems,
                     const int32_t *d_gather, const_ft_ge25519_array src, PtView scratch, hipStream_t s) {
-  CK(segsum_launch(view(out), d_off, nseg, n_elems, d_gather, view(src), scratch, s));
+  CK(segsum_launch(view(out), d_off, nseg, n_elems, d_gather, view(src), src->n, scratch, s));
   return FT_ERR_NO_ERROR;
 }
 
@@ -709,7 +709,7 @@ ft_error ft_reduce_isum(ft_ge25519_array res, const size_t *idxs, size_t n_idxs,
   CKE(dord.upload(order.data(), order.size() * 4));
   ArrGuard sums, scratch;
   CKE(alloc_array(&sums.a, m));
-  CKE(alloc_array(&scratch.a, seg_scratch_points(n_idxs)));
+  CKE(alloc_array(&scratch.a, segsum_gather_scratch_points(n_idxs, summand->n)));
   CKE(segsum_run(sums.a, (const int64_t *)doff.p, m, n_idxs, (const int32_t *)dord.p, summand,
                  view(scratch.a), G.stream));
   CK(launch_add(view(res), view(sums.a), m, false, G.stream));
@@ -927,6 +927,9 @@ void ftx_graph_free(ftx_graph graph) {
 }
 
 size_t ftx_segsum_scratch_points(size_t n_elems) { return seg_scratch_points(n_elems); }
+size_t ftx_gather_scratch_points(size_t n_elems, size_t n_src) {
+  return segsum_gather_scratch_points(n_elems, n_src);
+}
 
 ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg, size_t n_elems,
                                const void *d_gather, const_ft_ge25519_array src, ft_ge25519_array scratch,
@@ -936,7 +939,7 @@ ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size
   const size_t need = seg_scratch_points(n_elems);
   ArrGuard own;
   if (!scratch) {
-    CKE(alloc_array(&own.a, need));
+    CKE(alloc_array(&own.a, d_gather ? segsum_gather_scratch_points(n_elems, src->n) : need));
     scratch = own.a;
   } else if (scratch->cap < need) {
     return FT_ERR_INSUFFICIENT_SPACE;

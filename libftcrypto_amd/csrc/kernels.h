@@ -51,10 +51,15 @@ hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipSt
 hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s);
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s);
 size_t segsum_scratch_points(size_t n_elems);     // enough for every schedule
+// gather form with n_src source points: a scratch of at least
+// segsum_gather_scratch_points(n_elems, n_src) points also holds a
+// point-contiguous copy of the source, from which every gathered point is
+// one 128-byte line (a smaller scratch gathers from the four planes)
+size_t segsum_gather_scratch_points(size_t n_elems, size_t n_src);
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
-                         PtView src, PtView scratch, hipStream_t s);
+                         PtView src, size_t n_src, PtView scratch, hipStream_t s);
 hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
-                            const int32_t *gather, PtView src, PtView scratch, int ch1, int chu,
+                            const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int chu,
                             hipStream_t s);
 
 // api.hip: device tables owned by the library

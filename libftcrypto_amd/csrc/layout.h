@@ -140,6 +140,21 @@ __device__ __forceinline__ void pin_p3(ge_p3 &p) {
   pin_fe(p.T);
 }
 
+// Point-contiguous copy kept in the tail slots of a scratch array's planes
+// (from slot `base` on): point j's 128 bytes (X, Y, Z, T) are the four
+// consecutive slots base + 4 (j >> 2) .. +3 of plane j & 3.
+__device__ __forceinline__ uint32_t *aos_point(const PtView v, size_t base, size_t j) {
+  return v.w + ((j & 3) * v.cap + base + 4 * (j >> 2)) * 8;
+}
+
+__device__ __forceinline__ void ld_pw_aos(PW &p, const PtView v, size_t base, size_t j) {
+  const uint32_t *s = aos_point(v, base, j);
+  ld_words(p.x, s);
+  ld_words(p.y, s + 8);
+  ld_words(p.z, s + 16);
+  ld_words(p.t, s + 24);
+}
+
 // points g .. g+3 (indices at or past `hi` re-read point g; their values are unused)
 #define FT_LOAD4(q0, q1, q2, q3, v, g, hi)                   \
   do {                                                       \

@@ -437,8 +437,8 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
     } else {
       PtView sv{reinterpret_cast<uint32_t *>(scr), pts};
       const int64_t *doff = reinterpret_cast<const int64_t *>(sl.dev);
-      e = segsum_launch(D.c1, doff, r.nseg, n_el, nullptr, Sx.c1, sv, sl.st);
-      if (e == hipSuccess) e = segsum_launch(D.c2, doff, r.nseg, n_el, nullptr, Sx.c2, sv, sl.st);
+      e = segsum_launch(D.c1, doff, r.nseg, n_el, nullptr, Sx.c1, 0, sv, sl.st);
+      if (e == hipSuccess) e = segsum_launch(D.c2, doff, r.nseg, n_el, nullptr, Sx.c2, 0, sv, sl.st);
       (void)hipFreeAsync(scr, sl.st);
     }
   }

@@ -272,7 +272,7 @@ ft_error ftx_autotune(int max_class, const char *path) {
     for (int a = CH_MIN; a <= CH_MAX; ++a)
       for (int b = CH_MIN; b <= CH_MAX; ++b) {
         ft_error e = time_ms(&t, ev, st.s, reps, [&] {
-          return segsum_launch_ch(O, (const int64_t *)doffc.p, nseg, n, nullptr, P, S, a, b, st.s);
+          return segsum_launch_ch(O, (const int64_t *)doffc.p, nseg, n, nullptr, P, 0, S, a, b, st.s);
         });
         if (e != FT_ERR_NO_ERROR) return e;
         if (t < bt) { bt = t; best.seg1 = a; best.segu = b; }

@@ -162,11 +162,15 @@ class DevBuf:
         self.hip.hipFree(self.p)
 
 
-@pytest.mark.parametrize("n_nodes,n_edges", [(1 << 12, 1 << 15), (1 << 22, 1 << 25)])
-def test_config3_sparse_step_graph(n_nodes, n_edges):
+@pytest.mark.parametrize("n_nodes,n_edges,contiguous", [(1 << 12, 1 << 15, False), (1 << 12, 1 << 15, True),
+                                                        (1 << 22, 1 << 25, True)])
+def test_config3_sparse_step_graph(n_nodes, n_edges, contiguous):
     """Gather-sum of in-neighbours then rerandomisation, captured in a hipGraph
     and replayed three times with eager work in between; every replay must
-    equal the eager result, which must satisfy the closed form."""
+    equal the eager result, which must satisfy the closed form.  With
+    `contiguous` the scratch also holds the point-contiguous source copy the
+    gather then reads (one line per point); without, it gathers from the
+    four coordinate planes."""
     seed = W.SEED_BASE + 3
     x, H, key = keypair(seed)
     rng = np.random.default_rng(seed)
@@ -176,7 +180,9 @@ def test_config3_sparse_step_graph(n_nodes, n_edges):
     rho = ft.random_scalars(n_nodes, seed, 2)
     c1, c2 = ft.encrypt(key, m, r, msg_bits=1)
     out1, out2 = ft.GeArray.alloc(n_nodes), ft.GeArray.alloc(n_nodes)
-    scratch = ft.GeArray.alloc(ft.lib().ftx_segsum_scratch_points(n_edges))
+    Lb = ft.lib()
+    scratch = ft.GeArray.alloc(Lb.ftx_gather_scratch_points(n_edges, n_nodes) if contiguous
+                               else Lb.ftx_segsum_scratch_points(n_edges))
     d_off, d_cols, d_rho = DevBuf(row_off), DevBuf(cols), DevBuf(rho)
     st = ft.Stream()
     sparse_step(st, out1, out2, c1, c2, d_off.p, n_nodes, n_edges, d_cols.p, key, d_rho.p, scratch)
