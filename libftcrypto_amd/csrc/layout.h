@@ -110,6 +110,31 @@ __device__ __forceinline__ void st_pw(const PtView v, size_t i, const PW &p) {
   st_words(v.w + (3 * v.cap + i) * 8, p.t);
 }
 
+// points g .. g + cnt - 1 (cnt <= 4) stored coordinate by coordinate, so that
+// each coordinate's line (four consecutive points, g a multiple of 4) is
+// written by consecutive stores
+__device__ __forceinline__ void st_pw4(const PtView v, size_t g, int cnt, const PW &a, const PW &b, const PW &c,
+                                       const PW &d) {
+  uint32_t *px = v.w + (0 * v.cap + g) * 8, *py = v.w + (1 * v.cap + g) * 8;
+  uint32_t *pz = v.w + (2 * v.cap + g) * 8, *pt = v.w + (3 * v.cap + g) * 8;
+  st_words(px, a.x);
+  if (cnt > 1) st_words(px + 8, b.x);
+  if (cnt > 2) st_words(px + 16, c.x);
+  if (cnt > 3) st_words(px + 24, d.x);
+  st_words(py, a.y);
+  if (cnt > 1) st_words(py + 8, b.y);
+  if (cnt > 2) st_words(py + 16, c.y);
+  if (cnt > 3) st_words(py + 24, d.y);
+  st_words(pz, a.z);
+  if (cnt > 1) st_words(pz + 8, b.z);
+  if (cnt > 2) st_words(pz + 16, c.z);
+  if (cnt > 3) st_words(pz + 24, d.z);
+  st_words(pt, a.t);
+  if (cnt > 1) st_words(pt + 8, b.t);
+  if (cnt > 2) st_words(pt + 16, c.t);
+  if (cnt > 3) st_words(pt + 24, d.t);
+}
+
 __device__ __forceinline__ void pk_pw(PW &p, const ge_p3 &r) {
   fe_store_words(p.x, r.X);
   fe_store_words(p.y, r.Y);
