@@ -236,6 +236,58 @@ __device__ __forceinline__ void build_cached4(ge_cached tab[4], const ge_p3 &p) 
   ge_p3_to_cached(tab[3], m3);
 }
 
+// The same four multiples scaled to one shared Z (lambda_j = prod of the
+// other entries' Z): each entry then needs three field elements, the select
+// merges 27 words per entry instead of 36 and the table takes 117 registers
+// instead of 144, which keeps the 256-bit window loop free of spills.  It
+// costs 19 multiplies per scalar mult: 1.6 % faster over 85 windows, slower
+// over the 11 windows of a 32-bit multiplier (which keeps the plain table).
+struct ge_cached3 { fe YpX, YmX, T2d; };
+
+__device__ __forceinline__ void scale_cached3(ge_cached3 &t, const ge_cached &c, const fe &l) {
+  fe_mul(t.YpX, c.YpX, l);
+  fe_mul(t.YmX, c.YmX, l);
+  fe_mul(t.T2d, c.T2d, l);
+}
+
+__device__ __forceinline__ void build_cached4_coz(ge_cached3 tab[4], fe &zc, const ge_p3 &p) {
+  ge_cached c[4];
+  build_cached4(c, p);
+  fe a, b, l;
+  fe_mul(a, c[0].Z, c[1].Z);
+  fe_mul(b, c[2].Z, c[3].Z);
+  fe_mul(zc, a, b);
+  fe_mul(l, c[1].Z, b);
+  scale_cached3(tab[0], c[0], l);
+  fe_mul(l, c[0].Z, b);
+  scale_cached3(tab[1], c[1], l);
+  fe_mul(l, c[3].Z, a);
+  scale_cached3(tab[2], c[2], l);
+  fe_mul(l, c[2].Z, a);
+  scale_cached3(tab[3], c[3], l);
+}
+
+// |b| * P from the shared-Z table; b = 0 gives the identity (zc, zc, zc, 0)
+__device__ __forceinline__ void cached_select4_coz(ge_cached &t, const ge_cached3 tab[4], const fe &zc, int b) {
+  const uint32_t neg = (uint32_t)b >> 31;
+  const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
+  t.YpX = zc;
+  t.YmX = zc;
+  fe_0(t.T2d);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      t.YpX.v[l] = m ? tab[j].YpX.v[l] : t.YpX.v[l];
+      t.YmX.v[l] = m ? tab[j].YmX.v[l] : t.YmX.v[l];
+      t.T2d.v[l] = m ? tab[j].T2d.v[l] : t.T2d.v[l];
+    }
+  }
+  t.Z = zc;
+  ge_cached_cneg(t, neg);
+}
+
 __device__ __forceinline__ void cached_select4(ge_cached &t, const ge_cached tab[4], int b) {
   const uint32_t neg = (uint32_t)b >> 31;
   const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
@@ -356,8 +408,9 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
   ge_p3_neg(np, p);
   fe_cmov(p.X, np.X, neg);
   fe_cmov(p.T, np.T, neg);
-  ge_cached tab[4];
-  build_cached4(tab, p);
+  ge_cached3 tab[4];
+  fe zc;
+  build_cached4_coz(tab, zc, p);
   ge_p3_0(h);
   ge_cached c;
   ge_p1p1 r;
@@ -366,7 +419,7 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
   asm volatile("" : "+s"(plo));
 #pragma unroll 1
   for (int q = 87; q > plo; --q) {
-    cached_select4(c, tab, pop_hi4(e, q));
+    cached_select4_coz(c, tab, zc, pop_hi4(e, q));
     ge_add(r, h, c);
     ge_p1p1_to_p2(s, r);
     ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
@@ -375,7 +428,7 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
     ge_p1p1_to_p3(h, r);
   }
   pin_p3(h);
-  cached_select4(c, tab, pop_hi4(e, plo));
+  cached_select4_coz(c, tab, zc, pop_hi4(e, plo));
   ge_add(r, h, c);
   ge_p1p1_to_p3(h, r);
 }
