@@ -57,8 +57,8 @@ POINT_BYTES = 128
 # register spills.
 PMC_N22 = {
     "fixed_base": {"read": 135.0e6, "write": 533.6e6, "valu_busy": 1.0},
-    "variable_base": {"read": 1750.9e6, "write": 1899.6e6, "valu_busy": 0.982},
-    "mul_u32": {"read": 1311.6e6, "write": 1434.0e6, "valu_busy": 0.929},
+    "variable_base": {"read": 823.8e6, "write": 900.2e6, "valu_busy": 0.992},
+    "mul_u32": {"read": 1119.6e6, "write": 1334.0e6, "valu_busy": 0.978},
 }
 ALG_BYTES = {"fixed_base": 32 + 128, "variable_base": 32 + 2 * 128, "mul_u32": 4 + 4 * 128}
 

@@ -66,6 +66,8 @@ __device__ __forceinline__ void mul_u32_point(PtView v, size_t i, uint32_t k) {
   ge_p3 p, h;
   if (W == 4) recode_u32(e, k);
   else recode_u32_r8(e, k);
+  // the stream words are finished here, not next to their first use
+  asm volatile("" : "+v"(e[13]), "+v"(e[14]), "+v"(e[15]));
   load_p3(p, v, i);
   if (W == 4) varbase_windows(h, p, e, 55);
   else varbase_windows8(h, p, e, 53);

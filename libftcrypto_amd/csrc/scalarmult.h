@@ -244,27 +244,39 @@ __device__ __forceinline__ void build_cached4(ge_cached tab[4], const ge_p3 &p) 
 // over the 11 windows of a 32-bit multiplier (which keeps the plain table).
 struct ge_cached3 { fe YpX, YmX, T2d; };
 
-__device__ __forceinline__ void scale_cached3(ge_cached3 &t, const ge_cached &c, const fe &l) {
-  fe_mul(t.YpX, c.YpX, l);
-  fe_mul(t.YmX, c.YmX, l);
-  fe_mul(t.T2d, c.T2d, l);
+// cached fields of the extended point m scaled by l: (Y+X) l, (Y-X) l, 2d T l
+__device__ __forceinline__ void scaled_cached3(ge_cached3 &t, const ge_p3 &m, const fe &l) {
+  fe u, w;
+  fe_add(u, m.Y, m.X);                 // A1
+  fe_mul(t.YpX, u, l);
+  fe_sub(u, m.Y, m.X);                 // A2
+  fe_mul(t.YmX, u, l);
+  fe_mul(w, m.T, l);
+  fe_mul(t.T2d, w, FE_D2);
 }
 
+// 1P..4P, each scaled to the shared Z = Z1 Z2 Z3 Z4; the multiples are
+// converted one at a time so that few full points are live together
 __device__ __forceinline__ void build_cached4_coz(ge_cached3 tab[4], fe &zc, const ge_p3 &p) {
-  ge_cached c[4];
-  build_cached4(c, p);
+  ge_p3 m1, m2, m3;
+  ge_p1p1 t;
+  ge_cached c1;
+  ge_p3_dbl(t, p);  ge_p1p1_to_p3(m1, t);            // 2P
+  ge_p3_to_cached(c1, m1);
+  ge_add(t, p, c1); ge_p1p1_to_p3(m2, t);            // 3P
+  ge_p3_dbl(t, m1); ge_p1p1_to_p3(m3, t);            // 4P
   fe a, b, l;
-  fe_mul(a, c[0].Z, c[1].Z);
-  fe_mul(b, c[2].Z, c[3].Z);
+  fe_mul(a, p.Z, m1.Z);
+  fe_mul(b, m2.Z, m3.Z);
   fe_mul(zc, a, b);
-  fe_mul(l, c[1].Z, b);
-  scale_cached3(tab[0], c[0], l);
-  fe_mul(l, c[0].Z, b);
-  scale_cached3(tab[1], c[1], l);
-  fe_mul(l, c[3].Z, a);
-  scale_cached3(tab[2], c[2], l);
-  fe_mul(l, c[2].Z, a);
-  scale_cached3(tab[3], c[3], l);
+  fe_mul(l, m1.Z, b);
+  scaled_cached3(tab[0], p, l);
+  fe_mul(l, p.Z, b);
+  scaled_cached3(tab[1], m1, l);
+  fe_mul(l, m3.Z, a);
+  scaled_cached3(tab[2], m2, l);
+  fe_mul(l, m2.Z, a);
+  scaled_cached3(tab[3], m3, l);
 }
 
 // |b| * P from the shared-Z table; b = 0 gives the identity (zc, zc, zc, 0)
@@ -404,6 +416,10 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
   uint32_t m[8], e[11];
   const uint32_t neg = scalar_effective(m, k);
   recode8(e, m);
+  // finish the digit stream here: left alone, the compiler keeps the 86
+  // digits apart (spilled) until the loop and packs them there
+  asm volatile("" : "+v"(e[0]), "+v"(e[1]), "+v"(e[2]), "+v"(e[3]), "+v"(e[4]), "+v"(e[5]), "+v"(e[6]),
+                    "+v"(e[7]), "+v"(e[8]), "+v"(e[9]), "+v"(e[10]));
   ge_p3 p = p_in, np;
   ge_p3_neg(np, p);
   fe_cmov(p.X, np.X, neg);
