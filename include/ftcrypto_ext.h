@@ -73,8 +73,10 @@ ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                          const void *d_m, const void *d_r, int msg_bits, void *stream);
 ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                              const void *d_r, void *stream);
-/* out must have n_seg elements; d_gather may be NULL (plain CSR); d_scratch
- * must hold ftx_segsum_scratch_points(n_elems) points (NULL: library-owned).
+/* out must have n_seg elements; d_gather may be NULL (plain CSR), otherwise
+ * n_elems int32 source indices, each below len(src) (device data: the caller
+ * guarantees this; ftx_gather_sum checks it on the host); d_scratch must hold
+ * ftx_segsum_scratch_points(n_elems) points (NULL: library-owned).
  * With d_gather, a scratch of ftx_gather_scratch_points(n_elems, len(src))
  * points also takes a point-contiguous copy of src, from which each gathered
  * point is one 128-byte line instead of four (faster random gathers). */

@@ -216,3 +216,33 @@ def test_config3_sparse_step_graph(n_nodes, n_edges, contiguous):
         + W.scalar_sum(rho)
     w1, w2 = enc_expected(H, tm, tr)
     assert reduce_enc(out1) == w1 and reduce_enc(out2) == w2
+
+
+@pytest.mark.parametrize("n_src", [4097, 4098, 4099])
+def test_gather_odd_sizes_unaligned_indices(n_src):
+    """Gather sums through the _dev entry with a source length that is not a
+    multiple of 4 and an index array starting 4 bytes past a 16-byte boundary
+    (the kernel then loads indices one at a time), from the planes and from
+    the point-contiguous copy; against the oracle."""
+    rng = np.random.default_rng(n_src)
+    n_rows = 700
+    deg = rng.poisson(6, size=n_rows)
+    deg[3] = 500
+    deg[4] = 0
+    off = np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+    ne = int(off[-1])
+    cols = rng.integers(0, n_src, size=ne, dtype=np.int32)
+    s = ft.random_scalars(n_src, 4700 + n_src, 0)
+    g, o = ft.GeArray.from_scalars(s), O.Array.from_scalars(s)
+    want = O.segmented_sum(off, o, idx=cols).to_bytes()
+    Lb = ft.lib()
+    d_off = DevBuf(off)
+    d_cols = DevBuf(np.concatenate([np.zeros(1, dtype=np.int32), cols]))     # cols start at byte 4
+    cols_ptr = ctypes.c_void_p(d_cols.p.value + 4)
+    st = ft.Stream()
+    for sc_points in (Lb.ftx_segsum_scratch_points(ne), Lb.ftx_gather_scratch_points(ne, n_src)):
+        scratch = ft.GeArray.alloc(sc_points)
+        out = ft.GeArray.alloc(n_rows)
+        ft.check(Lb.ftx_segmented_sum_dev(out.h, d_off.p, n_rows, ne, cols_ptr, g.h, scratch.h, st.h))
+        st.synchronize()
+        assert np.array_equal(out.to_bytes_folded(), want), sc_points
