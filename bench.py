@@ -322,11 +322,18 @@ def bench_cfg0(ft, hip, key, seed):
     L.ftx_stream_destroy(s2)
     hip.L.hipFree(m)
     hip.L.hipFree(r)
+    # elementwise ciphertext add at 2^22 (bandwidth-sized: 2 x 384 B per ct)
+    nb = 1 << 22
+    big = [ft.GeArray.from_small_scalars(np.arange(1, nb + 1, dtype=np.uint64)) for _ in range(4)]
+    big_ms = T(lambda: (ft.check(L.ftx_add_dev(big[0].h, big[2].h, stream)),
+                        ft.check(L.ftx_add_dev(big[1].h, big[3].h, stream))))
+    del big
     return {"compress": {"ms": round(cmp_ms, 4), "points": 6 * n, "points_per_s": 6 * n / (cmp_ms * 1e-3),
                          "note": "canonical 32-byte encodings, batch inversion; not part of step_ms"},
             "encrypt": {"ms": round(enc_ms, 4), "ciphertexts_per_s": 2 * n / (enc_ms * 1e-3),
                         "valu_frac": 2 * n * (2 * PRODUCTS["fixed_base"]) / (enc_ms * 1e-3) / PEAK_MUL32},
             "ct_add": adds_entry(add_ms, 2 * n, 3 * POINT_BYTES),
+            "ct_add_n2^22": adds_entry(big_ms, 2 * nb, 3 * POINT_BYTES),
             "reduce": adds_entry(red_ms, 2 * (n - 1), POINT_BYTES),
             "step_ms_one_stream": round(enc_ms + add_ms + red_ms, 4),
             "step_ms_two_streams": round(two_ms, 4),
