@@ -396,6 +396,10 @@ def bench_cfg2(ft, hip, key, seed, args):
     ms_s = T(lambda: job_scan(stream))
     res["scan"] = adds_entry(ms_s, 2 * (ns - 1), 2 * POINT_BYTES)
 
+    if args.cfg2_alone_only:
+        ft.check(L.ftx_synchronize(None))
+        hip.L.hipFree(d_off)
+        return res
     log("[bench] cfg2: all three at once on three streams")
     sts = []
     for _ in range(2):
@@ -582,6 +586,8 @@ def main():
                     help="first run one plain copy kernel over the N points (N x 128 B read and written)")
     ap.add_argument("--cfg2-reduce-log2", type=int, default=26)
     ap.add_argument("--cfg2-log2", type=int, default=24)
+    ap.add_argument("--cfg2-alone-only", action="store_true",
+                    help="skip the concurrent run of the three aggregation jobs (per-kernel counters stay separate)")
     ap.add_argument("--cfg3-nodes-log2", type=int, default=22)
     ap.add_argument("--cfg3-edges-log2", type=int, default=25)
     ap.add_argument("--cfg4-requests", type=int, default=1000)
