@@ -85,6 +85,18 @@ size_t ftx_gather_scratch_points(size_t n_elems, size_t n_src);
 ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg,
                                size_t n_elems, const void *d_gather,
                                const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream);
+/* Device-key form of ft_reduce_isum (ftcrypto.h:338-348, ftcrypto.cu:1150-1217):
+ * res[keys[i]] += summand[i] for the len(summand) uint64 keys at d_keys.  A key
+ * >= len(res) adds nothing (ft_reduce_isum rejects such keys on the host).
+ * Keys are sorted on the device (stable radix sort), so the summands of one
+ * key are added in increasing i and results are the same on every run.
+ * scratch: ftx_reduce_isum_scratch_points(n_keys, len(res)) points; d_work:
+ * ftx_reduce_isum_work_bytes(n_keys, len(res)) bytes of device memory, 8-byte
+ * aligned.  Either may be NULL: the call then allocates it and synchronises. */
+size_t ftx_reduce_isum_scratch_points(size_t n_keys, size_t n_res);
+size_t ftx_reduce_isum_work_bytes(size_t n_keys, size_t n_res);
+ft_error ftx_reduce_isum_dev(ft_ge25519_array res, const void *d_keys, const_ft_ge25519_array summand,
+                             ft_ge25519_array scratch, void *d_work, void *stream);
 /* out[0] = sum of src (out needs 1 element).  scratch: an array of at least
  * ftx_reduce_scratch_points(len(src)) points, or NULL (then the call allocates
  * and synchronises). */
@@ -110,8 +122,8 @@ void ftx_stream_destroy(void *stream);
 /* Capture the _dev calls issued on `stream` (not the library stream) between
  * begin and end into a graph.  Capturable: ftx_from_scalars_dev, ftx_mul_dev,
  * ftx_mul_u32_dev, ftx_add_dev, ftx_encrypt_dev, ftx_rerandomize_dev,
- * ftx_encode_dev, and ftx_segmented_sum_dev / ftx_reduce_dev / ftx_scan_dev with
- * caller-provided scratch. */
+ * ftx_encode_dev, and ftx_segmented_sum_dev / ftx_reduce_dev / ftx_scan_dev /
+ * ftx_reduce_isum_dev with caller-provided scratch (and work memory). */
 ft_error ftx_capture_begin(void *stream);
 ft_error ftx_capture_end(void *stream, ftx_graph *graph);
 ft_error ftx_graph_launch(ftx_graph graph, void *stream);

@@ -97,6 +97,9 @@ _SIGS = {
     "ftx_rerandomize_dev": (i32, [vp, vp, vp, vp, vp]),
     "ftx_segsum_scratch_points": (sz, [sz]),
     "ftx_segmented_sum_dev": (i32, [vp, vp, sz, sz, vp, vp, vp, vp]),
+    "ftx_reduce_isum_scratch_points": (sz, [sz, sz]),
+    "ftx_reduce_isum_work_bytes": (sz, [sz, sz]),
+    "ftx_reduce_isum_dev": (i32, [vp, vp, vp, vp, vp, vp]),
     "ftx_reduce_scratch_points": (sz, [sz]),
     "ftx_reduce_dev": (i32, [vp, vp, vp, vp]),
     "ftx_scan_scratch_points": (sz, [sz]),

@@ -688,31 +688,57 @@ ft_error ft_sub(ft_ge25519_array res, const_ft_ge25519_array summand) {
 
 void ft_neg(ft_ge25519_array array) { (void)launch_neg(view(array), array->n, G.stream); }
 
+// res[keys[i]] += summand[i] with device keys, stream-ordered.  Keys sort on
+// the device into m + 1 CSR buckets (bucket m takes every key >= m); the
+// gather form of the segmented sum adds each bucket's elements in increasing
+// i, bucket m's sum is dropped, and the others are added into res.
+static size_t isum_work_bytes(size_t n, size_t m) { return 8 * (m + 2) + csr_sort_bytes(n); }
+static size_t isum_scratch_points(size_t n, size_t m) { return segsum_gather_scratch_points(n, n) + m + 1; }
+
+static ft_error isum_run(ft_ge25519_array res, const uint64_t *d_keys, const_ft_ge25519_array summand,
+                         PtView scratch, void *work, hipStream_t s) {
+  const size_t n = summand->n, m = res->n;
+  int64_t *off = static_cast<int64_t *>(work);
+  const int32_t *order = nullptr;
+  CK(csr_sort_keys(off, &order, d_keys, n, m, off + (m + 2), s));
+  const size_t tail = segsum_gather_scratch_points(n, n);
+  const PtView sums{scratch.w + tail * 8, scratch.cap};      // element slots [tail, tail + m + 1)
+  CK(segsum_launch(sums, off, m + 1, n, order, view(summand), n, scratch, s));
+  CK(launch_add(view(res), sums, m, false, s));
+  return FT_ERR_NO_ERROR;
+}
+
+static ft_error isum_checked(ft_ge25519_array res, const void *d_keys, const_ft_ge25519_array summand,
+                             ft_ge25519_array scratch, void *d_work, hipStream_t s) {
+  const size_t n = summand->n, m = res->n;
+  if (n == 0 || m == 0) return FT_ERR_NO_ERROR;
+  if (n >= ((size_t)1 << 31) || m >= ((size_t)1 << 31) - 1) return FT_ERR_BAD_SIZE;
+  if (scratch && scratch->cap < isum_scratch_points(n, m)) return FT_ERR_INSUFFICIENT_SPACE;
+  if (d_work && (reinterpret_cast<uintptr_t>(d_work) & 7)) return FT_ERR_BAD_SIZE;
+  ArrGuard own;
+  DevBuf work;
+  if (!scratch) {
+    CKE(alloc_array(&own.a, isum_scratch_points(n, m)));
+    scratch = own.a;
+  }
+  if (!d_work) {
+    CKE(work.alloc(isum_work_bytes(n, m)));
+    d_work = work.p;
+  }
+  CKE(isum_run(res, static_cast<const uint64_t *>(d_keys), summand, view(scratch), d_work, s));
+  if (own.a || work.p) CK(hipStreamSynchronize(s));
+  return FT_ERR_NO_ERROR;
+}
+
 ft_error ft_reduce_isum(ft_ge25519_array res, const size_t *idxs, size_t n_idxs,
                         const_ft_ge25519_array summand) {
   // ftcrypto.cu:1150-1217: res[idxs[i]] += summand[i], duplicate keys accumulate
   if (n_idxs != summand->n) return FT_ERR_SIZE_MISMATCH;
   CKE(check_idxs(idxs, n_idxs, res->n));
   if (n_idxs == 0) return FT_ERR_NO_ERROR;
-  // counting sort of the keys into CSR over [0, len(res))
-  const size_t m = res->n;
-  std::vector<int64_t> off(m + 1, 0);
-  for (size_t i = 0; i < n_idxs; ++i) off[idxs[i] + 1]++;
-  for (size_t k = 0; k < m; ++k) off[k + 1] += off[k];
-  std::vector<int32_t> order(n_idxs);
-  {
-    std::vector<int64_t> pos(off.begin(), off.end() - 1);
-    for (size_t i = 0; i < n_idxs; ++i) order[pos[idxs[i]]++] = (int32_t)i;
-  }
-  DevBuf doff, dord;
-  CKE(doff.upload(off.data(), off.size() * 8));
-  CKE(dord.upload(order.data(), order.size() * 4));
-  ArrGuard sums, scratch;
-  CKE(alloc_array(&sums.a, m));
-  CKE(alloc_array(&scratch.a, segsum_gather_scratch_points(n_idxs, summand->n)));
-  CKE(segsum_run(sums.a, (const int64_t *)doff.p, m, n_idxs, (const int32_t *)dord.p, summand,
-                 view(scratch.a), G.stream));
-  CK(launch_add(view(res), view(sums.a), m, false, G.stream));
+  DevBuf dkeys;
+  CKE(dkeys.upload(idxs, n_idxs * sizeof(size_t)));
+  CKE(isum_checked(res, dkeys.p, summand, nullptr, nullptr, G.stream));
   CK(hipStreamSynchronize(G.stream));
   return FT_ERR_NO_ERROR;
 }
@@ -927,6 +953,13 @@ void ftx_graph_free(ftx_graph graph) {
 }
 
 size_t ftx_segsum_scratch_points(size_t n_elems) { return seg_scratch_points(n_elems); }
+size_t ftx_reduce_isum_scratch_points(size_t n_keys, size_t n_res) { return isum_scratch_points(n_keys, n_res); }
+size_t ftx_reduce_isum_work_bytes(size_t n_keys, size_t n_res) { return isum_work_bytes(n_keys, n_res); }
+
+ft_error ftx_reduce_isum_dev(ft_ge25519_array res, const void *d_keys, const_ft_ge25519_array summand,
+                             ft_ge25519_array scratch, void *d_work, void *stream) {
+  return isum_checked(res, d_keys, summand, scratch, d_work, S(stream));
+}
 size_t ftx_gather_scratch_points(size_t n_elems, size_t n_src) {
   return segsum_gather_scratch_points(n_elems, n_src);
 }

@@ -62,6 +62,12 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
                             const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int chu,
                             hipStream_t s);
 
+// sort.hip: stable device sort of n keys into the CSR order of m + 1
+// buckets (keys 0..m-1, then every key >= m)
+size_t csr_sort_bytes(size_t n);
+hipError_t csr_sort_keys(int64_t *off, const int32_t **order, const uint64_t *keys, size_t n, size_t m,
+                         void *scratch, hipStream_t s);
+
 // api.hip: device tables owned by the library
 const uint32_t *base_table();
 const uint32_t *key_table(const struct ftx_key_s *key);
