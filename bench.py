@@ -228,6 +228,8 @@ def run(args):
         configs["cfg2_aggregation"] = bench_cfg2(ft, hip, key, seed, args)
     if "cfg3" in sections:
         configs["cfg3_sparse_step_graph"] = bench_cfg3(ft, hip, key, seed, args)
+    if "isum" in sections:
+        configs["reduce_isum"] = bench_isum(ft, hip, seed, args)
     if "cfg4" in sections:
         configs["cfg4_serving"] = bench_cfg4(ft, key, seed, args)
     detail["configs"] = configs
@@ -430,6 +432,35 @@ def bench_cfg2(ft, hip, key, seed, args):
     return res
 
 
+def bench_isum(ft, hip, seed, args):
+    """ft_reduce_isum, res[key[i]] += src[i], for 2^22 summands over 2^19
+    result elements with uniform random keys: the device-key form (keys
+    sorted on the device, caller scratch) and the reference-signature form
+    (host keys, checked and copied per call).  Each summand is one point add."""
+    L = ft.lib()
+    stream = L.ftx_default_stream()
+    T = Timer(hip, stream)
+    rng = np.random.default_rng(seed + 5)
+    n, m = 1 << args.isum_log2, 1 << (args.isum_log2 - 3)
+    keys = rng.integers(0, m, size=n, dtype=np.uint64)
+    src = ft.GeArray.from_small_scalars(rng.integers(0, 1 << 40, size=n, dtype=np.uint64))
+    res = ft.GeArray.from_small_scalars(np.zeros(m, dtype=np.uint64))
+    d_keys = hip.upload(keys)
+    scr = ft.GeArray.alloc(L.ftx_reduce_isum_scratch_points(n, m))
+    work = hip.malloc(L.ftx_reduce_isum_work_bytes(n, m))
+    log("[bench] isum: 2^%d summands over 2^%d keys" % (args.isum_log2, args.isum_log2 - 3))
+    ms_dev = T(lambda: ft.check(L.ftx_reduce_isum_dev(res.h, d_keys, src.h, scr.h, work, stream)), reps=5)
+    out = {"n": n, "m": m, "device_keys": adds_entry(ms_dev, n)}
+    t0 = time.perf_counter()
+    for _ in range(3):
+        res.reduce_isum(keys, src)
+    out["host_keys_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 4)
+    ft.check(L.ftx_synchronize(None))
+    hip.L.hipFree(d_keys)
+    hip.L.hipFree(work)
+    return out
+
+
 def bench_cfg3(ft, hip, key, seed, args):
     """Gather-sum of in-neighbours + rerandomisation, one hipGraph per step."""
     from libftcrypto_amd import workloads as W
@@ -580,7 +611,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default aggregation schedules")
     ap.add_argument("--headline-only", action="store_true", help="skip the configurations other than the N=2^22 scalar mults")
-    ap.add_argument("--sections", default="cfg0,cfg2,cfg3,cfg4",
+    ap.add_argument("--sections", default="cfg0,cfg2,cfg3,isum,cfg4",
                     help="configurations measured after the headline step (comma list of cfg0,cfg2,cfg3,cfg4)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first run one plain copy kernel over the N points (N x 128 B read and written)")
@@ -588,6 +619,7 @@ def main():
     ap.add_argument("--cfg2-log2", type=int, default=24)
     ap.add_argument("--cfg2-alone-only", action="store_true",
                     help="skip the concurrent run of the three aggregation jobs (per-kernel counters stay separate)")
+    ap.add_argument("--isum-log2", type=int, default=22)
     ap.add_argument("--cfg3-nodes-log2", type=int, default=22)
     ap.add_argument("--cfg3-edges-log2", type=int, default=25)
     ap.add_argument("--cfg4-requests", type=int, default=1000)

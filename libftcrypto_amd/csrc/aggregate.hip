@@ -198,9 +198,10 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
 // Level 1: lane t walks elements [2^CH t, 2^CH (t + 1)) of the input (through
 // the gather indices when given), segment boundaries from the offsets.
 // MODE 0: elements in order; 1: gathered from the planes of src; 2:
-// gathered from the point-contiguous copy `aos`
+// gathered from the point-contiguous copy `aos`.  Two waves per SIMD (the
+// gather forms would otherwise take a few registers over 256).
 template <int CH, int MODE>
-__global__ void __launch_bounds__(ATPB) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
+__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
                                                      size_t n, const int64_t *off, size_t nseg,
                                                      const int32_t *gather, PtView aos, size_t aos_base) {
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;

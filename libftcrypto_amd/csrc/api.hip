@@ -16,7 +16,6 @@
 #include <mutex>
 #include <numeric>
 #include <string>
-#include <unordered_set>
 #include <vector>
 
 #include "../../include/ftcrypto.h"
@@ -787,15 +786,20 @@ ft_error ft_not_equal(int64_t *res, const_ft_ge25519_array a, const_ft_ge25519_a
 }
 
 ft_error ft_array_contains(int64_t *res, const_ft_ge25519_array a1, const_ft_ge25519_array a2) {
-  // canonical encodings are unique per point: hash the set of a2's
-  std::vector<uint8_t> e1(a1->n * 32), e2(a2->n * 32);
-  CKE(ft_array_to_bytes_folded(e1.data(), e1.size(), a1));
-  CKE(ft_array_to_bytes_folded(e2.data(), e2.size(), a2));
-  std::unordered_set<std::string> set;
-  set.reserve(a2->n * 2);
-  for (size_t j = 0; j < a2->n; ++j) set.emplace((const char *)&e2[32 * j], 32);
-  for (size_t i = 0; i < a1->n; ++i) res[i] = set.count(std::string((const char *)&e1[32 * i], 32)) ? 1 : 0;
-  return FT_ERR_NO_ERROR;
+  // ftcrypto.cu:1046-1086: res[i] = 1 when a1[i] equals some element of a2;
+  // canonical encodings of both, then a device hash set of a2's
+  const size_t n1 = a1->n, n2 = a2->n;
+  if (n1 == 0) return FT_ERR_NO_ERROR;
+  DevBuf e1, e2, slots, out;
+  CKE(e1.alloc(n1 * 32));
+  CKE(e2.alloc(n2 * 32));
+  CKE(slots.alloc(set_slots(n2) * 4));
+  CKE(out.alloc(n1 * 8));
+  CK(launch_encode((uint32_t *)e1.p, view(a1), n1, G.stream));
+  CK(launch_encode((uint32_t *)e2.p, view(a2), n2, G.stream));
+  CK(launch_set_contains((int64_t *)out.p, (const uint32_t *)e1.p, n1, (const uint32_t *)e2.p, n2,
+                         (uint32_t *)slots.p, G.stream));
+  return download_i64(res, out.p, n1);
 }
 
 // ------------------------------------------------------------ extensions

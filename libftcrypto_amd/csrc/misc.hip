@@ -160,6 +160,82 @@ __global__ void __launch_bounds__(MTPB) k_from_raw(PtView v, const uint32_t *in,
   }
 }
 
+// ------------------------------------------------------- set membership
+//
+// Canonical encodings are unique per point, so "array1[i] is in array2" is
+// membership of 32-byte encodings.  The second array's encodings go into an
+// open-addressing table of mask + 1 uint32 slots (a power of two, at least
+// twice their number) holding index + 1, 0 = empty, with linear probing.  An
+// insert that meets an equal encoding stops, so duplicates share one slot and
+// probe chains stay short whatever the input.  The table always keeps empty
+// slots, so every probe sequence ends within mask + 1 steps.  The reference
+// compares every pair (ftcrypto.cu:1046-1086); this is O(n1 + n2).
+
+__device__ __forceinline__ void ld_enc(uint4 &a, uint4 &b, const uint4 *enc, size_t i) {
+  a = enc[2 * i];
+  b = enc[2 * i + 1];
+}
+
+__device__ __forceinline__ bool enc_eq(const uint4 &a, const uint4 &b, const uint4 &c, const uint4 &d) {
+  return ((a.x ^ c.x) | (a.y ^ c.y) | (a.z ^ c.z) | (a.w ^ c.w) | (b.x ^ d.x) | (b.y ^ d.y) | (b.z ^ d.z) |
+          (b.w ^ d.w)) == 0;
+}
+
+__device__ __forceinline__ size_t enc_hash(const uint4 &a, const uint4 &b) {
+  const uint64_t w[4] = {(uint64_t)a.y << 32 | a.x, (uint64_t)a.w << 32 | a.z, (uint64_t)b.y << 32 | b.x,
+                         (uint64_t)b.w << 32 | b.z};
+  uint64_t h = 0x243F6A8885A308D3ull;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    h = (h ^ w[q]) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 31;
+  }
+  return (size_t)h;
+}
+
+__global__ void __launch_bounds__(MTPB) k_fill_u32(uint32_t *p, size_t n) {
+  const size_t i = blockIdx.x * (size_t)MTPB + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void __launch_bounds__(MTPB) k_set_insert(uint32_t *slots, size_t mask, const uint4 *enc, size_t n) {
+  const size_t j = blockIdx.x * (size_t)MTPB + threadIdx.x;
+  if (j >= n) return;
+  uint4 a, b;
+  ld_enc(a, b, enc, j);
+  size_t h = enc_hash(a, b) & mask;
+  for (size_t probe = 0; probe <= mask; ++probe) {
+    const uint32_t prev = atomicCAS(&slots[h], 0u, (uint32_t)(j + 1));
+    if (prev == 0) return;                       // inserted
+    uint4 c, d;
+    ld_enc(c, d, enc, prev - 1);
+    if (enc_eq(a, b, c, d)) return;              // already present
+    h = (h + 1) & mask;
+  }
+}
+
+__global__ void __launch_bounds__(MTPB) k_set_lookup(int64_t *res, const uint32_t *slots, size_t mask,
+                                                     const uint4 *set_enc, const uint4 *enc, size_t n) {
+  const size_t i = blockIdx.x * (size_t)MTPB + threadIdx.x;
+  if (i >= n) return;
+  uint4 a, b;
+  ld_enc(a, b, enc, i);
+  size_t h = enc_hash(a, b) & mask;
+  int64_t found = 0;
+  for (size_t probe = 0; probe <= mask; ++probe) {
+    const uint32_t cur = slots[h];
+    if (cur == 0) break;
+    uint4 c, d;
+    ld_enc(c, d, set_enc, cur - 1);
+    if (enc_eq(a, b, c, d)) {
+      found = 1;
+      break;
+    }
+    h = (h + 1) & mask;
+  }
+  res[i] = found;
+}
+
 // ---------------------------------------------------------------- launchers
 
 #define MCHK() return hipGetLastError()
@@ -194,6 +270,24 @@ hipError_t launch_equal(int64_t *res, PtView a, PtView b, size_t n, bool neq, hi
 hipError_t launch_validate(unsigned long long *bad, PtView v, size_t n, const uint32_t *lwords, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(k_validate, dim3(mgrid(n)), dim3(MTPB), 0, s, bad, v, n, lwords);
+  MCHK();
+}
+size_t set_slots(size_t n_set) {
+  size_t t = 64;
+  while (t < 2 * n_set) t <<= 1;
+  return t;
+}
+hipError_t launch_set_contains(int64_t *res, const uint32_t *enc, size_t n, const uint32_t *set_enc, size_t n_set,
+                               uint32_t *slots, hipStream_t s) {
+  if (!n) return hipSuccess;
+  const size_t t = set_slots(n_set);
+  if (n_set >= 0xffffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fill_u32, dim3(mgrid(t)), dim3(MTPB), 0, s, slots, t);
+  if (n_set)
+    hipLaunchKernelGGL(k_set_insert, dim3(mgrid(n_set)), dim3(MTPB), 0, s, slots, t - 1,
+                       reinterpret_cast<const uint4 *>(set_enc), n_set);
+  hipLaunchKernelGGL(k_set_lookup, dim3(mgrid(n)), dim3(MTPB), 0, s, res, slots, t - 1,
+                     reinterpret_cast<const uint4 *>(set_enc), reinterpret_cast<const uint4 *>(enc), n);
   MCHK();
 }
 hipError_t launch_to_affine(uint32_t *out, PtView v, size_t n, hipStream_t s) {

@@ -356,6 +356,43 @@ def test_predicates():
     assert bytes(t.to_bytes_folded()) == b"".join(O.scalarmult_base(bytes(sb(v))) for v in [1, 8, 3])
 
 
+def test_contains_device_set():
+    """ft_array_contains through the device hash set, against membership of
+    the scalars (k B are distinct for distinct k < l): heavy duplicates in the
+    set, the identity, negated points (encodings differing in the sign bit
+    only), points equal in value but built by another path, empty sets."""
+    rng = np.random.default_rng(21)
+    set_k = rng.integers(0, 3000, size=50000).astype(np.uint64)
+    set_k[:7] = 0
+    probe_k = np.arange(6000, dtype=np.uint64)
+    a = ft.GeArray.from_small_scalars(probe_k - 1)           # (k - 1) B + B, k >= 1
+    a.add(ft.GeArray.from_small_scalars(np.ones(6000, dtype=np.uint64)))
+    a.assign_to_subset([0], ft.GeArray.from_small_scalars([0]))
+    want = np.isin(probe_k, set_k).astype(np.int64)
+    got = a.contains(ft.GeArray.from_small_scalars(set_k))
+    assert np.array_equal(got, want)
+    n = ft.GeArray.from_small_scalars(probe_k[1:])
+    n.neg()
+    assert not n.contains(ft.GeArray.from_small_scalars(set_k)).any()
+    dup = ft.GeArray.from_small_scalars(np.full(5000, 7, dtype=np.uint64))
+    assert list(np.nonzero(ft.GeArray.from_small_scalars(probe_k[:20]).contains(dup))[0]) == [7]
+    empty = ft.GeArray.from_small_scalars(np.zeros(0, dtype=np.uint64))
+    assert not ft.GeArray.from_small_scalars(probe_k[:100]).contains(empty).any()
+    assert len(empty.contains(dup)) == 0
+
+
+def test_contains_large():
+    n = 1 << 20
+    rng = np.random.default_rng(22)
+    vals = rng.integers(0, 2**63, size=2 * n, dtype=np.uint64)
+    vals = np.unique(vals)[: 2 * n - 64]
+    set_k, other = vals[:n], vals[n:]
+    probe = np.concatenate([set_k[::2], other[: n // 2]])
+    rng.shuffle(probe)
+    got = ft.GeArray.from_small_scalars(probe).contains(ft.GeArray.from_small_scalars(set_k))
+    assert np.array_equal(got, np.isin(probe, set_k).astype(np.int64))
+
+
 # ------------------------------------------------------------ the small reference workload
 
 def test_config0_workload():
