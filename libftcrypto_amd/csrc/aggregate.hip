@@ -182,17 +182,18 @@ __device__ __forceinline__ size_t seg_find1(const int64_t *off, size_t a, size_t
   return a;
 }
 
+// Segment s's sum over this chunk goes to out[s] when the segment is complete
+// in the chunk, else to the chunk's head record (it began before the chunk)
+// or tail record (it continues after it).  One pack and one store sequence
+// with a per-lane destination: lanes of a wave that flush to different
+// places do not run three copies of the store.
 __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys, size_t s, int64_t lo,
                                           int64_t hi, int64_t clo, int64_t chi, size_t t, const ge_p3 &acc) {
-  if (lo >= clo && hi <= chi) {
-    store_p3(out, s, acc);                      // complete in this chunk
-  } else if (lo < clo) {
-    store_p3(rec, 2 * t, acc);                  // began before the chunk
-    keys[2 * t] = (int64_t)s;
-  } else {
-    store_p3(rec, 2 * t + 1, acc);              // continues after it
-    keys[2 * t + 1] = (int64_t)s;
-  }
+  const bool inside = lo >= clo && hi <= chi;
+  const size_t slot = 2 * t + (lo < clo ? 0 : 1);
+  const PtView v{inside ? out.w : rec.w, inside ? out.cap : rec.cap};
+  store_p3(v, inside ? s : slot, acc);
+  if (!inside) keys[slot] = (int64_t)s;
 }
 
 // Level 1: lane t walks elements [2^CH t, 2^CH (t + 1)) of the input (through
