@@ -120,7 +120,7 @@ def run(args):
         table = ft.autotune(3)
         schedules = {"picked_by": "ftx_autotune on this GPU", "seconds": round(time.time() - t0, 2),
                      "classes": {"n<2^14": table[0], "n<2^18": table[1], "n<2^22": table[2], "n>=2^22": table[3]},
-                     "fields": "(seg1, segu, scan: log2 elements per lane; reduce: max blocks; window width of ct x u32 and of variable-base)"}
+                     "fields": "(seg1, scan: log2 elements per lane; segu: log2 radix of the segmented-sum run tree; reduce: max blocks; window width of ct x u32 and of variable-base)"}
         log("[bench] autotuned aggregation schedules in %.1fs: %s" % (schedules["seconds"], table))
 
     log("[bench] generating %d x 4 inputs" % n)

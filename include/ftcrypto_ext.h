@@ -167,9 +167,11 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
 
 /* Launch schedules of the reduce, scan and segmented sum and the window widths
  * of the variable-base mults, one per size class of the input length (class
- * 0: n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).  seg1 / segu / scan are log2
- * elements per lane (4..6) at segmented-sum level 1, its upper levels and the
- * scan; reduce_blocks (1..4096) bounds the first reduce pass; u32_window
+ * 0: n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).  seg1 and scan are log2
+ * elements per lane (4..6) at segmented-sum level 1 and in the scan; segu
+ * (1..4) is log2 of the radix of the tree that adds the partial sums of
+ * segments spanning several level-1 chunks; reduce_blocks (1..4096) bounds
+ * the first reduce pass; u32_window
  * (ciphertext x u32) and varbase_window (ft_mul, ft_scale) are 4 (radix-16
  * digits, 8-entry table) or 3 (radix 8, 4 entries).  Results are the same
  * under every schedule.

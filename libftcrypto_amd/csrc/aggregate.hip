@@ -13,12 +13,14 @@
 // Segmented sum: level 1 walks the elements; a segment that lies inside one
 // chunk is written to the output directly, one that crosses a chunk edge
 // leaves a partial sum and its segment id in the chunk's head slot (segment
-// began before the chunk) or tail slot (segment continues after it).  The
-// slots form the sorted, keyed element array of level 2, where segment s
-// covers slots [2 c_first + 1, 2 c_last + 1) -- a range computed from the
-// offsets alone, which decides whether a run of s is complete -- and level 2
-// runs the same chunked walk over the records, until one chunk holds all.
-// Level 1 and the upper levels may use different chunk sizes.
+// began before the chunk) or tail slot (segment continues after it).  A
+// segment spanning chunks c0..c1 thus has one partial per chunk: c0's tail
+// slot, then the head slots of c0+1..c1.  A radix-R tree over each such run
+// (R = 2, 4, 8 or 16) adds them up in ceil(log_R(chunks)) passes of at most
+// R - 1 adds per lane, and the run's first slot goes to the output in the
+// pass that completes it.  At these sizes the GPU is latency-bound: a wave
+// waits for its lane with the most adds, so radix 2 (one add per lane and
+// pass) suits long, skewed runs, larger radices fewer passes over short ones.
 #include "kernels.h"
 
 namespace ft {
@@ -152,25 +154,6 @@ hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipSt
 
 // --------------------------------------------------------- segmented sum
 
-// range of segment s in the element space of `level` (1 = the input); the
-// step from level l to l + 1 uses chunks of 2^ch1 (l = 1) or 2^chu
-__device__ __forceinline__ void seg_range(const int64_t *off, size_t s, int level, int ch1, int chu,
-                                          int64_t &lo, int64_t &hi) {
-  lo = off[s];
-  hi = off[s + 1];
-#pragma unroll 1
-  for (int l = 1; l < level; ++l) {
-    const int ch = (l == 1) ? ch1 : chu;
-    const int64_t c0 = lo >> ch;
-    if (hi > lo && c0 != ((hi - 1) >> ch)) {
-      lo = 2 * c0 + 1;
-      hi = 2 * ((hi - 1) >> ch) + 1;
-    } else {
-      lo = hi = 2 * c0 + 1;
-    }
-  }
-}
-
 // first segment in [a, nseg) whose level-1 range ends after position p
 __device__ __forceinline__ size_t seg_find1(const int64_t *off, size_t a, size_t nseg, int64_t p) {
   size_t b = nseg;
@@ -210,9 +193,7 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   if (clo >= (int64_t)n) return;
   const int64_t chi = clo + (1 << CH);
   const int64_t cend = chi < (int64_t)n ? chi : (int64_t)n;
-  if (rec.w) {
-    store_identity(rec, 2 * t);
-    store_identity(rec, 2 * t + 1);
+  if (rec.w) {                                   // slots left unused keep key -1
     keys[2 * t] = -1;
     keys[2 * t + 1] = -1;
   }
@@ -299,49 +280,50 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
 }
 
-// Level L >= 2: lane t walks records [2^CH t, 2^CH (t + 1)) of level L-1,
-// whose segment keys are sorted (-1 marks an unused slot holding the
-// identity); a run is complete here when the segment's level-L range lies
-// in the chunk.
-template <int CH>
-__global__ void __launch_bounds__(ATPB) k_seg_levelN(PtView out, PtView rec, int64_t *keys, PtView in,
-                                                     const int64_t *in_keys, size_t n, const int64_t *off,
-                                                     int level, int ch1) {
-  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
-  const int64_t clo = (int64_t)(t << CH);
-  if (clo >= (int64_t)n) return;
-  const int64_t chi = clo + (1 << CH);
-  const int64_t cend = chi < (int64_t)n ? chi : (int64_t)n;
-  if (rec.w) {
-    store_identity(rec, 2 * t);
-    store_identity(rec, 2 * t + 1);
-    keys[2 * t] = -1;
-    keys[2 * t + 1] = -1;
-  }
-  int64_t cur = -1;
-  ge_p3 acc, x;
-  ge_p3_0(acc);
+// One pass of the run tree with radix 2^LR and stride d: lane c (a level-1
+// chunk) adds partials c + d, ..., c + (2^LR - 1) d of a run into its own
+// when its offset in the run is a multiple of 2^LR d, for the run its head
+// slot belongs to (offset >= 1) and for the run its tail slot starts (offset
+// 0).  Runs that this pass completes write their sum to the output.  Reads
+// and writes of one pass touch disjoint slots.
+template <int LR>
+__device__ __forceinline__ void tree_step(PtView rec, size_t slot, size_t c, size_t c1, size_t d, ge_p3 &acc) {
+  load_p3(acc, rec, slot);
 #pragma unroll 1
-  for (int64_t i = clo; i < cend; ++i) {
-    const int64_t k = in_keys[i];
-    if (k < 0) continue;
-    if (k != cur) {
-      if (cur >= 0) {
-        int64_t lo, hi;
-        seg_range(off, (size_t)cur, level, ch1, CH, lo, hi);
-        seg_flush(out, rec, keys, (size_t)cur, lo, hi, clo, chi, t, acc);
-      }
-      cur = k;
-      load_p3(acc, in, (size_t)i);
-    } else {
-      load_p3(x, in, (size_t)i);
-      ge_p3_add(acc, acc, x);
+  for (int j = 1; j < (1 << LR); ++j) {
+    const size_t x = c + (size_t)j * d;
+    if (x > c1) break;
+    ge_p3 y;
+    load_p3(y, rec, 2 * x);
+    ge_p3_add(acc, acc, y);
+  }
+}
+
+template <int LR>
+__global__ void __launch_bounds__(ATPB) k_seg_tree(PtView out, PtView rec, const int64_t *keys, size_t T,
+                                                   const int64_t *off, int ch1, size_t d) {
+  const size_t c = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  if (c >= T) return;
+  const size_t span = d << LR;
+  const int64_t h = keys[2 * c];
+  if (h >= 0) {
+    const size_t c0 = (size_t)(off[h] >> ch1), c1 = (size_t)((off[h + 1] - 1) >> ch1);
+    if ((c - c0) % span == 0 && c + d <= c1) {
+      ge_p3 acc;
+      tree_step<LR>(rec, 2 * c, c, c1, d, acc);
+      store_p3(rec, 2 * c, acc);
     }
   }
-  if (cur >= 0) {
-    int64_t lo, hi;
-    seg_range(off, (size_t)cur, level, ch1, CH, lo, hi);
-    seg_flush(out, rec, keys, (size_t)cur, lo, hi, clo, chi, t, acc);
+  const int64_t t = keys[2 * c + 1];
+  if (t >= 0) {
+    const size_t c1 = (size_t)((off[t + 1] - 1) >> ch1);
+    const size_t len = c1 - c + 1;                 // >= 2 chunks
+    if (d < len) {                                 // not completed by an earlier pass
+      ge_p3 acc;
+      tree_step<LR>(rec, 2 * c + 1, c, c1, d, acc);
+      if (span >= len) store_p3(out, (size_t)t, acc);
+      else store_p3(rec, 2 * c + 1, acc);
+    }
   }
 }
 
@@ -363,53 +345,54 @@ __global__ void __launch_bounds__(ATPB) k_seg_empty(PtView out, const int64_t *o
   if (s < nseg && off[s] == off[s + 1]) store_identity(out, s);
 }
 
-static hipError_t launch_seg_level(PtView out, PtView rec, int64_t *keys, PtView elems, const int64_t *in_keys,
-                                   size_t n, const int64_t *off, size_t nseg, const int32_t *gather, int level,
-                                   int ch, int ch1, PtView aos, size_t aos_base, hipStream_t s) {
+static hipError_t launch_seg_level1(PtView out, PtView rec, int64_t *keys, PtView elems, size_t n,
+                                    const int64_t *off, size_t nseg, const int32_t *gather, int ch, PtView aos,
+                                    size_t aos_base, hipStream_t s) {
   const size_t T = tiles(n, ch);
   if (!T || !nseg) return hipSuccess;
   const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
-  if (level == 1) {
-    const int mode = !gather ? 0 : (aos.w ? 2 : 1);
+  const int mode = !gather ? 0 : (aos.w ? 2 : 1);
 #define FT_SEG1(C, M)                                                                                    \
   hipLaunchKernelGGL((k_seg_level1<C, M>), grid, dim3(ATPB), 0, s, out, rec, keys, elems, n, off, nseg, gather, \
                      aos, aos_base)
-    switch (ch * 4 + mode) {
-      case 16: FT_SEG1(4, 0); break;
-      case 17: FT_SEG1(4, 1); break;
-      case 18: FT_SEG1(4, 2); break;
-      case 20: FT_SEG1(5, 0); break;
-      case 21: FT_SEG1(5, 1); break;
-      case 22: FT_SEG1(5, 2); break;
-      case 24: FT_SEG1(6, 0); break;
-      case 25: FT_SEG1(6, 1); break;
-      case 26: FT_SEG1(6, 2); break;
-      default: return hipErrorInvalidValue;
-    }
-#undef FT_SEG1
-  } else {
-    switch (ch) {
-      case 4: hipLaunchKernelGGL(k_seg_levelN<4>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
-      case 5: hipLaunchKernelGGL(k_seg_levelN<5>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
-      case 6: hipLaunchKernelGGL(k_seg_levelN<6>, grid, dim3(ATPB), 0, s, out, rec, keys, elems, in_keys, n, off, level, ch1); break;
-      default: return hipErrorInvalidValue;
-    }
+  switch (ch * 4 + mode) {
+    case 16: FT_SEG1(4, 0); break;
+    case 17: FT_SEG1(4, 1); break;
+    case 18: FT_SEG1(4, 2); break;
+    case 20: FT_SEG1(5, 0); break;
+    case 21: FT_SEG1(5, 1); break;
+    case 22: FT_SEG1(5, 2); break;
+    case 24: FT_SEG1(6, 0); break;
+    case 25: FT_SEG1(6, 1); break;
+    case 26: FT_SEG1(6, 2); break;
+    default: return hipErrorInvalidValue;
   }
+#undef FT_SEG1
   return hipGetLastError();
 }
 
-// records of level 1 (buffer A, reused by levels 3, 5, ...) and level 2
-// (buffer B, reused by levels 4, 6, ...): two per chunk
-static size_t segsum_records(size_t n, int ch1, int chu) {
-  const size_t t1 = tiles(n, ch1), t2 = tiles(2 * t1, chu);
-  return 2 * t1 + 2 * t2;
+static hipError_t launch_seg_tree(PtView out, PtView rec, const int64_t *keys, size_t T, const int64_t *off,
+                                  int ch1, int lr, hipStream_t s) {
+  const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
+  for (size_t d = 1; d < T; d <<= lr) {
+    switch (lr) {
+      case 1: hipLaunchKernelGGL(k_seg_tree<1>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
+      case 2: hipLaunchKernelGGL(k_seg_tree<2>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
+      case 3: hipLaunchKernelGGL(k_seg_tree<3>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
+      case 4: hipLaunchKernelGGL(k_seg_tree<4>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
+      default: return hipErrorInvalidValue;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
-// Scratch of the segmented sum, as points of one array: records in plane
-// slots [0, R), their int64 segment keys in plane-0 slots [R, R + R/4), R
-// for the smallest chunks (the most records) so that any schedule fits.
+// Scratch of the segmented sum, as points of one array: the level-1 records
+// (two per chunk, for the smallest chunks so that any schedule fits) in plane
+// slots [0, R), their int64 segment keys in plane-0 slots [R, R + R/4).
 size_t segsum_scratch_points(size_t n) {
-  const size_t R = segsum_records(n, CH_MIN, CH_MIN);
+  const size_t R = 2 * tiles(n, CH_MIN);
   return R + R / 4 + 8;
 }
 
@@ -421,10 +404,10 @@ size_t segsum_gather_scratch_points(size_t n_elems, size_t n_src) {
 }
 
 hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
-                            const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int chu,
+                            const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int lr,
                             hipStream_t s) {
   if (nseg == 0) return hipSuccess;
-  if (ch1 < CH_MIN || ch1 > CH_MAX || chu < CH_MIN || chu > CH_MAX) return hipErrorInvalidValue;
+  if (ch1 < CH_MIN || ch1 > CH_MAX || lr < TREE_LR_MIN || lr > TREE_LR_MAX) return hipErrorInvalidValue;
   hipError_t e = launch_seg_empty(out, d_off, nseg, s);
   if (e != hipSuccess) return e;
   PtView aos{nullptr, 0};
@@ -436,32 +419,12 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  const size_t t1 = tiles(n_elems, ch1);
-  const size_t Rmax = segsum_records(n_elems, CH_MIN, CH_MIN);
-  PtView bufA{scratch.w, scratch.cap};
-  PtView bufB{scratch.w + 2 * t1 * 8, scratch.cap};     // same planes, later element slots
-  int64_t *keysA = reinterpret_cast<int64_t *>(scratch.w + Rmax * 8);
-  int64_t *keysB = keysA + 2 * t1;
-  PtView elems = src;
-  const int64_t *in_keys = nullptr;
-  size_t n = n_elems;
-  int level = 1;
-  bool use_a = true;
-  while (n > 0) {
-    const int ch = (level == 1) ? ch1 : chu;
-    const size_t T = tiles(n, ch);
-    const bool last = T <= 1;
-    PtView rec = last ? PtView{nullptr, 0} : (use_a ? bufA : bufB);
-    int64_t *keys = use_a ? keysA : keysB;
-    e = launch_seg_level(out, rec, keys, elems, in_keys, n, d_off, nseg, gather, level, ch, ch1, aos, aos_base, s);
-    if (e != hipSuccess || last) return e;
-    elems = rec;
-    in_keys = keys;
-    n = 2 * T;
-    ++level;
-    use_a = !use_a;
-  }
-  return hipSuccess;
+  const size_t T = tiles(n_elems, ch1);
+  const PtView rec = T > 1 ? scratch : PtView{nullptr, 0};
+  int64_t *keys = reinterpret_cast<int64_t *>(scratch.w + 2 * tiles(n_elems, CH_MIN) * 8);
+  e = launch_seg_level1(out, rec, keys, src, n_elems, d_off, nseg, gather, ch1, aos, aos_base, s);
+  if (e != hipSuccess || T <= 1) return e;
+  return launch_seg_tree(out, rec, keys, T, d_off, ch1, lr, s);
 }
 
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,

@@ -33,13 +33,14 @@ hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, 
 constexpr int TUNE_CLASSES = 4;
 struct AggTune {
   int seg1;          // log2 elements per lane, segmented-sum level 1
-  int segu;          // log2 records per lane, segmented-sum upper levels
+  int segu;          // log2 radix of the segmented-sum run tree (1..4)
   int scan;          // log2 elements per lane, every scan level
   int reduce;        // most blocks of 256 lanes in the first reduce pass
   int u32w;          // window width of the 32-bit-multiplier kernels (3 or 4)
   int vbw;           // window width of the 256-bit variable-base kernel (3 or 4)
 };
 constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
+constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed run-tree radix log2 range
 constexpr unsigned REDUCE_BLOCKS_MAX = 4096;
 int tune_class(size_t n);
 AggTune tune_for(size_t n);
@@ -59,7 +60,7 @@ size_t segsum_gather_scratch_points(size_t n_elems, size_t n_src);
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
                          PtView src, size_t n_src, PtView scratch, hipStream_t s);
 hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
-                            const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int chu,
+                            const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int lr,
                             hipStream_t s);
 
 // sort.hip: stable device sort of n keys into the CSR order of m + 1

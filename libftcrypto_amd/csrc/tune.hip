@@ -4,7 +4,8 @@
 // A schedule is chosen per size class of the input length n (n < 2^14,
 // < 2^18, < 2^22, larger):
 //   seg1    log2 elements per lane at segmented-sum level 1 (4..6)
-//   segu    log2 records per lane at the upper segmented-sum levels (4..6)
+//   segu    log2 radix of the tree that adds the chunk partials of segments
+//           spanning several level-1 chunks (1..4: radix 2, 4, 8, 16)
 //   scan    log2 elements per lane at every scan level (4..6)
 //   reduce  most blocks of 256 lanes in the first reduce pass (512..4096)
 //   u32w    window width of ciphertext x 32-bit constant: 4 (radix 16,
@@ -17,7 +18,7 @@
 // ftx_*_scratch_points stay valid whatever is picked.
 //
 // ftx_autotune times every candidate of every class on synthetic points
-// (s_i B for small s_i, segments of mean length 32) with HIP events, keeps
+// (s_i B for small s_i, lognormal segment lengths) with HIP events, keeps
 // the fastest and can write the table to a text file; ftx_tune_load reads
 // such a file back, and ft_crypto_init loads the file named by the
 // FTX_TUNE_FILE environment variable when it is set.
@@ -39,7 +40,7 @@ namespace ft {
 
 namespace {
 
-constexpr AggTune DEFAULT_TUNE = {4, 4, 4, 2048, 3, 3};
+constexpr AggTune DEFAULT_TUNE = {4, 1, 4, 2048, 3, 3};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
     T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES];
@@ -59,7 +60,7 @@ void tune_defaults() {
 
 bool valid(const AggTune &t) {
   auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
-  return ch_ok(t.seg1) && ch_ok(t.segu) && ch_ok(t.scan) && t.reduce >= 1 &&
+  return ch_ok(t.seg1) && t.segu >= TREE_LR_MIN && t.segu <= TREE_LR_MAX && ch_ok(t.scan) && t.reduce >= 1 &&
          (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4) &&
          (t.vbw == 3 || t.vbw == 4);
 }
@@ -242,13 +243,18 @@ ft_error ftx_autotune(int max_class, const char *path) {
   const PtView P{(uint32_t *)pts.p, nmax};
   TCK(launch_fixed_base(P, sc.p, true, nmax, base_table(), st.s));
   TCK(hipStreamSynchronize(st.s));
-  // segment lengths 1 + floor(-31 ln u): mean 32, a long tail
+  // skewed segment lengths: lognormal(ln 16, 1.5) clipped to [1, 65536]
+  // (median 16, mean about 49, a few segments spanning thousands of chunks),
+  // normal variates by Box-Muller from two uniforms of the SplitMix64 stream
   std::vector<int64_t> off;
-  off.reserve(nmax / 16);
+  off.reserve(nmax / 8);
   off.push_back(0);
   for (uint64_t k = 0; off.back() < (int64_t)nmax; ++k) {
-    const double u = ((mix64(0xF7C0AB00ull + k) >> 11) + 1) * (1.0 / 9007199254740993.0);
-    int64_t len = 1 + (int64_t)std::floor(-31.0 * std::log(u));
+    const double u1 = ((mix64(0xF7C0AB00ull + 2 * k) >> 11) + 1) * (1.0 / 9007199254740993.0);
+    const double u2 = (mix64(0xF7C0AB00ull + 2 * k + 1) >> 11) * (1.0 / 9007199254740992.0);
+    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    const double x = std::exp(std::log(16.0) + 1.5 * z);
+    const int64_t len = std::min<int64_t>(65536, std::max<int64_t>(1, (int64_t)std::llround(x)));
     off.push_back(std::min<int64_t>(off.back() + len, (int64_t)nmax));
   }
   TCK(hipMalloc(&out.p, off.size() * 128));          // at least one slot per segment
@@ -270,7 +276,7 @@ ft_error ftx_autotune(int max_class, const char *path) {
     AggTune best = tune_for(n);
     float bt = INFINITY, t = 0;
     for (int a = CH_MIN; a <= CH_MAX; ++a)
-      for (int b = CH_MIN; b <= CH_MAX; ++b) {
+      for (int b = TREE_LR_MIN; b <= TREE_LR_MAX; ++b) {
         ft_error e = time_ms(&t, ev, st.s, reps, [&] {
           return segsum_launch_ch(O, (const int64_t *)doffc.p, nseg, n, nullptr, P, 0, S, a, b, st.s);
         });

@@ -52,7 +52,7 @@ def test_every_segsum_schedule(n):
     rng = np.random.default_rng(n)
     cols = rng.integers(0, n, size=n, dtype=np.int32)
     want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
-    for ch1, chu in itertools.product((4, 5, 6), (4, 5, 6)):
+    for ch1, chu in itertools.product((4, 5, 6), (1, 2, 3, 4)):
         ft.tune_set(c, ch1, chu, *base[2:])
         got = ft.segmented_sum(off, g)
         assert np.array_equal(got.to_bytes_folded(), want), (ch1, chu)
@@ -159,7 +159,7 @@ def test_autotune_keeps_results(tmp_path):
     table = ft.autotune(max_class=2, path=path)
     assert os.path.exists(path)
     for c, (a, b, sc, r, w, v) in table.items():
-        assert 4 <= a <= 6 and 4 <= b <= 6 and 4 <= sc <= 6 and 1 <= r <= 4096 and w in (3, 4) and v in (3, 4)
+        assert 4 <= a <= 6 and 1 <= b <= 4 and 4 <= sc <= 6 and 1 <= r <= 4096 and w in (3, 4) and v in (3, 4)
     ft.tune_load(path)
     assert ft.tune_table() == table
     n = 20000

@@ -20,10 +20,10 @@ def test_size_classes():
 
 
 def test_set_get_save_load(tmp_path):
-    ft.tune_set(0, 5, 6, 4, 512, 3, 4)
-    ft.tune_set(3, 6, 4, 5, 4096, 4, 3)
+    ft.tune_set(0, 5, 4, 4, 512, 3, 4)
+    ft.tune_set(3, 6, 2, 5, 4096, 4, 3)
     t = ft.tune_table()
-    assert t[0] == (5, 6, 4, 512, 3, 4) and t[3] == (6, 4, 5, 4096, 4, 3)
+    assert t[0] == (5, 4, 4, 512, 3, 4) and t[3] == (6, 2, 5, 4096, 4, 3)
     p = str(tmp_path / "t.txt")
     ft.tune_save(p)
     ft.tune_set(0, 4, 4, 4, 2048, 4, 4)
@@ -31,7 +31,8 @@ def test_set_get_save_load(tmp_path):
     assert ft.tune_table() == t
 
 
-@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048, 4, 4), (0, 4, 7, 4, 2048, 4, 4), (0, 4, 4, 4, 0, 4, 4),
+@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048, 4, 4), (0, 4, 5, 4, 2048, 4, 4), (0, 4, 0, 4, 2048, 4, 4),
+                                 (0, 4, 4, 4, 0, 4, 4),
                                  (0, 4, 4, 4, 4097, 4, 4), (0, 4, 4, 4, 2048, 5, 4), (0, 4, 4, 4, 2048, 2, 4),
                                  (0, 4, 4, 4, 2048, 4, 5)])
 def test_set_rejects_out_of_range(bad):
