@@ -115,15 +115,120 @@ static hipError_t launch_chunk_scan(PtView v, size_t n, PtView pre, bool exclusi
   return hipGetLastError();
 }
 
-// chunk sums of every level, for the smallest chunk (the most sums)
-size_t scan_scratch_points(size_t n) {
-  size_t tot = 0;
-  for (size_t m = tiles(n, CH_MIN); m > 1; m = tiles(m, CH_MIN)) tot += m;
-  return tot + 1;
+// ---------------------------------------------- block scan (short levels)
+//
+// A level of at most BSCAN_MAX points is scanned by blocks of 256 lanes, one
+// point per lane: six shuffle-and-add steps scan each wave, the four wave
+// totals meet in LDS, and block totals are scanned the same way (exclusive)
+// and added back.  That is about ten adds per point, against two for a chunk
+// level, but a chain about ten adds deep instead of two chains of 2^ch: the
+// short top levels of a scan are latency-bound, the long bottom ones not.
+constexpr size_t BSCAN_MAX = (size_t)1 << 16;
+
+__device__ __forceinline__ void shfl_up_fe(fe &r, const fe &a, int d) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r.v[k] = (uint32_t)__shfl_up((int)a.v[k], (unsigned)d, 64);
 }
 
-// inclusive / exclusive scan of n points in place with chunks of 2^ch;
-// chunk sums of every level live in scratch, no allocation, no sync
+__device__ __forceinline__ void shfl_up_p3(ge_p3 &r, const ge_p3 &a, int d) {
+  shfl_up_fe(r.X, a.X, d);
+  shfl_up_fe(r.Y, a.Y, d);
+  shfl_up_fe(r.Z, a.Z, d);
+  shfl_up_fe(r.T, a.T, d);
+}
+
+// in-block inclusive (exclusive) scan of v[0, m) in place; block totals to
+// tot when tot.w is set
+__global__ void __launch_bounds__(ATPB) k_block_scan(PtView v, size_t m, PtView tot, int exclusive) {
+  __shared__ ge_p3 wt[ATPB / 64];
+  const size_t i = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  ge_p3 x;
+  if (i < m) load_p3(x, v, i);
+  else ge_p3_0(x);
+#pragma unroll 1
+  for (int d = 1; d < 64; d <<= 1) {
+    ge_p3 y;
+    shfl_up_p3(y, x, d);
+    if (lane >= d) ge_p3_add(x, x, y);
+  }
+  if (lane == 63) wt[w] = x;
+  __syncthreads();
+  ge_p3 pre;                                     // sum of the earlier waves (w >= 1)
+  if (w > 0) {
+    pre = wt[0];
+#pragma unroll 1
+    for (int q = 1; q < w; ++q) ge_p3_add(pre, pre, wt[q]);
+    ge_p3_add(x, pre, x);
+  }
+  if (tot.w && threadIdx.x == ATPB - 1) store_p3(tot, blockIdx.x, x);
+  if (exclusive) {
+    ge_p3 y;
+    shfl_up_p3(y, x, 1);
+    if (lane == 0) {
+      if (w > 0) y = pre;
+      else ge_p3_0(y);
+    }
+    x = y;
+  }
+  if (i < m) store_p3(v, i, x);
+}
+
+// v[i] = pre[i / 256] + v[i] for the blocks after the first
+__global__ void __launch_bounds__(ATPB) k_block_add(PtView v, size_t m, PtView pre) {
+  const size_t b = blockIdx.x + 1;
+  const size_t i = b * ATPB + threadIdx.x;
+  if (i >= m) return;
+  ge_p3 x, p;
+  load_p3(x, v, i);
+  load_p3(p, pre, b);
+  ge_p3_add(x, p, x);
+  store_p3(v, i, x);
+}
+
+// points of scratch the block scan of m points needs (its block totals)
+static size_t block_scan_points(size_t m) {
+  size_t tot = 0;
+  for (size_t k = tiles(m, 8); k > 1; k = tiles(k, 8)) tot += k;
+  return tot;
+}
+
+static hipError_t block_scan(PtView v, size_t m, bool exclusive, PtView work, hipStream_t s) {
+  const size_t nb = tiles(m, 8);
+  if (nb <= 1) {
+    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(ATPB), 0, s, v, m, PtView{nullptr, 0}, (int)exclusive);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nb), dim3(ATPB), 0, s, v, m, work, (int)exclusive);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = block_scan(work, nb, true, PtView{work.w + nb * 8, work.cap}, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_block_add, dim3((unsigned)(nb - 1)), dim3(ATPB), 0, s, v, m, work);
+  return hipGetLastError();
+}
+
+// scratch of a scan of n points with chunks of 2^ch: the chunk sums of every
+// level longer than BSCAN_MAX, then the block totals of the top level
+static size_t scan_plan_points(size_t n, int ch) {
+  size_t used = 0, m = n;
+  while (m > BSCAN_MAX) {
+    m = tiles(m, ch);
+    used += m;
+  }
+  return used + block_scan_points(m);
+}
+
+size_t scan_scratch_points(size_t n) {
+  size_t r = 1;
+  for (int ch = CH_MIN; ch <= CH_MAX; ++ch) r = std::max(r, scan_plan_points(n, ch));
+  return r;
+}
+
+// inclusive / exclusive scan of n points in place: chunk levels (2^ch points
+// per lane) until a level has at most BSCAN_MAX points, a block scan of that
+// level, then the chunk walks back down from each chunk's prefix; sums and
+// totals live in scratch, no allocation, no sync
 hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (ch < CH_MIN || ch > CH_MAX) return hipErrorInvalidValue;
@@ -133,7 +238,7 @@ hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, in
   lv[0] = v;
   ln[0] = n;
   size_t used = 0;
-  while (tiles(ln[L], ch) > 1) {
+  while (ln[L] > BSCAN_MAX) {
     const size_t m = tiles(ln[L], ch);
     lv[L + 1] = PtView{scratch.w + used * 8, scratch.cap};
     ln[L + 1] = m;
@@ -142,7 +247,7 @@ hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, in
     if (e != hipSuccess) return e;
     ++L;
   }
-  hipError_t e = launch_chunk_scan(lv[L], ln[L], PtView{nullptr, 0}, L > 0 ? true : exclusive, ch, s);
+  hipError_t e = block_scan(lv[L], ln[L], L > 0 ? true : exclusive, PtView{scratch.w + used * 8, scratch.cap}, s);
   for (int l = L - 1; l >= 0 && e == hipSuccess; --l)
     e = launch_chunk_scan(lv[l], ln[l], lv[l + 1], l > 0 ? true : exclusive, ch, s);
   return e;
