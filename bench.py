@@ -230,6 +230,8 @@ def run(args):
         configs["cfg3_sparse_step_graph"] = bench_cfg3(ft, hip, key, seed, args)
     if "isum" in sections:
         configs["reduce_isum"] = bench_isum(ft, hip, seed, args)
+    if "io" in sections:
+        configs["encode_decode"] = bench_io(ft, hip, seed, args)
     if "cfg4" in sections:
         configs["cfg4_serving"] = bench_cfg4(ft, key, seed, args)
     detail["configs"] = configs
@@ -461,6 +463,33 @@ def bench_isum(ft, hip, seed, args):
     return out
 
 
+def bench_io(ft, hip, seed, args):
+    """Compressed encode (canonical 32-byte form, batch inversion) and decode
+    (square root per point, failure index) of 2^n device-resident points,
+    device to device."""
+    L = ft.lib()
+    stream = L.ftx_default_stream()
+    T = Timer(hip, stream)
+    n = 1 << args.n
+    rng = np.random.default_rng(seed + 6)
+    g = ft.GeArray.from_small_scalars(rng.integers(0, 1 << 62, size=n, dtype=np.uint64))
+    d_enc, d_fail = hip.malloc(32 * n), hip.malloc(8)
+    out = ft.GeArray.alloc(n)
+    log("[bench] io: encode / decode of 2^%d points" % args.n)
+    ms_e = T(lambda: ft.check(L.ftx_encode_dev(d_enc, g.h, stream)))
+    ms_d = T(lambda: ft.check(L.ftx_decode_dev(out.h, d_enc, d_fail, stream)))
+    fail = np.zeros(1, dtype=np.uint64)
+    hip.L.hipMemcpy(fail.ctypes.data_as(ctypes.c_void_p), d_fail, 8, 2)
+    ft.check(L.ftx_synchronize(None))
+    hip.L.hipFree(d_enc)
+    hip.L.hipFree(d_fail)
+    if fail[0] != 0:
+        raise RuntimeError("decode of valid encodings reported a failure")
+    ent = lambda ms, b: {"ms": round(ms, 4), "points_per_s": n / (ms * 1e-3),
+                         "hbm_GBps": n * b / (ms * 1e-3) / 1e9}
+    return {"n": n, "encode": ent(ms_e, 128 + 32), "decode": ent(ms_d, 32 + 128)}
+
+
 def bench_cfg3(ft, hip, key, seed, args):
     """Gather-sum of in-neighbours + rerandomisation, one hipGraph per step."""
     from libftcrypto_amd import workloads as W
@@ -611,7 +640,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default aggregation schedules")
     ap.add_argument("--headline-only", action="store_true", help="skip the configurations other than the N=2^22 scalar mults")
-    ap.add_argument("--sections", default="cfg0,cfg2,cfg3,isum,cfg4",
+    ap.add_argument("--sections", default="cfg0,cfg2,cfg3,isum,io,cfg4",
                     help="configurations measured after the headline step (comma list of cfg0,cfg2,cfg3,cfg4)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first run one plain copy kernel over the N points (N x 128 B read and written)")

@@ -109,6 +109,11 @@ size_t ftx_scan_scratch_points(size_t n);
 ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, ft_ge25519_array scratch, void *stream);
 /* 32-byte canonical compressed encodings into device memory d_out */
 ft_error ftx_encode_dev(void *d_out, const_ft_ge25519_array array, void *stream);
+/* Device form of ft_array_from_bytes_folded (ftcrypto.h:193-212): decodes
+ * len(out) 32-byte encodings at d_in into out; the device uint64 at d_fail
+ * becomes 1 + the largest index that failed to decode, 0 when all decoded
+ * (the host function returns that index, or -1). */
+ft_error ftx_decode_dev(ft_ge25519_array out, const void *d_in, void *d_fail, void *stream);
 
 /* ---------------------------------------------------------- streams, graphs */
 

@@ -106,6 +106,7 @@ _SIGS = {
     "ftx_scan_dev": (i32, [vp, i32, vp, vp]),
     "ftx_gather_scratch_points": (sz, [sz, sz]),
     "ftx_encode_dev": (i32, [vp, vp, vp]),
+    "ftx_decode_dev": (i32, [vp, vp, vp, vp]),
     "ftx_stream_create": (i32, [vp]),
     "ftx_stream_destroy": (None, [vp]),
     "ftx_capture_begin": (i32, [vp]),

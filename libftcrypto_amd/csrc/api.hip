@@ -549,6 +549,13 @@ ft_error ftx_encode_dev(void *d_out, const_ft_ge25519_array array, void *stream)
   return FT_ERR_NO_ERROR;
 }
 
+ft_error ftx_decode_dev(ft_ge25519_array out, const void *d_in, void *d_fail, void *stream) {
+  // ftcrypto.cu:718-790 with device input: *d_fail = 1 + largest failing index
+  if (!d_fail) return FT_ERR_BAD_SIZE;
+  CK(launch_decode(view(out), (const uint32_t *)d_in, out->n, (unsigned long long *)d_fail, S(stream)));
+  return FT_ERR_NO_ERROR;
+}
+
 ft_error ft_array_to_bytes_folded(uint8_t *mem, size_t n_bytes, const_ft_ge25519_array array) {
   if (n_bytes / FT_FOLDED_POINT_BYTES < array->n) return FT_ERR_INSUFFICIENT_SPACE;
   DevBuf out;

@@ -162,6 +162,39 @@ def test_encode_decode_roundtrip_and_failures():
     assert e.value.code == ft.FT_ERR_BAD_SIZE
 
 
+def test_decode_dev_matches_host_decode():
+    """ftx_decode_dev (device bytes, stream-ordered, failure index in device
+    memory) against ft_array_from_bytes_folded on valid and random bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    L = ft.lib()
+    rng = np.random.default_rng(9)
+    valid = O.Array.from_scalars(scalars(3000, 113)).to_bytes()
+    junk = rng.integers(0, 256, size=2500 * 32, dtype=np.uint8)
+    junk[-32:] = valid[:32]                     # last one decodes: the failure index is an earlier one
+    st = ft.Stream()
+    for data in (valid, junk):
+        n = len(data) // 32
+        d_in, d_fail = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(d_in), len(data)) == 0
+        assert hip.hipMalloc(ctypes.byref(d_fail), 8) == 0
+        assert hip.hipMemcpy(d_in, data.ctypes.data_as(ctypes.c_void_p), len(data), 1) == 0
+        out = ft.GeArray.alloc(n)
+        ft.check(L.ftx_decode_dev(out.h, d_in, d_fail, st.h))
+        st.synchronize()
+        fail = np.zeros(1, dtype=np.uint64)
+        assert hip.hipMemcpy(fail.ctypes.data_as(ctypes.c_void_p), d_fail, 8, 2) == 0
+        hip.hipFree(d_in)
+        hip.hipFree(d_fail)
+        g, bad = ft.GeArray.from_bytes_folded(data)
+        assert int(fail[0]) - 1 == bad
+        assert np.array_equal(out.to_bytes_folded(), g.to_bytes_folded())
+    assert bad > 0
+
+
 def test_affine_and_extended_roundtrip_and_validate():
     s = scalars(300, 112)
     g = ft.GeArray.from_scalars(s)
