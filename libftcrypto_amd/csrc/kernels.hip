@@ -228,7 +228,9 @@ __global__ void __launch_bounds__(TPB) k_encode(uint32_t *out, PtView v, size_t 
 }
 
 // Decode n encodings; bad[0] receives 1 + the largest failing index (0: none).
-__global__ void __launch_bounds__(TPB) k_decode(PtView v, const uint32_t *in, size_t n,
+// Two waves per SIMD (a few spilled words outside the square-root chain)
+// hide the latency of that chain better than one wave in 334 registers.
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_decode(PtView v, const uint32_t *in, size_t n,
                                                 unsigned long long *bad) {
   const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
   long long fail = -1;
