@@ -260,15 +260,23 @@ __global__ void __launch_bounds__(TPB) k_reduce_blocks(PtView part, PtView v, si
   __shared__ __align__(16) uint32_t sh[TPB / 2 * 32];     // 128 points x 128 B
   const size_t lanes = (size_t)gridDim.x * TPB;
   size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
-  ge_p3 acc, q, nq;
+  ge_p3 acc, q;
+  PW nw;
   ge_p3_0(acc);
   if (i < n) { load_p3(acc, v, i); i += lanes; }
   if (i < n) load_p3(q, v, i);
 #pragma unroll 1
   for (; i < n; i += lanes) {
-    if (i + lanes < n) load_p3(nq, v, i + lanes);     // next point in flight during the add
+    // the next point's words are loaded before the add and unpacked after
+    // it: the barrier keeps the loads first, pinning q keeps the add after
+    // them, pinning acc keeps the wait for the words after the add
+    ld_pw(nw, v, i + lanes < n ? i + lanes : i);
+    asm volatile("" ::: "memory");
+    pin_p3(q);
     ge_p3_add(acc, acc, q);
-    q = nq;
+    pin_p3(acc);
+    pin_pw(nw);
+    un_pw(q, nw);
   }
   // tree: lanes [s, 2s) hand their sums to lanes [0, s)
   for (int s = TPB / 2; s > 0; s >>= 1) {
