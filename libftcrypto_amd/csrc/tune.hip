@@ -40,7 +40,12 @@ namespace ft {
 
 namespace {
 
-constexpr AggTune DEFAULT_TUNE = {4, 1, 4, 2048, 3, 3};
+// Defaults per size class: what the tuner picks on MI355X (elements per lane
+// 2^4 below 2^22 and 2^6 above, radix-2 then radix-4 run trees, radix-16
+// windows for the variable base below 2^18), so an untuned process is close
+// to a tuned one.
+constexpr AggTune DEFAULT_TUNE[TUNE_CLASSES] = {
+    {4, 1, 4, 2048, 3, 4}, {4, 1, 4, 512, 3, 4}, {4, 1, 4, 512, 3, 3}, {6, 2, 4, 2048, 3, 3}};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
     T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES];
@@ -48,12 +53,12 @@ std::atomic<bool> T_ready{false};
 
 void tune_defaults() {
   for (int c = 0; c < TUNE_CLASSES; ++c) {
-    T_seg1[c] = DEFAULT_TUNE.seg1;
-    T_segu[c] = DEFAULT_TUNE.segu;
-    T_scan[c] = DEFAULT_TUNE.scan;
-    T_reduce[c] = DEFAULT_TUNE.reduce;
-    T_u32w[c] = DEFAULT_TUNE.u32w;
-    T_vbw[c] = DEFAULT_TUNE.vbw;
+    T_seg1[c] = DEFAULT_TUNE[c].seg1;
+    T_segu[c] = DEFAULT_TUNE[c].segu;
+    T_scan[c] = DEFAULT_TUNE[c].scan;
+    T_reduce[c] = DEFAULT_TUNE[c].reduce;
+    T_u32w[c] = DEFAULT_TUNE[c].u32w;
+    T_vbw[c] = DEFAULT_TUNE[c].vbw;
   }
   T_ready = true;
 }
