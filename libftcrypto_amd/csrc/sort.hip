@@ -10,7 +10,7 @@
 // (no such element of res) goes to an extra bucket m after every valid key,
 // whose sum the caller discards.
 //
-// Radix pass over blocks of RS_ITEMS = 1024 elements (256 lanes x 4 rounds,
+// Radix pass over blocks of RS_ITEMS = 4096 elements (256 lanes x 16 rounds,
 // element r * 256 + lane of the block in round r):
 //   1. k_rs_hist: per-block digit histogram into hist[digit * nblocks + b];
 //   2. exclusive scan of hist (digit-major), so a digit's elements of block b
@@ -19,7 +19,9 @@
 //   3. k_rs_scatter: each element's rank among equal digits earlier in its
 //      block (wave ballots give the rank inside a wave, per-wave counts in
 //      LDS the rank across waves, a running count the rank across rounds)
-//      plus its block's start gives its place.
+//      places it in LDS in digit order; the block then writes its elements
+//      out in that order, so elements of one digit leave as one contiguous
+//      run (about 16 per digit and block) instead of 4-byte scattered writes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,7 +34,7 @@ namespace ft {
 namespace {
 
 constexpr int RS_TPB = 256;
-constexpr int RS_ROUNDS = 4;
+constexpr int RS_ROUNDS = 16;
 constexpr int RS_ITEMS = RS_TPB * RS_ROUNDS;
 constexpr int SCAN_TPB = 1024;
 
@@ -141,10 +143,25 @@ __global__ void __launch_bounds__(RS_TPB) k_rs_hist(int32_t *hist, const uint32_
 
 __global__ void __launch_bounds__(RS_TPB) k_rs_scatter(uint32_t *okeys, uint32_t *ovals, const uint32_t *keys,
                                                        const uint32_t *vals, size_t n, int shift,
-                                                       const int64_t *start, size_t nblocks) {
-  __shared__ int32_t run[256];            // elements of each digit placed in earlier rounds
+                                                       const int32_t *hist, const int64_t *start, size_t nblocks) {
+  __shared__ uint32_t sk[RS_ITEMS], sv[RS_ITEMS];
+  __shared__ int32_t dstart[256];           // first LDS slot of each digit in this block
+  __shared__ int32_t run[256];              // elements of each digit placed in earlier rounds
   __shared__ int32_t wc[RS_TPB / 64][256];  // this round: elements of each digit per wave
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  // exclusive scan of this block's digit counts (Hillis-Steele over 256)
+  const int32_t cnt = hist[(size_t)t * nblocks + blockIdx.x];
+  dstart[t] = cnt;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const int32_t y = t >= d ? dstart[t - d] : 0;
+    __syncthreads();
+    dstart[t] += y;
+    __syncthreads();
+  }
+  const int32_t mystart = dstart[t] - cnt;
+  __syncthreads();
+  dstart[t] = mystart;
   run[t] = 0;
 #pragma unroll
   for (int q = 0; q < RS_TPB / 64; ++q) wc[q][t] = 0;
@@ -167,11 +184,10 @@ __global__ void __launch_bounds__(RS_TPB) k_rs_scatter(uint32_t *okeys, uint32_t
     if (valid && lrank == 0) wc[w][d] = __popcll(same);
     __syncthreads();
     if (valid) {
-      int pre = run[d] + lrank;
-      for (int q = 0; q < w; ++q) pre += wc[q][d];
-      const size_t pos = (size_t)start[(size_t)d * nblocks + blockIdx.x] + (size_t)pre;
-      okeys[pos] = k;
-      ovals[pos] = vals[i];
+      int pos = dstart[d] + run[d] + lrank;
+      for (int q = 0; q < w; ++q) pos += wc[q][d];
+      sk[pos] = k;
+      sv[pos] = vals[i];
     }
     __syncthreads();
     int add = 0;
@@ -182,6 +198,15 @@ __global__ void __launch_bounds__(RS_TPB) k_rs_scatter(uint32_t *okeys, uint32_t
     }
     run[t] += add;
     __syncthreads();
+  }
+  // LDS slot p holds the element of local rank p - dstart[d] among digit d
+  const size_t m = n - base < (size_t)RS_ITEMS ? n - base : (size_t)RS_ITEMS;
+  for (size_t p = t; p < m; p += RS_TPB) {
+    const uint32_t k = sk[p];
+    const uint32_t d = digit_of(k, shift);
+    const size_t pos = (size_t)start[(size_t)d * nblocks + blockIdx.x] + (p - (size_t)dstart[d]);
+    okeys[pos] = k;
+    ovals[pos] = sv[p];
   }
 }
 
@@ -232,7 +257,7 @@ hipError_t csr_sort_keys(int64_t *off, const int32_t **order, const uint64_t *ke
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nblocks), dim3(RS_TPB), 0, s, hist, ka, n, shift, nblocks);
     hipError_t e = scan_counts(hstart, hist, hist_n, bsum, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nblocks), dim3(RS_TPB), 0, s, kb, vb, ka, va, n, shift,
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nblocks), dim3(RS_TPB), 0, s, kb, vb, ka, va, n, shift, hist,
                        hstart, nblocks);
     std::swap(ka, kb);
     std::swap(va, vb);
