@@ -7,8 +7,10 @@
 // is a template parameter in [2^4, 2^6]; the schedule used for an input size
 // comes from the tuning table (tune.hip).
 //
-// Scan: level sums of every chunk are scanned recursively (exclusive), then
-// every lane walks its chunk once more starting from its chunk's prefix.
+// Scan: chunk sums are taken level by level until a level has at most 2^16
+// points; that level is scanned by blocks (wave shuffles, LDS), then every
+// lane of each level below walks its chunk once more from its chunk's
+// prefix.
 //
 // Segmented sum: level 1 walks the elements; a segment that lies inside one
 // chunk is written to the output directly, one that crosses a chunk edge
