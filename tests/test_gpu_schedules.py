@@ -61,7 +61,27 @@ def test_every_segsum_schedule(n):
     ft.tune_set(c, *base)
 
 
-@pytest.mark.parametrize("n", [1, 63, 5000, 70000])
+def test_segsum_one_segment_spanning_every_chunk():
+    """One segment over all n elements between empty segments: its run of
+    chunk partials is as long as there are chunks, so every tree pass of every
+    radix combines; the gather form with reversed indices as well."""
+    n = 70001
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    s = ft.random_scalars(n, 4300, 0)
+    g, o = ft.GeArray.from_scalars(s), O.Array.from_scalars(s)
+    off = np.array([0, 0, n, n], dtype=np.int64)
+    want = O.segmented_sum(off, o).to_bytes()
+    cols = np.arange(n - 1, -1, -1, dtype=np.int32)
+    want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
+    for ch1, lr in itertools.product((4, 5, 6), (1, 2, 3, 4)):
+        ft.tune_set(c, ch1, lr, *base[2:])
+        assert np.array_equal(ft.segmented_sum(off, g).to_bytes_folded(), want), (ch1, lr)
+        assert np.array_equal(ft.gather_sum(off, cols, g).to_bytes_folded(), want_g), ("gather", ch1, lr)
+    ft.tune_set(c, *base)
+
+
+@pytest.mark.parametrize("n", [1, 63, 5000, 65536, 65537, 70000])
 def test_every_scan_schedule(n):
     c = ft.size_class(n)
     base = ft.tune_table()[c]
