@@ -217,6 +217,83 @@ def test_affine_and_extended_roundtrip_and_validate():
     assert ft.GeArray.from_bytes_affine(bad).validate() == 41
 
 
+# ------------------------------------------------------------ small-order components
+
+def order8_torsion():
+    """E[8] = {k T : k = 0..7} for a point T of order 8 ([l] P of a decoded P)."""
+    y = 2
+    while True:
+        pt = edref.decode(int.to_bytes(y, 32, "little"))
+        if pt is not None:
+            t = edref.mul(L, pt)
+            if edref.mul(4, t) != edref.IDENT:
+                return [edref.mul(k, t) for k in range(8)]
+        y += 1
+
+
+def test_points_with_torsion_components():
+    """Points s B + k T (T of order 8): decode, variable-base mult (edge
+    scalars included), scale, x u32, add / sub / neg, reduce and contains
+    against the oracle; validate follows the reference's main-subgroup test
+    X(l P) = 0 (sodium_ge.cu:795-882), which passes a component of order 1
+    or 2 and fails one of order 4 or 8."""
+    e8 = order8_torsion()
+    rng = np.random.default_rng(31)
+    n = 128
+    sv = [int(v) for v in rng.integers(1, 1 << 30, size=n)]
+    pts = [edref.add(edref.mul(sv[i], edref.G), e8[i % 8]) for i in range(n)]
+    enc = np.frombuffer(b"".join(edref.encode(q) for q in pts), dtype=np.uint8)
+    g, bad = ft.GeArray.from_bytes_folded(enc)
+    o, obad = O.Array.from_bytes(enc)
+    assert bad == -1 and obad == -1
+    assert np.array_equal(g.to_bytes_folded(), o.to_bytes())
+    # variable-base mult, random and edge scalars
+    ks = np.concatenate([edge_scalars(), scalars(n - len(edge_scalars()), 130)])
+    gm, om = ft.GeArray.from_bytes_folded(enc)[0], O.Array.from_bytes(enc)[0]
+    gm.mul(ks)
+    om.mul(ks)
+    same(gm, om)
+    gs, osc = ft.GeArray.from_bytes_folded(enc)[0], O.Array.from_bytes(enc)[0]
+    gs.scale(ks[5])
+    osc.scale(bytes(ks[5]))
+    same(gs, osc)
+    k32 = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    k32[:4] = [0, 1, 8, 2**32 - 1]
+    gu, ou = ft.GeArray.from_bytes_folded(enc)[0], O.Array.from_bytes(enc)[0]
+    gu.mul_u32(k32)
+    ou.mul_u32(k32)
+    same(gu, ou)
+    # add / sub / neg against arrays shifted by other torsion elements
+    sh = [edref.add(edref.mul(sv[i] + 7, edref.G), e8[(3 * i + 1) % 8]) for i in range(n)]
+    enc2 = np.frombuffer(b"".join(edref.encode(q) for q in sh), dtype=np.uint8)
+    for op in ("add", "sub"):
+        ga, oa = ft.GeArray.from_bytes_folded(enc)[0], O.Array.from_bytes(enc)[0]
+        getattr(ga, op)(ft.GeArray.from_bytes_folded(enc2)[0])
+        getattr(oa, op)(O.Array.from_bytes(enc2)[0])
+        same(ga, oa)
+    gn, on = ft.GeArray.from_bytes_folded(enc)[0], O.Array.from_bytes(enc)[0]
+    gn.neg()
+    on.neg()
+    same(gn, on)
+    gr, orr = ft.GeArray.from_bytes_folded(enc)[0], O.Array.from_bytes(enc)[0]
+    gr.reduce()
+    assert bytes(gr.to_bytes_folded()) == bytes(orr.reduce())
+    # contains: points differing only by a torsion component are different
+    probe = [edref.add(pts[i], e8[4]) if i % 2 else pts[i] for i in range(n)]
+    encp = np.frombuffer(b"".join(edref.encode(q) for q in probe), dtype=np.uint8)
+    got = ft.GeArray.from_bytes_folded(encp)[0].contains(g)
+    want = [1 if encp[32 * i:32 * i + 32].tobytes() in {enc[32 * j:32 * j + 32].tobytes() for j in range(n)}
+            else 0 for i in range(n)]
+    assert list(got) == want and sum(want) >= n // 2
+    # validate: components of order 1 or 2 pass (k = 0, 4), order 4 or 8 fail
+    for k in range(8):
+        q = edref.add(edref.mul(sv[0], edref.G), e8[k])
+        one = ft.GeArray.from_bytes_folded(np.frombuffer(edref.encode(q), dtype=np.uint8))[0]
+        clean = ft.GeArray.from_small_scalars(np.arange(1, 17, dtype=np.uint64))
+        clean.assign_to_subset([9], one)
+        assert clean.validate() == (-1 if k in (0, 4) else 9), k
+
+
 # ------------------------------------------------------------ aggregation
 
 @pytest.mark.parametrize("n", [1, 9, 256, 257, 4097, 70000])
