@@ -658,7 +658,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this library's hot path runs on a single device "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(sharding across GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
     res = run(args)
     print(json.dumps(res))
