@@ -51,9 +51,16 @@ struct Lib {
   unsigned long long *flag = nullptr;
   void *stage = nullptr;           // host-input staging
   size_t stage_bytes = 0;
-  std::mutex mu;
+  std::mutex mu;                   // initialisation
+  std::mutex io;                   // the staging buffer and the flag word
 };
 Lib G;
+
+// Calls that use the shared staging buffer or flag word hold G.io from start
+// to end, so host threads calling the library at once do not reuse either
+// while another call's work still reads it (growing the staging buffer frees
+// the old one).
+#define FT_IO_LOCK std::lock_guard<std::mutex> io_lock(G.io)
 
 const char *ERR_STR[FT_ERR_MAX_ERR + 1] = {
     "No error",
@@ -406,6 +413,7 @@ ft_error ft_array_set_length(ft_ge25519_array a, size_t newlen) {
 }
 
 ft_error ft_array_init_scalar(ft_ge25519_array *array, size_t n, const uint8_t *scalar) {
+  FT_IO_LOCK;
   ArrGuard g;
   CKE(alloc_array(&g.a, n));
   if (n) {
@@ -587,6 +595,7 @@ ft_error ft_array_to_bytes(uint8_t *mem, size_t n_bytes, const_ft_ge25519_array 
 }
 
 ft_error ft_array_from_bytes_folded(int64_t *res, ft_ge25519_array *array, const uint8_t *mem, size_t n_bytes) {
+  FT_IO_LOCK;
   if (n_bytes % FT_FOLDED_POINT_BYTES) return FT_ERR_BAD_SIZE;
   const size_t n = n_bytes / 32;
   ArrGuard g;
@@ -629,6 +638,7 @@ ft_error ft_array_from_bytes(ft_ge25519_array *array, const uint8_t *mem, size_t
 }
 
 ft_error ft_array_validate(int64_t *res, const_ft_ge25519_array array) {
+  FT_IO_LOCK;
   CK(launch_decode(PtView{nullptr, 0}, nullptr, 0, G.flag, G.stream));    // clears the flag
   CK(launch_validate(G.flag, view(array), array->n, G.lwords, G.stream));
   unsigned long long bad = 0;
@@ -641,6 +651,7 @@ ft_error ft_array_validate(int64_t *res, const_ft_ge25519_array array) {
 // ------------------------------------------------------------ scalar mults
 
 ft_error ft_array_from_scalars(ft_ge25519_array *array, const uint8_t *scalars, size_t n) {
+  FT_IO_LOCK;
   ArrGuard g;
   CKE(alloc_array(&g.a, n));
   void *d;
@@ -652,6 +663,7 @@ ft_error ft_array_from_scalars(ft_ge25519_array *array, const uint8_t *scalars, 
 }
 
 ft_error ft_array_from_small_scalars(ft_ge25519_array *array, const uint64_t *small, size_t n) {
+  FT_IO_LOCK;
   ArrGuard g;
   CKE(alloc_array(&g.a, n));
   void *d;
@@ -663,6 +675,7 @@ ft_error ft_array_from_small_scalars(ft_ge25519_array *array, const uint64_t *sm
 }
 
 void ft_scale(ft_ge25519_array array, const uint8_t *scalar) {
+  FT_IO_LOCK;
   void *d;
   if (stage(scalar, 32, &d) != FT_ERR_NO_ERROR) return;
   (void)launch_varbase(view(array), (const uint32_t *)d, 0, array->n, G.stream);
@@ -670,6 +683,7 @@ void ft_scale(ft_ge25519_array array, const uint8_t *scalar) {
 }
 
 ft_error ft_mul(ft_ge25519_array array, const uint8_t *scalars, size_t n) {
+  FT_IO_LOCK;
   if (n != array->n) return FT_ERR_SIZE_MISMATCH;
   void *d;
   CKE(stage(scalars, n * 32, &d));
