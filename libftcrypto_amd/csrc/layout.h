@@ -72,14 +72,16 @@ __device__ __forceinline__ void store_identity(const PtView v, size_t i) {
   st_words(v.w + (3 * v.cap + i) * 8, z);
 }
 
-// copy of the 32 packed bytes of each coordinate, no arithmetic
+// copy of the 32 packed bytes of each coordinate, no arithmetic.  All four
+// coordinates are loaded before any is stored: the two arrays may alias, so
+// the compiler would otherwise keep each load behind the previous store (four
+// memory round trips in a row instead of one).
 __device__ __forceinline__ void copy_point(const PtView d, size_t i, const PtView s, size_t j) {
+  uint32_t w[4][8];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    uint32_t w[8];
-    ld_words(w, s.w + (c * s.cap + j) * 8);
-    st_words(d.w + (c * d.cap + i) * 8, w);
-  }
+  for (int c = 0; c < 4; ++c) ld_words(w[c], s.w + (c * s.cap + j) * 8);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) st_words(d.w + (c * d.cap + i) * 8, w[c]);
 }
 
 // A point as its packed 32-byte coordinates.  A lane walking consecutive
