@@ -752,12 +752,19 @@ static ft_error isum_checked(ft_ge25519_array res, const void *d_keys, const_ft_
 
 ft_error ft_reduce_isum(ft_ge25519_array res, const size_t *idxs, size_t n_idxs,
                         const_ft_ge25519_array summand) {
-  // ftcrypto.cu:1150-1217: res[idxs[i]] += summand[i], duplicate keys accumulate
+  // ftcrypto.cu:1150-1217: res[idxs[i]] += summand[i], duplicate keys accumulate;
+  // the keys are range-checked on the device after their copy (one pass over
+  // the caller's buffer instead of two), before anything is added
   if (n_idxs != summand->n) return FT_ERR_SIZE_MISMATCH;
-  CKE(check_idxs(idxs, n_idxs, res->n));
   if (n_idxs == 0) return FT_ERR_NO_ERROR;
+  FT_IO_LOCK;
   DevBuf dkeys;
   CKE(dkeys.upload(idxs, n_idxs * sizeof(size_t)));
+  CK(keys_out_of_range(G.flag, static_cast<const uint64_t *>(dkeys.p), n_idxs, res->n, G.stream));
+  unsigned long long bad = 0;
+  CK(hipMemcpyAsync(&bad, G.flag, 8, hipMemcpyDeviceToHost, G.stream));
+  CK(hipStreamSynchronize(G.stream));
+  if (bad) return FT_ERR_OUT_OF_BOUNDS;
   CKE(isum_checked(res, dkeys.p, summand, nullptr, nullptr, G.stream));
   CK(hipStreamSynchronize(G.stream));
   return FT_ERR_NO_ERROR;

@@ -66,6 +66,7 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
 // sort.hip: stable device sort of n keys into the CSR order of m + 1
 // buckets (keys 0..m-1, then every key >= m)
 size_t csr_sort_bytes(size_t n);
+hipError_t keys_out_of_range(unsigned long long *bad, const uint64_t *keys, size_t n, size_t m, hipStream_t s);
 hipError_t csr_sort_keys(int64_t *off, const int32_t **order, const uint64_t *keys, size_t n, size_t m,
                          void *scratch, hipStream_t s);
 

@@ -50,6 +50,16 @@ __global__ void __launch_bounds__(256) k_keys_init(uint32_t *key32, uint32_t *id
   idx[i] = (uint32_t)i;
 }
 
+// *bad = 1 when some key is >= m (one atomic per wave that sees one)
+__global__ void __launch_bounds__(256) k_keys_bound(unsigned long long *bad, const uint64_t *keys, size_t n,
+                                                    uint64_t m) {
+  const size_t i = blockIdx.x * (size_t)256 + threadIdx.x;
+  const bool out = i < n && keys[i] >= m;
+  if (__ballot(out) && (threadIdx.x & 63) == 0) atomicOr(bad, 1ull);
+}
+
+__global__ void k_bound_clear(unsigned long long *bad) { *bad = 0; }
+
 // off[b] = first position of a key >= b in the sorted keys, b in [0, nb]
 __global__ void __launch_bounds__(256) k_offsets(int64_t *off, const uint32_t *sorted, size_t n, size_t nb) {
   const size_t b = blockIdx.x * (size_t)256 + threadIdx.x;
@@ -219,6 +229,13 @@ hipError_t scan_counts(int64_t *out, const int32_t *v, size_t n, int64_t *bsum, 
 }
 
 }  // namespace
+
+// *bad (a device word) = 1 when some of the n keys is >= m, else 0
+hipError_t keys_out_of_range(unsigned long long *bad, const uint64_t *keys, size_t n, size_t m, hipStream_t s) {
+  hipLaunchKernelGGL(k_bound_clear, dim3(1), dim3(1), 0, s, bad);
+  if (n) hipLaunchKernelGGL(k_keys_bound, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bad, keys, n, (uint64_t)m);
+  return hipGetLastError();
+}
 
 // Device scratch of csr_sort_keys, in bytes, for n keys.
 size_t csr_sort_bytes(size_t n) {

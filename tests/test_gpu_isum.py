@@ -154,3 +154,26 @@ def test_isum_dev_ignores_out_of_range_keys_and_replays_in_graph():
     small = ft.GeArray.alloc(10)
     with pytest.raises(ft.FtError):
         ft.check(Lb.ftx_reduce_isum_dev(res.h, d_keys.p, summand.h, small.h, work.p, st.h))
+
+
+def test_isum_out_of_range_key_leaves_res_unchanged():
+    """ft_reduce_isum checks its keys on the device after the copy and before
+    adding anything: one key >= len(res) (the last, or 2^64 - 1) is
+    FT_ERR_OUT_OF_BOUNDS and res keeps its bytes; an empty res rejects any
+    key."""
+    rng = np.random.default_rng(3)
+    m, n = 1000, 50000
+    res = ft.GeArray.from_scalars(ft.random_scalars(m, 41, 0))
+    before = res.to_bytes()
+    src = ft.GeArray.from_small_scalars(rng.integers(0, 1 << 30, size=n, dtype=np.uint64))
+    for bad in (m, 2**64 - 1):
+        keys = rng.integers(0, m, size=n).astype(np.uint64)
+        keys[-1] = bad
+        with pytest.raises(ft.FtError) as e:
+            res.reduce_isum(keys, src)
+        assert e.value.code == ft.FT_ERR_OUT_OF_BOUNDS
+        assert np.array_equal(res.to_bytes(), before)
+    empty = ft.GeArray.alloc(0)
+    with pytest.raises(ft.FtError) as e:
+        empty.reduce_isum(np.zeros(n, dtype=np.uint64), src)
+    assert e.value.code == ft.FT_ERR_OUT_OF_BOUNDS
