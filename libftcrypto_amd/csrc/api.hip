@@ -449,13 +449,26 @@ static ft_error check_idxs(const size_t *idxs, size_t n, size_t bound) {
   return FT_ERR_NO_ERROR;
 }
 
+// Copy n host indices to the device and range-check them there (one pass
+// over the caller's buffer): FT_ERR_OUT_OF_BOUNDS when one is >= bound.
+// Uses the flag word: the caller holds FT_IO_LOCK.
+static ft_error upload_idxs(DevBuf &di, const size_t *idxs, size_t n, size_t bound) {
+  CKE(di.upload(idxs, n * sizeof(size_t)));
+  if (!n) return FT_ERR_NO_ERROR;
+  CK(keys_out_of_range(G.flag, static_cast<const uint64_t *>(di.p), n, bound, G.stream));
+  unsigned long long bad = 0;
+  CK(hipMemcpyAsync(&bad, G.flag, 8, hipMemcpyDeviceToHost, G.stream));
+  CK(hipStreamSynchronize(G.stream));
+  return bad ? FT_ERR_OUT_OF_BOUNDS : FT_ERR_NO_ERROR;
+}
+
 ft_error ft_array_get_subset(ft_ge25519_array *array, const size_t *idxs, size_t n_idxs,
                              const_ft_ge25519_array src) {
-  CKE(check_idxs(idxs, n_idxs, src->n));
+  FT_IO_LOCK;
+  DevBuf di;
+  CKE(upload_idxs(di, idxs, n_idxs, src->n));
   ArrGuard g;
   CKE(alloc_array(&g.a, n_idxs));
-  DevBuf di;
-  CKE(di.upload(idxs, n_idxs * sizeof(size_t)));
   CK(launch_gather(view(g.a), view(src), (const uint64_t *)di.p, 0, 0, n_idxs, G.stream));
   CK(hipStreamSynchronize(G.stream));
   *array = g.release();
@@ -483,17 +496,17 @@ ft_error ft_array_get_range(ft_ge25519_array *array, size_t start, size_t stop, 
 ft_error ft_array_assign_to_subset(ft_ge25519_array array, const size_t *idxs, size_t n_idxs,
                                    const_ft_ge25519_array src) {
   if (n_idxs != src->n) return FT_ERR_SIZE_MISMATCH;
-  CKE(check_idxs(idxs, n_idxs, array->n));
+  FT_IO_LOCK;
   DevBuf di;
-  CKE(di.upload(idxs, n_idxs * sizeof(size_t)));
+  CKE(upload_idxs(di, idxs, n_idxs, array->n));
   CK(launch_scatter(view(array), view(src), (const uint64_t *)di.p, 0, 0, n_idxs, G.stream));
   return FT_ERR_NO_ERROR;
 }
 
 ft_error ft_array_assign_zero_to_subset(ft_ge25519_array array, const size_t *idxs, size_t n_idxs) {
-  CKE(check_idxs(idxs, n_idxs, array->n));
+  FT_IO_LOCK;
   DevBuf di;
-  CKE(di.upload(idxs, n_idxs * sizeof(size_t)));
+  CKE(upload_idxs(di, idxs, n_idxs, array->n));
   CK(launch_scatter(view(array), PtView{nullptr, 0}, (const uint64_t *)di.p, 0, 0, n_idxs, G.stream));
   return FT_ERR_NO_ERROR;
 }
