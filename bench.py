@@ -466,7 +466,8 @@ def bench_isum(ft, hip, seed, args):
 def bench_io(ft, hip, seed, args):
     """Compressed encode (canonical 32-byte form, batch inversion) and decode
     (square root per point, failure index) of 2^n device-resident points,
-    device to device."""
+    device to device, and validation (on the curve, X(l P) = 0) of the
+    decoded points."""
     L = ft.lib()
     stream = L.ftx_default_stream()
     T = Timer(hip, stream)
@@ -478,16 +479,22 @@ def bench_io(ft, hip, seed, args):
     log("[bench] io: encode / decode of 2^%d points" % args.n)
     ms_e = T(lambda: ft.check(L.ftx_encode_dev(d_enc, g.h, stream)))
     ms_d = T(lambda: ft.check(L.ftx_decode_dev(out.h, d_enc, d_fail, stream)))
+    bad = [None]
+
+    def validate():
+        bad[0] = out.validate()
+    ms_v = T(validate)
     fail = np.zeros(1, dtype=np.uint64)
     hip.L.hipMemcpy(fail.ctypes.data_as(ctypes.c_void_p), d_fail, 8, 2)
     ft.check(L.ftx_synchronize(None))
     hip.L.hipFree(d_enc)
     hip.L.hipFree(d_fail)
-    if fail[0] != 0:
-        raise RuntimeError("decode of valid encodings reported a failure")
+    if fail[0] != 0 or bad[0] != -1:
+        raise RuntimeError("decode or validation of valid points reported a failure")
     ent = lambda ms, b: {"ms": round(ms, 4), "points_per_s": n / (ms * 1e-3),
                          "hbm_GBps": n * b / (ms * 1e-3) / 1e9}
-    return {"n": n, "encode": ent(ms_e, 128 + 32), "decode": ent(ms_d, 32 + 128)}
+    return {"n": n, "encode": ent(ms_e, 128 + 32), "decode": ent(ms_d, 32 + 128),
+            "validate": ent(ms_v, 128)}
 
 
 def bench_cfg3(ft, hip, key, seed, args):

@@ -47,7 +47,8 @@ struct Lib {
   int device = 0;
   hipStream_t stream = nullptr;
   uint32_t *btab = nullptr;        // comb table of B
-  uint32_t *lwords = nullptr;      // the group order l, 8 words
+  int8_t *lnaf = nullptr;          // width-3 NAF digits of the group order l
+  int lnaf_len = 0;
   unsigned long long *flag = nullptr;
   void *stage = nullptr;           // host-input staging
   size_t stage_bytes = 0;
@@ -208,6 +209,44 @@ ft_error segsum_run(ft_ge25519_array out, const int64_t *d_off, size_t nseg, siz
 // 32-bit-word helpers for the scalar generator / reducer
 const uint32_t L_W[8] = {0x5cf5d3edu, 0x5812631au, 0xa2f79cd6u, 0x14def9deu, 0, 0, 0, 0x10000000u};
 
+// Width-3 NAF of l, lowest digit first: digits in {0, +-1, +-3}, every
+// nonzero digit followed by at least two zeros; returns the digit count.
+// l is public, so the validation kernel may branch on these digits.
+static int naf3_of_l(int8_t d[256]) {
+  uint32_t k[9] = {0};
+  for (int q = 0; q < 8; ++q) k[q] = L_W[q];
+  int len = 0;
+  auto nonzero = [&k] {
+    for (int q = 0; q < 9; ++q)
+      if (k[q]) return true;
+    return false;
+  };
+  while (nonzero()) {
+    int v = 0;
+    if (k[0] & 1u) {
+      v = (int)(k[0] & 7u);
+      if (v > 4) v -= 8;
+      // k -= v (v odd, |v| <= 3): borrow or carry through the words
+      uint64_t c = (uint64_t)(v < 0 ? -v : v);
+      for (int q = 0; q < 9 && c; ++q) {
+        if (v > 0) {
+          const uint64_t t = (uint64_t)k[q] - c;
+          k[q] = (uint32_t)t;
+          c = (t >> 63) & 1;
+        } else {
+          const uint64_t t = (uint64_t)k[q] + c;
+          k[q] = (uint32_t)t;
+          c = t >> 32;
+        }
+      }
+    }
+    d[len++] = (int8_t)v;
+    for (int q = 0; q < 8; ++q) k[q] = (k[q] >> 1) | (k[q + 1] << 31);
+    k[8] >>= 1;
+  }
+  return len;
+}
+
 struct Barrett {
   uint32_t mu[9];   // floor(2^512 / l), 261 bits
   Barrett() {
@@ -348,9 +387,11 @@ ft_error ft_crypto_init(void) {
   if (std::string(p.gcnArchName).rfind("gfx950", 0) != 0) return FT_ERR_INSUFFICIENT_DEVICE_OR_DRIVER;
   CK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
   CK(hipMalloc(&G.btab, COMB_TABLE_WORDS * 4));
-  CK(hipMalloc(&G.lwords, 32));
+  static int8_t naf[256];
+  G.lnaf_len = naf3_of_l(naf);
+  CK(hipMalloc(&G.lnaf, sizeof naf));
   CK(hipMalloc(&G.flag, 8));
-  CK(hipMemcpyAsync(G.lwords, L_W, 32, hipMemcpyHostToDevice, G.stream));
+  CK(hipMemcpyAsync(G.lnaf, naf, sizeof naf, hipMemcpyHostToDevice, G.stream));
   CK(launch_build_comb(G.btab, PtView{nullptr, 0}, 0, true, G.stream));
   CK(hipStreamSynchronize(G.stream));
   tune_init_from_env();
@@ -653,7 +694,7 @@ ft_error ft_array_from_bytes(ft_ge25519_array *array, const uint8_t *mem, size_t
 ft_error ft_array_validate(int64_t *res, const_ft_ge25519_array array) {
   FT_IO_LOCK;
   CK(launch_decode(PtView{nullptr, 0}, nullptr, 0, G.flag, G.stream));    // clears the flag
-  CK(launch_validate(G.flag, view(array), array->n, G.lwords, G.stream));
+  CK(launch_validate(G.flag, view(array), array->n, G.lnaf, G.lnaf_len, G.stream));
   unsigned long long bad = 0;
   CK(hipMemcpyAsync(&bad, G.flag, 8, hipMemcpyDeviceToHost, G.stream));
   CK(hipStreamSynchronize(G.stream));

@@ -67,6 +67,25 @@ FT_HD void ge_add(ge_p1p1 &r, const ge_p3 &p, const ge_cached &q) {
   fe_carry(r.F);              // N
 }
 
+// r = p - q (p + (-q) with -q = (YmX, YpX, Z, -T2d) read in place); F and G
+// are carried so that every product of the p1p1 conversions stays in class
+FT_HD void ge_sub(ge_p1p1 &r, const ge_p3 &p, const ge_cached &q) {
+  fe a, b, A, B, C, D;
+  fe_sub(a, p.Y, p.X);        // A2
+  fe_add(b, p.Y, p.X);        // A1
+  fe_mul(A, a, q.YpX);
+  fe_mul(B, b, q.YmX);
+  fe_mul(C, p.T, q.T2d);
+  fe_mul(D, p.Z, q.Z);
+  fe_add(D, D, D);            // A1
+  fe_sub(r.E, B, A);          // A2
+  fe_add(r.H, B, A);          // A1
+  fe_add(r.F, D, C);          // A2
+  fe_carry(r.F);              // N
+  fe_sub(r.G, D, C);          // <= 2^31 + 2^18
+  fe_carry(r.G);              // N
+}
+
 // r = p + q for an affine precomputed q
 FT_HD void ge_madd(ge_p1p1 &r, const ge_p3 &p, const ge_precomp &q) {
   fe a, b, A, B, C, D;
@@ -102,6 +121,23 @@ FT_HD void ge_dbl(ge_p1p1 &r, const fe &X, const fe &Y, const fe &Z) {
 
 FT_HD void ge_p3_dbl(ge_p1p1 &r, const ge_p3 &p) { ge_dbl(r, p.X, p.Y, p.Z); }
 FT_HD void ge_p2_dbl(ge_p1p1 &r, const ge_p2 &p) { ge_dbl(r, p.X, p.Y, p.Z); }
+
+// One step of a double-and-add over PUBLIC signed digits: s = 2 s + d P for
+// d in {0, +-1, +-3}, c1 = P and c3 = 3P in cached form.  Branches on d, so
+// only for scalars that are not secret.
+FT_HD void ge_dbl_add_public(ge_p2 &s, int d, const ge_cached &c1, const ge_cached &c3) {
+  ge_p1p1 r;
+  ge_p2_dbl(r, s);
+  if (d != 0) {
+    ge_p3 h;
+    ge_p1p1_to_p3(h, r);
+    if (d == 1) ge_add(r, h, c1);
+    else if (d == -1) ge_sub(r, h, c1);
+    else if (d == 3) ge_add(r, h, c3);
+    else ge_sub(r, h, c3);
+  }
+  ge_p1p1_to_p2(s, r);
+}
 
 // -q for a cached point: swap Y+X and Y-X, negate 2dT (T2d of class N)
 FT_HD void ge_cached_cneg(ge_cached &q, uint32_t neg) {

@@ -12,7 +12,9 @@ hipError_t launch_scatter(PtView dst, PtView src, const uint64_t *idx, size_t st
 hipError_t launch_mux(PtView t, const int64_t *cond, PtView f, size_t n, hipStream_t s);
 hipError_t launch_index(int64_t *res, PtView v, size_t n, hipStream_t s);
 hipError_t launch_equal(int64_t *res, PtView a, PtView b, size_t n, bool neq, hipStream_t s);
-hipError_t launch_validate(unsigned long long *bad, PtView v, size_t n, const uint32_t *lwords, hipStream_t s);
+// bad = 1 + the largest index of a point off the curve or with X(l P) != 0;
+// naf: the len width-3 NAF digits of l, lowest first, in device memory
+hipError_t launch_validate(unsigned long long *bad, PtView v, size_t n, const int8_t *naf, int len, hipStream_t s);
 // res[i] = 1 when the 32-byte encoding enc[i] equals one of set_enc[0..n_set),
 // else 0; slots: set_slots(n_set) uint32 of device scratch
 size_t set_slots(size_t n_set);

@@ -63,9 +63,13 @@ __global__ void __launch_bounds__(MTPB) k_equal(int64_t *res, PtView a, PtView b
 }
 
 // on the curve ((Y^2 - X^2) Z^2 = Z^4 + d X^2 Y^2) and X(l P) = 0
-// (the reference's is_on_curve / is_on_main_subgroup, sodium_ge.cu:849-882)
+// (the reference's is_on_curve / is_on_main_subgroup, sodium_ge.cu:849-882).
+// l is public and the same for every lane, so l P is a double-and-add over
+// its width-3 NAF (digits read as wave-uniform scalars: the branches are
+// scalar, nothing depends on secret data): 252 doublings, about 30 additions
+// of P or 3P, instead of a constant-time windowed multiply.
 __global__ void __launch_bounds__(MTPB) k_validate(unsigned long long *bad, PtView v, size_t n,
-                                                   const uint32_t *lwords) {
+                                                   const int8_t *naf, int len) {
   const size_t i = blockIdx.x * (size_t)MTPB + threadIdx.x;
   long long fail = -1;
   if (i < n) {
@@ -84,13 +88,23 @@ __global__ void __launch_bounds__(MTPB) k_validate(unsigned long long *bad, PtVi
     fe_carry(t1);
     fe_sub(t0, t0, t1);
     const uint32_t on_curve = fe_iszero(t0);
-    uint32_t k[8], e[16];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) k[q] = lwords[q];
-    recode16_words(e, k);
-    ge_p3 h;
-    varbase_mul(h, p, e);
-    const uint32_t in_sub = fe_iszero(h.X);
+    ge_cached c1, c3;
+    ge_p1p1 r;
+    ge_p3 q;
+    ge_p3_to_cached(c1, p);
+    ge_p3_dbl(r, p);
+    ge_p1p1_to_p3(q, r);                        // 2P
+    ge_add(r, q, c1);
+    ge_p1p1_to_p3(q, r);                        // 3P
+    ge_p3_to_cached(c3, q);
+    ge_p2 s;                                    // the top digit (positive) starts the sum
+    const int top = __builtin_amdgcn_readfirstlane((int)naf[len - 1]);
+    if (top == 3) { s.X = q.X; s.Y = q.Y; s.Z = q.Z; }
+    else { s.X = p.X; s.Y = p.Y; s.Z = p.Z; }
+#pragma unroll 1
+    for (int b = len - 2; b >= 0; --b)
+      ge_dbl_add_public(s, __builtin_amdgcn_readfirstlane((int)naf[b]), c1, c3);
+    const uint32_t in_sub = fe_iszero(s.X);
     fail = (on_curve & in_sub) ? -1 : (long long)i;
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -267,9 +281,9 @@ hipError_t launch_equal(int64_t *res, PtView a, PtView b, size_t n, bool neq, hi
   hipLaunchKernelGGL(k_equal, dim3(mgrid(n)), dim3(MTPB), 0, s, res, a, b, n, (int)neq);
   MCHK();
 }
-hipError_t launch_validate(unsigned long long *bad, PtView v, size_t n, const uint32_t *lwords, hipStream_t s) {
+hipError_t launch_validate(unsigned long long *bad, PtView v, size_t n, const int8_t *naf, int len, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_validate, dim3(mgrid(n)), dim3(MTPB), 0, s, bad, v, n, lwords);
+  hipLaunchKernelGGL(k_validate, dim3(mgrid(n)), dim3(MTPB), 0, s, bad, v, n, naf, len);
   MCHK();
 }
 size_t set_slots(size_t n_set) {

@@ -48,6 +48,28 @@ void h_dbl_encoded(uint32_t *out, const uint32_t *a) {
   ge_p1p1_to_p3(p, t);
   ge_p3_encode(out, p);
 }
+// l-style public multiple: digits (lowest first, top digit positive, in
+// {0, +-1, +-3}) applied to the decoded point with ge_dbl_add_public
+void h_mul_public_encoded(uint32_t *out, const uint32_t *a, const int8_t *digits, int len) {
+  ge_p3 p, q;
+  ge_p1p1 r;
+  ge_cached c1, c3;
+  ge_decode(p, a);
+  ge_p3_to_cached(c1, p);
+  ge_p3_dbl(r, p);
+  ge_p1p1_to_p3(q, r);
+  ge_add(r, q, c1);
+  ge_p1p1_to_p3(q, r);
+  ge_p3_to_cached(c3, q);
+  ge_p2 s;
+  if (digits[len - 1] == 3) { s.X = q.X; s.Y = q.Y; s.Z = q.Z; }
+  else { s.X = p.X; s.Y = p.Y; s.Z = p.Z; }
+  for (int b = len - 2; b >= 0; --b) ge_dbl_add_public(s, digits[b], c1, c3);
+  ge_p3 h;
+  h.X = s.X; h.Y = s.Y; h.Z = s.Z;
+  fe_0(h.T);                      // the encoding reads X, Y, Z only
+  ge_p3_encode(out, h);
+}
 void h_neg_encoded(uint32_t *out, const uint32_t *a) {
   ge_p3 p;
   ge_decode(p, a);

@@ -162,6 +162,48 @@ def test_group_formulas(H):
         assert words_val(out) == int.from_bytes(edref.encode(edref.neg(pa)), "little")
 
 
+def naf3(k):
+    """Width-3 NAF digits of k > 0, lowest first (digits in {0, +-1, +-3})."""
+    d = []
+    while k:
+        v = 0
+        if k & 1:
+            v = k & 7
+            if v > 4:
+                v -= 8
+            k -= v
+        d.append(v)
+        k >>= 1
+    return d
+
+
+def test_public_double_and_add(H):
+    """ge_dbl_add_public (and ge_sub inside it), the step of the validation
+    kernel's l P, against Python: l and random public scalars times random
+    points and points with an order-8 component."""
+    H.h_mul_public_encoded.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    rng = random.Random(9)
+    t8 = None
+    y = 2
+    while t8 is None:
+        pt = edref.decode(int.to_bytes(y, 32, "little"))
+        if pt is not None:
+            t = edref.mul(edref.L, pt)
+            if edref.mul(4, t) != edref.IDENT:
+                t8 = t
+        y += 1
+    out = (ctypes.c_uint32 * 8)()
+    for trial in range(12):
+        k = edref.L if trial < 4 else rng.randrange(1, 2**253)
+        p = edref.mul(rng.randrange(1, edref.L), edref.G)
+        if trial % 2:
+            p = edref.add(p, edref.mul(trial, t8))
+        d = naf3(k)
+        dig = (ctypes.c_int8 * len(d))(*d)
+        H.h_mul_public_encoded(out, arr(enc_words(p)), dig, len(d))
+        assert words_val(out) == int.from_bytes(edref.encode(edref.mul(k, p)), "little"), trial
+
+
 def test_decode_semantics(H):
     rng = random.Random(8)
     out = (ctypes.c_uint32 * 8)()
