@@ -427,6 +427,21 @@ def bench_cfg2(ft, hip, key, seed, args):
     adds = 2 * (nr - 1) + 2 * (ns - nseg) + 2 * (ns - 1)
     res["all_three_concurrent"] = adds_entry(ms_all, adds)
     res["all_three_concurrent"]["sum_of_alone_ms"] = round(ms_r + ms_g + ms_s, 4)
+
+    log("[bench] cfg2: all three in one persistent launch")
+    outb = [ft.GeArray.alloc(1) for _ in range(2)]
+    jobs = [ft.agg_job(ft.AGG_REDUCE, outb[0], r1), ft.agg_job(ft.AGG_REDUCE, outb[1], r2),
+            ft.agg_job(ft.AGG_SEGSUM, s1, g1, d_off, nseg, ns), ft.agg_job(ft.AGG_SEGSUM, s2, g2, d_off, nseg, ns),
+            ft.agg_job(ft.AGG_SCAN, q1), ft.agg_job(ft.AGG_SCAN, q2)]
+    arr = (ft.AggJob * len(jobs))(*jobs)
+    bscr = ft.GeArray.alloc(L.ftx_aggregate_batch_scratch_points(arr, len(jobs)))
+    ms_b = T(lambda: ft.check(L.ftx_aggregate_batch_dev(arr, len(jobs), bscr.h, stream)))
+    res["one_persistent_launch"] = adds_entry(ms_b, adds)
+    res["one_persistent_launch"]["note"] = (
+        "ftx_aggregate_batch_dev: both halves of the reduce, segmented sum and scan as six jobs; their "
+        "element walks (the scan's first level included) share one persistent launch whose waves take "
+        "tiles from a device counter, then run-tree passes, the reduces' partial sums and the scans' "
+        "upper levels and second walk")
     ft.check(L.ftx_synchronize(None))
     for st in sts:
         L.ftx_stream_destroy(st)

@@ -85,6 +85,41 @@ size_t ftx_gather_scratch_points(size_t n_elems, size_t n_src);
 ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size_t n_seg,
                                size_t n_elems, const void *d_gather,
                                const_ft_ge25519_array src, ft_ge25519_array scratch, void *stream);
+/* Several aggregation jobs whose element walks share ONE persistent,
+ * load-balanced launch (no counterpart in the reference, which launches
+ * each reduce, scan and per-segment sum separately: ftcrypto.cu:1113-1217,
+ * 1342-1586).  Every wave takes the next tile of any job from a device
+ * counter, so uneven jobs and segment lengths balance over the GPU; run-tree
+ * passes, the sums of the reduces' partials and the upper scan levels follow
+ * on the same stream.  Jobs run concurrently: no job's out may be another
+ * job's out or src.  Kinds:
+ *   FTX_AGG_SEGSUM  out[s] = sum of src[off[s] .. off[s+1]-1], s < n_seg;
+ *                   d_offsets: n_seg + 1 device int64 values (caller-checked
+ *                   as for ftx_segmented_sum_dev), n_elems = off[n_seg]; or
+ *                   d_offsets NULL: segment s = [s seg_len, min((s+1)
+ *                   seg_len, n_elems)) with n_seg = ceil(n_elems / seg_len)
+ *   FTX_AGG_REDUCE  out[0] = sum of src (all len(src) elements)
+ *   FTX_AGG_SCAN    inclusive scan of out in place (src unused)
+ *   FTX_AGG_XSCAN   exclusive scan of out in place
+ * scratch: ftx_aggregate_batch_scratch_points(jobs, n_jobs) points (NULL: the
+ * call allocates it and synchronises).  At most FTX_AGG_MAX_JOBS jobs. */
+#define FTX_AGG_SEGSUM 0
+#define FTX_AGG_REDUCE 1
+#define FTX_AGG_SCAN 2
+#define FTX_AGG_XSCAN 3
+#define FTX_AGG_MAX_JOBS 32
+typedef struct {
+  int kind;
+  ft_ge25519_array out;
+  const_ft_ge25519_array src;
+  const void *d_offsets;
+  size_t n_seg;
+  size_t n_elems;
+  size_t seg_len;
+} ftx_agg_job;
+size_t ftx_aggregate_batch_scratch_points(const ftx_agg_job *jobs, size_t n_jobs);
+ft_error ftx_aggregate_batch_dev(const ftx_agg_job *jobs, size_t n_jobs, ft_ge25519_array scratch,
+                                 void *stream);
 /* Device-key form of ft_reduce_isum (ftcrypto.h:338-348, ftcrypto.cu:1150-1217):
  * res[keys[i]] += summand[i] for the len(summand) uint64 keys at d_keys.  A key
  * >= len(res) adds nothing (ft_reduce_isum rejects such keys on the host).
@@ -128,7 +163,8 @@ void ftx_stream_destroy(void *stream);
  * begin and end into a graph.  Capturable: ftx_from_scalars_dev, ftx_mul_dev,
  * ftx_mul_u32_dev, ftx_add_dev, ftx_encrypt_dev, ftx_rerandomize_dev,
  * ftx_encode_dev, and ftx_segmented_sum_dev / ftx_reduce_dev / ftx_scan_dev /
- * ftx_reduce_isum_dev with caller-provided scratch (and work memory). */
+ * ftx_reduce_isum_dev / ftx_aggregate_batch_dev with caller-provided scratch
+ * (and work memory). */
 ft_error ftx_capture_begin(void *stream);
 ft_error ftx_capture_end(void *stream, ftx_graph *graph);
 ft_error ftx_graph_launch(ftx_graph graph, void *stream);

@@ -97,6 +97,8 @@ _SIGS = {
     "ftx_rerandomize_dev": (i32, [vp, vp, vp, vp, vp]),
     "ftx_segsum_scratch_points": (sz, [sz]),
     "ftx_segmented_sum_dev": (i32, [vp, vp, sz, sz, vp, vp, vp, vp]),
+    "ftx_aggregate_batch_scratch_points": (sz, [vp, sz]),
+    "ftx_aggregate_batch_dev": (i32, [vp, sz, vp, vp]),
     "ftx_reduce_isum_scratch_points": (sz, [sz, sz]),
     "ftx_reduce_isum_work_bytes": (sz, [sz, sz]),
     "ftx_reduce_isum_dev": (i32, [vp, vp, vp, vp, vp, vp]),
@@ -387,6 +389,37 @@ def segmented_sum(offsets, src):
     out = ctypes.c_void_p()
     check(lib().ftx_segmented_sum(ctypes.byref(out), _p(off), len(off) - 1, src.h), "ftx_segmented_sum")
     return GeArray(out)
+
+
+AGG_SEGSUM, AGG_REDUCE, AGG_SCAN, AGG_XSCAN = 0, 1, 2, 3
+AGG_MAX_JOBS = 32
+
+
+class AggJob(ctypes.Structure):
+    """ftx_agg_job: one job of ftx_aggregate_batch_dev."""
+    _fields_ = [("kind", ctypes.c_int), ("out", ctypes.c_void_p), ("src", ctypes.c_void_p),
+                ("d_offsets", ctypes.c_void_p), ("n_seg", ctypes.c_size_t), ("n_elems", ctypes.c_size_t),
+                ("seg_len", ctypes.c_size_t)]
+
+
+def agg_job(kind, out, src=None, d_offsets=None, n_seg=0, n_elems=0, seg_len=0):
+    return AggJob(kind, out.h, src.h if src is not None else None, d_offsets, n_seg, n_elems, seg_len)
+
+
+def aggregate_batch(jobs, scratch=None, stream=None):
+    """Run the jobs (AggJob list) in one persistent launch on `stream`
+    (default: the library stream, then synchronised)."""
+    arr = (AggJob * max(1, len(jobs)))(*jobs)
+    st = stream.h if stream is not None else None
+    check(lib().ftx_aggregate_batch_dev(arr, len(jobs), scratch.h if scratch is not None else None, st),
+          "ftx_aggregate_batch_dev")
+    if stream is None:
+        check(lib().ftx_synchronize(None), "ftx_synchronize")
+
+
+def aggregate_batch_scratch_points(jobs):
+    arr = (AggJob * max(1, len(jobs)))(*jobs)
+    return lib().ftx_aggregate_batch_scratch_points(arr, len(jobs))
 
 
 def gather_sum(row_off, cols, src):

@@ -23,6 +23,10 @@
 // pass that completes it.  At these sizes the GPU is latency-bound: a wave
 // waits for its lane with the most adds, so radix 2 (one add per lane and
 // pass) suits long, skewed runs, larger radices fewer passes over short ones.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+
 #include "kernels.h"
 
 namespace ft {
@@ -230,8 +234,11 @@ size_t scan_scratch_points(size_t n) {
 // inclusive / exclusive scan of n points in place: chunk levels (2^ch points
 // per lane) until a level has at most BSCAN_MAX points, a block scan of that
 // level, then the chunk walks back down from each chunk's prefix; sums and
-// totals live in scratch, no allocation, no sync
-hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s) {
+// totals live in scratch, no allocation, no sync.  level1_done: the first
+// level's chunk sums are already in scratch slots [0, ceil(n / 2^ch)) (the
+// batched launch computed them).
+hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s,
+                          bool level1_done) {
   if (n == 0) return hipSuccess;
   if (ch < CH_MIN || ch > CH_MAX) return hipErrorInvalidValue;
   PtView lv[48];
@@ -245,8 +252,10 @@ hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, in
     lv[L + 1] = PtView{scratch.w + used * 8, scratch.cap};
     ln[L + 1] = m;
     used += m;
-    hipError_t e = launch_chunk_sums(lv[L + 1], lv[L], ln[L], ch, s);
-    if (e != hipSuccess) return e;
+    if (L > 0 || !level1_done) {
+      hipError_t e = launch_chunk_sums(lv[L + 1], lv[L], ln[L], ch, s);
+      if (e != hipSuccess) return e;
+    }
     ++L;
   }
   hipError_t e = block_scan(lv[L], ln[L], L > 0 ? true : exclusive, PtView{scratch.w + used * 8, scratch.cap}, s);
@@ -261,8 +270,23 @@ hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipSt
 
 // --------------------------------------------------------- segmented sum
 
+// Segment boundaries: CSR offsets in device memory, or uniform segments
+// s = [s len, min((s + 1) len, n)) computed on the fly.
+struct OffArr {
+  const int64_t *p;
+  __device__ __forceinline__ int64_t operator[](size_t s) const { return p[s]; }
+};
+struct OffUni {
+  int64_t len, n;
+  __device__ __forceinline__ int64_t operator[](size_t s) const {
+    const int64_t v = (int64_t)s * len;
+    return v < n ? v : n;
+  }
+};
+
 // first segment in [a, nseg) whose level-1 range ends after position p
-__device__ __forceinline__ size_t seg_find1(const int64_t *off, size_t a, size_t nseg, int64_t p) {
+template <class OffT>
+__device__ __forceinline__ size_t seg_find1(const OffT off, size_t a, size_t nseg, int64_t p) {
   size_t b = nseg;
   while (a < b) {
     const size_t mid = (a + b) >> 1;
@@ -291,11 +315,10 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
 // MODE 0: elements in order; 1: gathered from the planes of src; 2:
 // gathered from the point-contiguous copy `aos`.  Two waves per SIMD (the
 // gather forms would otherwise take a few registers over 256).
-template <int CH, int MODE>
-__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
-                                                     size_t n, const int64_t *off, size_t nseg,
-                                                     const int32_t *gather, PtView aos, size_t aos_base) {
-  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+template <int CH, int MODE, class OffT>
+__device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, PtView src, size_t n,
+                                         const OffT off, size_t nseg, const int32_t *gather, PtView aos,
+                                         size_t aos_base, size_t t) {
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
   const int64_t chi = clo + (1 << CH);
@@ -387,6 +410,14 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
 }
 
+template <int CH, int MODE>
+__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
+                                                     size_t n, const int64_t *off, size_t nseg,
+                                                     const int32_t *gather, PtView aos, size_t aos_base) {
+  seg_walk<CH, MODE>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base,
+                     blockIdx.x * (size_t)ATPB + threadIdx.x);
+}
+
 // One pass of the run tree with radix 2^LR and stride d: lane c (a level-1
 // chunk) adds partials c + d, ..., c + (2^LR - 1) d of a run into its own
 // when its offset in the run is a multiple of 2^LR d, for the run its head
@@ -406,11 +437,9 @@ __device__ __forceinline__ void tree_step(PtView rec, size_t slot, size_t c, siz
   }
 }
 
-template <int LR>
-__global__ void __launch_bounds__(ATPB) k_seg_tree(PtView out, PtView rec, const int64_t *keys, size_t T,
-                                                   const int64_t *off, int ch1, size_t d) {
-  const size_t c = blockIdx.x * (size_t)ATPB + threadIdx.x;
-  if (c >= T) return;
+template <int LR, class OffT>
+__device__ __forceinline__ void seg_tree_lane(PtView out, PtView rec, const int64_t *keys, size_t c,
+                                              const OffT off, int ch1, size_t d) {
   const size_t span = d << LR;
   const int64_t h = keys[2 * c];
   if (h >= 0) {
@@ -432,6 +461,14 @@ __global__ void __launch_bounds__(ATPB) k_seg_tree(PtView out, PtView rec, const
       else store_p3(rec, 2 * c + 1, acc);
     }
   }
+}
+
+template <int LR>
+__global__ void __launch_bounds__(ATPB) k_seg_tree(PtView out, PtView rec, const int64_t *keys, size_t T,
+                                                   const int64_t *off, int ch1, size_t d) {
+  const size_t c = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  if (c >= T) return;
+  seg_tree_lane<LR>(out, rec, keys, c, OffArr{off}, ch1, d);
 }
 
 // point-contiguous copy of src into the scratch planes' tails (aos_slot)
@@ -545,6 +582,324 @@ hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStre
   hipLaunchKernelGGL(k_seg_empty, dim3((unsigned)((nseg + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, out, off,
                      nseg);
   return hipGetLastError();
+}
+
+// ------------------------------------------------ batched aggregation
+//
+// Several reduce / segmented-sum / scan jobs in ONE persistent launch.  The
+// element work of every job is cut into wave tiles: 64 level-1 chunks of a
+// segmented sum (a scan's first level is a segmented sum with uniform
+// segments of 2^ch), or 64 lanes x 2^RCH elements of a reduce, which each
+// lane walks with a stride of 64 (coalesced) and leaves as one partial.  The
+// grid holds as many waves as the GPU keeps resident; each wave takes the
+// next tile from an atomic counter until none is left, so short and long
+// jobs, and tiles that flush more or fewer segments, balance over the whole
+// GPU.  No wave waits for another.  The dependent work follows in ordinary
+// launches: one run-tree pass per radix step over all jobs' records, the
+// sum of each reduce's partials, and the upper scan levels with the second
+// walk.
+
+constexpr int RCH = 8;                     // reduce: 2^RCH elements per lane and tile
+
+// One job as the kernels see it: 88 bytes, so that a table of AGG_BATCH_MAX
+// jobs passes by value as a kernel argument (well under the 4 KiB limit) and
+// a captured graph holds its own copy.  Tile, chunk, segment and partial
+// counts of one GPU's arrays fit 32 bits (2^32 points would be 512 GiB).
+struct BJob {
+  PtView out, src;
+  const int64_t *off;                      // CSR offsets; null: uniform segments of len
+  int64_t len;
+  uint64_t nseg, n;
+  uint32_t tb;                             // first global tile
+  uint32_t cb, cn;                         // chunk-kind jobs: first global chunk (multiple of 64), chunks
+  uint32_t sb;                             // CSR jobs: first global segment (multiple of 64)
+  uint32_t pb;                             // reduce: first partial slot
+  int kind;                                // 0 chunk walk (segmented sum, scan level 1), 1 reduce
+};
+static_assert(sizeof(BJob) == 88, "job table layout");
+
+struct BBatch {
+  int J;
+  unsigned tiles;
+  BJob job[AGG_BATCH_MAX];
+};
+static_assert(sizeof(BBatch) + 48 <= 3072, "job table must stay well inside the kernel-argument limit");
+
+// reduce tile: lane walks elements base, base + 64, ... (2^RCH of them, below
+// n) with the next point's words in flight during each add
+__device__ __forceinline__ void reduce_tile(PtView part, size_t slot, PtView v, size_t n, size_t base) {
+  ge_p3 acc, q;
+  PW nw;
+  ge_p3_0(acc);
+  const size_t end0 = base + ((size_t)64 << RCH);
+  const size_t end = end0 < n + 63 ? end0 : n + 63;    // lanes stop at their last i < n
+  size_t i = base;
+  if (i < n) { load_p3(acc, v, i); i += 64; }
+  if (i < n && i < end) load_p3(q, v, i);
+#pragma unroll 1
+  for (; i < n && i < end; i += 64) {
+    const size_t nx = (i + 64 < n && i + 64 < end) ? i + 64 : i;
+    ld_pw(nw, v, nx);
+    asm volatile("" ::: "memory");
+    pin_p3(q);
+    ge_p3_add(acc, acc, q);
+    pin_p3(acc);
+    pin_pw(nw);
+    un_pw(q, nw);
+  }
+  store_p3(part, slot, acc);
+}
+
+template <int CH>
+__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ctr) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll 1
+  for (;;) {
+    unsigned w = 0;
+    if (lane == 0) w = atomicAdd(ctr, 1u);
+    const unsigned tile = __builtin_amdgcn_readfirstlane(__shfl(w, 0));
+    if (tile >= B.tiles) break;
+    int j = 0;
+#pragma unroll 1
+    for (int q = 1; q < B.J; ++q)
+      if (B.job[q].tb <= tile) j = q;
+    const size_t lt = (size_t)(tile - B.job[j].tb);
+    if (B.job[j].kind == 1) {
+      reduce_tile(part, (size_t)B.job[j].pb + lt * 64 + lane, B.job[j].src, B.job[j].n, (lt << (6 + RCH)) + lane);
+    } else {
+      const PtView r{rec.w + (size_t)B.job[j].cb * 2 * 8, rec.cap};
+      int64_t *k = keys + 2 * (size_t)B.job[j].cb;
+      const size_t c = lt * 64 + lane;
+      if (B.job[j].off)
+        seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, OffArr{B.job[j].off}, B.job[j].nseg,
+                        nullptr, PtView{nullptr, 0}, 0, c);
+      else
+        seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n,
+                        OffUni{B.job[j].len, (int64_t)B.job[j].n}, B.job[j].nseg, nullptr, PtView{nullptr, 0},
+                        0, c);
+    }
+  }
+}
+
+// job owning global index g of the chunk (seg = false) or segment ranges;
+// ranges start at multiples of 64, so the answer is the same for a wave
+__device__ __forceinline__ size_t csr_segs(const BJob &b) { return b.off ? (size_t)b.nseg : 0; }
+
+__device__ __forceinline__ int batch_job_of(const BBatch &B, size_t g, bool seg) {
+  int j = -1;
+#pragma unroll 1
+  for (int q = 0; q < B.J; ++q) {
+    const size_t b = seg ? B.job[q].sb : B.job[q].cb;
+    const size_t m = seg ? csr_segs(B.job[q]) : B.job[q].cn;
+    if (m && b <= g) j = q;
+  }
+  return j;
+}
+
+template <int LR>
+__global__ void __launch_bounds__(ATPB) k_agg_tree(const BBatch B, PtView rec, const int64_t *keys, int ch1,
+                                                   size_t d) {
+  const size_t g = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int j = batch_job_of(B, g & ~(size_t)63, false);
+  if (j < 0) return;
+  const size_t c = g - B.job[j].cb;
+  if (c >= B.job[j].cn || d >= B.job[j].cn) return;
+  const PtView r{rec.w + (size_t)B.job[j].cb * 2 * 8, rec.cap};
+  const int64_t *k = keys + 2 * (size_t)B.job[j].cb;
+  if (B.job[j].off) seg_tree_lane<LR>(B.job[j].out, r, k, c, OffArr{B.job[j].off}, ch1, d);
+  else seg_tree_lane<LR>(B.job[j].out, r, k, c, OffUni{B.job[j].len, (int64_t)B.job[j].n}, ch1, d);
+}
+
+__global__ void __launch_bounds__(ATPB) k_agg_empty(const BBatch B) {
+  const size_t g = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int j = batch_job_of(B, g & ~(size_t)63, true);
+  if (j < 0) return;
+  const size_t s = g - B.job[j].sb;
+  if (s < csr_segs(B.job[j]) && B.job[j].off[s] == B.job[j].off[s + 1]) store_identity(B.job[j].out, s);
+}
+
+__global__ void k_agg_ctr0(unsigned *ctr) { *ctr = 0u; }
+
+namespace {
+
+struct BatchPlan {
+  BBatch B;
+  size_t C = 0, Sg = 0, P = 0;             // global chunks, CSR segments, reduce partials
+  size_t maxT = 0;                         // most chunks of one job
+  size_t rec_pts = 0, part_base = 0, red_base = 0, scan_base = 0, total = 0;
+  size_t scan_at[AGG_BATCH_MAX];           // scan jobs (by caller's index): their scratch region
+  int scan_ch[AGG_BATCH_MAX];
+  int from[AGG_BATCH_MAX];                 // caller's index of each laid-out job
+  bool fits = true;                        // every count fits the table's 32-bit fields
+};
+
+size_t up64(size_t x) { return (x + 63) & ~(size_t)63; }
+
+bool scan_kind(int k) { return k == AGG_SCAN || k == AGG_XSCAN; }
+
+// lay out the jobs: reduces first (their tiles are the longest), then the
+// chunk walks; ch1 is the level-1 chunk log2 (CH_MIN sizes the scratch for
+// any schedule)
+BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
+  BatchPlan p;
+  std::memset(&p.B, 0, sizeof p.B);
+  size_t nt = 0;
+  int q = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i < J; ++i) {
+      const AggJob &a = jobs[i];
+      const bool red = a.kind == AGG_REDUCE;
+      if ((pass == 0) != red) continue;
+      p.from[q] = i;
+      BJob &b = p.B.job[q++];
+      b.tb = (uint32_t)nt;
+      if (red) {
+        const size_t t = (a.n + ((size_t)64 << RCH) - 1) >> (6 + RCH);
+        b.kind = 1;
+        b.src = a.src;
+        b.n = a.n;
+        b.pb = (uint32_t)p.P;
+        b.cb = (uint32_t)p.C;
+        p.P += 64 * t;
+        nt += t;
+        continue;
+      }
+      const int sch = scan_kind(a.kind) ? tune_for(a.n).scan : 0;
+      b.kind = 0;
+      b.src = a.src;
+      b.n = a.n;
+      if (scan_kind(a.kind)) {
+        b.src = a.out;
+        b.len = (int64_t)1 << sch;
+        b.nseg = a.n > BSCAN_MAX ? tiles(a.n, sch) : 0;
+        if (!b.nseg) b.n = 0;                 // short scans have no chunk level
+      } else {
+        b.out = a.out;
+        b.off = a.off;
+        b.len = (int64_t)a.len;
+        b.nseg = a.nseg;
+      }
+      const size_t T = (b.n && b.nseg) ? tiles(b.n, ch1) : 0;
+      b.cb = (uint32_t)p.C;
+      b.cn = (uint32_t)T;
+      p.C += up64(T);
+      p.maxT = std::max(p.maxT, T);
+      b.sb = (uint32_t)p.Sg;
+      p.Sg += up64(b.off ? b.nseg : 0);
+      nt += up64(T) / 64;
+    }
+  }
+  p.B.J = q;
+  p.B.tiles = (unsigned)nt;
+  p.fits = nt < (1ull << 32) && p.C < (1ull << 32) && p.Sg < (1ull << 32) && p.P < (1ull << 32);
+  // scratch: records [0, 2C), keys (2C int64) and the tile counter in plane 0
+  // after them, reduce partials, one reduce's block sums, scan regions
+  p.rec_pts = 2 * p.C + (2 * p.C * 8 + 31) / 32 + 1;
+  p.part_base = p.rec_pts;
+  p.red_base = p.part_base + p.P;
+  p.scan_base = p.red_base + REDUCE_BLOCKS_MAX;
+  size_t at = p.scan_base;
+  for (int i = 0; i < J; ++i) {
+    p.scan_at[i] = at;
+    p.scan_ch[i] = scan_kind(jobs[i].kind) ? tune_for(jobs[i].n).scan : 0;
+    if (scan_kind(jobs[i].kind)) at += scan_scratch_points(jobs[i].n);
+  }
+  p.total = at;
+  return p;
+}
+
+}  // namespace
+
+// resident blocks of k_agg_batch<CH> on the whole GPU (queried once)
+template <int CH>
+unsigned agg_resident_blocks() {
+  static std::atomic<unsigned> cached{0};
+  unsigned v = cached.load(std::memory_order_relaxed);
+  if (v) return v;
+  int nb = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_agg_batch<CH>, ATPB, 0) != hipSuccess || nb < 1) nb = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 1;
+  v = (unsigned)(nb * cus);
+  cached.store(v, std::memory_order_relaxed);
+  return v;
+}
+
+template <int CH>
+void launch_agg_batch(const BBatch &B, PtView rec, int64_t *keys, PtView part, unsigned *ctr, hipStream_t s) {
+  const unsigned want = (B.tiles + ATPB / 64 - 1) / (ATPB / 64);
+  const unsigned res = agg_resident_blocks<CH>();
+  hipLaunchKernelGGL(k_agg_batch<CH>, dim3(want < res ? want : res), dim3(ATPB), 0, s, B, rec, keys, part, ctr);
+}
+
+size_t agg_batch_scratch_points(const AggJob *jobs, int J) {
+  if (J <= 0 || J > AGG_BATCH_MAX) return 1;
+  return plan_batch(jobs, J, CH_MIN).total;
+}
+
+hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream_t s) {
+  if (J <= 0) return hipSuccess;
+  if (J > AGG_BATCH_MAX) return hipErrorInvalidValue;
+  size_t walked = 0;
+  for (int i = 0; i < J; ++i)
+    if (jobs[i].kind != AGG_REDUCE) walked += jobs[i].n;
+  const AggTune tu = tune_for(walked);
+  const int ch1 = tu.seg1, lr = tu.segu;
+  if (ch1 < CH_MIN || ch1 > CH_MAX || lr < TREE_LR_MIN || lr > TREE_LR_MAX) return hipErrorInvalidValue;
+  BatchPlan p = plan_batch(jobs, J, ch1);
+  if (!p.fits || scratch.cap < p.total) return hipErrorInvalidValue;
+  auto at = [&](size_t slot) { return PtView{scratch.w + slot * 8, scratch.cap}; };
+  for (int q = 0; q < p.B.J; ++q) {
+    const int i = p.from[q];
+    if (scan_kind(jobs[i].kind)) p.B.job[q].out = at(p.scan_at[i]);
+  }
+  const PtView rec = at(0);
+  int64_t *keys = reinterpret_cast<int64_t *>(scratch.w + 2 * p.C * 8);
+  unsigned *ctr = reinterpret_cast<unsigned *>(scratch.w + (p.rec_pts - 1) * 8);
+  hipLaunchKernelGGL(k_agg_ctr0, dim3(1), dim3(1), 0, s, ctr);
+  if (p.Sg) hipLaunchKernelGGL(k_agg_empty, dim3((unsigned)((p.Sg + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, p.B);
+  if (p.B.tiles) {
+    switch (ch1) {
+      case 4: launch_agg_batch<4>(p.B, rec, keys, at(p.part_base), ctr, s); break;
+      case 5: launch_agg_batch<5>(p.B, rec, keys, at(p.part_base), ctr, s); break;
+      default: launch_agg_batch<6>(p.B, rec, keys, at(p.part_base), ctr, s); break;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 tgrid((unsigned)((p.C + ATPB - 1) / ATPB));
+  for (size_t d = 1; d < p.maxT; d <<= lr) {
+    switch (lr) {
+      case 1: hipLaunchKernelGGL(k_agg_tree<1>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
+      case 2: hipLaunchKernelGGL(k_agg_tree<2>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
+      case 3: hipLaunchKernelGGL(k_agg_tree<3>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
+      default: hipLaunchKernelGGL(k_agg_tree<4>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  for (int q = 0; q < p.B.J; ++q) {
+    const AggJob &a = jobs[p.from[q]];
+    if (a.kind != AGG_REDUCE) continue;
+    if (!a.n) {
+      e = launch_set_identity(a.out, 0, 1, s);
+    } else {
+      const BJob &b = p.B.job[q];
+      const size_t np = 64 * ((a.n + ((size_t)64 << RCH) - 1) >> (6 + RCH));
+      const unsigned g1 = reduce_grid(np, REDUCE_BLOCKS_MAX);
+      e = launch_reduce_blocks(at(p.red_base), at(p.part_base + b.pb), np, g1, s);
+      if (e == hipSuccess) e = launch_reduce_blocks(a.out, at(p.red_base), g1, 1, s);
+    }
+    if (e != hipSuccess) return e;
+  }
+  for (int i = 0; i < J; ++i) {
+    if (!scan_kind(jobs[i].kind)) continue;
+    e = scan_launch_ch(jobs[i].out, jobs[i].n, jobs[i].kind == AGG_XSCAN, at(p.scan_at[i]), p.scan_ch[i], s,
+                       jobs[i].n > BSCAN_MAX);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace ft

@@ -1069,6 +1069,80 @@ ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size
   return FT_ERR_NO_ERROR;
 }
 
+// job table of ftx_aggregate_batch_dev -> kernel jobs; FT_ERR_NO_ERROR or
+// the first size error
+static ft_error agg_jobs(const ftx_agg_job *jobs, size_t n_jobs, AggJob *out) {
+  if (n_jobs > (size_t)FTX_AGG_MAX_JOBS) return FT_ERR_BAD_SIZE;
+  for (size_t i = 0; i < n_jobs; ++i) {
+    const ftx_agg_job &a = jobs[i];
+    AggJob &b = out[i];
+    std::memset(&b, 0, sizeof b);
+    b.kind = a.kind;
+    if (!a.out) return FT_ERR_BAD_SIZE;
+    b.out = view(a.out);
+    switch (a.kind) {
+      case FTX_AGG_SEGSUM:
+        if (!a.src || a.src->d == a.out->d) return FT_ERR_BAD_SIZE;
+        if (a.out->n < a.n_seg) return FT_ERR_INSUFFICIENT_SPACE;
+        if (a.n_elems > a.src->n) return FT_ERR_OUT_OF_BOUNDS;
+        if (!a.d_offsets) {
+          if (a.seg_len == 0 || a.n_seg != (a.n_elems + a.seg_len - 1) / a.seg_len) return FT_ERR_BAD_SIZE;
+        }
+        b.src = view(a.src);
+        b.off = (const int64_t *)a.d_offsets;
+        b.len = a.seg_len;
+        b.nseg = a.n_seg;
+        b.n = a.n_elems;
+        break;
+      case FTX_AGG_REDUCE:
+        if (!a.src) return FT_ERR_BAD_SIZE;
+        if (a.out->n < 1) return FT_ERR_INSUFFICIENT_SPACE;
+        b.src = view(a.src);
+        b.n = a.src->n;
+        break;
+      case FTX_AGG_SCAN:
+      case FTX_AGG_XSCAN:
+        b.n = a.out->n;
+        break;
+      default:
+        return FT_ERR_BAD_SIZE;
+    }
+  }
+  // jobs run concurrently: a job's output is no other job's output or input
+  for (size_t i = 0; i < n_jobs; ++i)
+    for (size_t j = 0; j < n_jobs; ++j) {
+      if (i == j) continue;
+      const uint32_t *oi = jobs[i].out->d;
+      const bool scan_j = jobs[j].kind == FTX_AGG_SCAN || jobs[j].kind == FTX_AGG_XSCAN;
+      if (oi == jobs[j].out->d || (!scan_j && oi == jobs[j].src->d)) return FT_ERR_BAD_SIZE;
+    }
+  return FT_ERR_NO_ERROR;
+}
+
+size_t ftx_aggregate_batch_scratch_points(const ftx_agg_job *jobs, size_t n_jobs) {
+  AggJob kj[FTX_AGG_MAX_JOBS];
+  if (agg_jobs(jobs, n_jobs, kj) != FT_ERR_NO_ERROR) return 1;
+  return agg_batch_scratch_points(kj, (int)n_jobs);
+}
+
+ft_error ftx_aggregate_batch_dev(const ftx_agg_job *jobs, size_t n_jobs, ft_ge25519_array scratch,
+                                 void *stream) {
+  AggJob kj[FTX_AGG_MAX_JOBS];
+  CKE(agg_jobs(jobs, n_jobs, kj));
+  if (!n_jobs) return FT_ERR_NO_ERROR;
+  const size_t need = agg_batch_scratch_points(kj, (int)n_jobs);
+  ArrGuard own;
+  if (!scratch) {
+    CKE(alloc_array(&own.a, need));
+    scratch = own.a;
+  } else if (scratch->cap < need) {
+    return FT_ERR_INSUFFICIENT_SPACE;
+  }
+  CK(agg_batch_launch(kj, (int)n_jobs, view(scratch), S(stream)));
+  if (own.a) CK(hipStreamSynchronize(S(stream)));
+  return FT_ERR_NO_ERROR;
+}
+
 static ft_error check_csr(const int64_t *off, size_t nrows, size_t limit) {
   if (off[0] < 0) return FT_ERR_OUT_OF_BOUNDS;
   for (size_t s = 0; s < nrows; ++s)

@@ -49,7 +49,8 @@ void tune_init_from_env();
 // aggregate.hip
 size_t scan_scratch_points(size_t n);             // enough for every schedule
 hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s);
-hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s);
+hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s,
+                          bool level1_done = false);
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s);
 size_t segsum_scratch_points(size_t n_elems);     // enough for every schedule
 // gather form with n_src source points: a scratch of at least
@@ -62,6 +63,23 @@ hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n
 hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems,
                             const int32_t *gather, PtView src, size_t n_src, PtView scratch, int ch1, int lr,
                             hipStream_t s);
+
+// Batched aggregation: up to AGG_BATCH_MAX jobs whose element walks share
+// one persistent, load-balanced launch (aggregate.hip).  Jobs run
+// concurrently: no job may write an array another job reads or writes.
+enum AggKind { AGG_SEGSUM = 0, AGG_REDUCE = 1, AGG_SCAN = 2, AGG_XSCAN = 3 };
+constexpr int AGG_BATCH_MAX = 32;
+struct AggJob {
+  int kind;
+  PtView out;          // segmented sum: nseg sums; reduce: one sum; scan: the array, in place
+  PtView src;          // segmented sum, reduce: the summed points
+  const int64_t *off;  // segmented sum: nseg + 1 device CSR offsets, or null: segments of len
+  size_t len;          // uniform segment length (nseg = ceil(n / len))
+  size_t nseg;
+  size_t n;            // elements walked
+};
+size_t agg_batch_scratch_points(const AggJob *jobs, int J);   // enough for every schedule
+hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream_t s);
 
 // sort.hip: stable device sort of n keys into the CSR order of m + 1
 // buckets (keys 0..m-1, then every key >= m)
