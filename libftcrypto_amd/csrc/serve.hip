@@ -8,7 +8,9 @@
 //   later queued request of the same kind that does not touch a vector an
 //   earlier skipped request touches (per-vector order is kept), up to a batch
 //   element cap, and launches them as ONE kernel whose lanes find their
-//   request in a descriptor table.  Batches go to the first idle of several
+//   request in a descriptor table; segmented sums (up to 16 requests) are
+//   the jobs of one persistent aggregation launch whose waves take tiles of
+//   any request from a device counter.  Batches go to the first idle of several
 //   default-priority streams, so batches of different kinds overlap on the
 //   device; a batch touching a vector still in flight on another stream waits
 //   for that stream's event.
@@ -140,6 +142,8 @@ using namespace ft;
 using Clock = std::chrono::steady_clock;
 
 enum Kind { K_ENCRYPT = 0, K_ADD = 1, K_MUL = 2, K_SEGSUM = 3 };
+// segmented sums per batch: two aggregation jobs each (the C1 and C2 halves)
+constexpr size_t SEGSUM_BATCH_MAX = AGG_BATCH_MAX / 2;
 struct Handle {
   size_t n = 0;             // ciphertexts
   bool allocated = false;
@@ -312,6 +316,7 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
   }
   // host staging: inputs then descriptors
   std::vector<BItem> items(batch.size());
+  std::vector<char *> dins(batch.size());
   size_t off = 0;
   uint64_t start = 0;
   int nd = 1;
@@ -328,6 +333,7 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
       it.b2 = Sx.c2;
     }
     char *din = sl.dev + off;
+    dins[j] = din;
     if (kind == K_ENCRYPT) {
       it.m = reinterpret_cast<const uint64_t *>(din);
       it.r = reinterpret_cast<const uint32_t *>(din + r.n * 8);
@@ -392,20 +398,34 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
     e = hipGetLastError();
   }
   if (e == hipSuccess && kind == K_SEGSUM) {
-    Request &r = *batch[0];
-    Handle &Sx = S->handles[r.src];
-    Handle &D = S->handles[r.dst];
-    const size_t n_el = (size_t)reinterpret_cast<const int64_t *>(in_ptr(r, 0))[r.nseg];
-    const size_t pts = segsum_scratch_points(n_el);
+    // every request of the batch is two jobs (C1 and C2 halves) of one
+    // persistent aggregation launch; the batch's requests touch distinct
+    // vectors, so no job writes what another reads
+    std::vector<AggJob> jobs;
+    for (size_t j = 0; j < batch.size(); ++j) {
+      Request &r = *batch[j];
+      Handle &Sx = S->handles[r.src];
+      Handle &D = S->handles[r.dst];
+      const size_t n_el = (size_t)reinterpret_cast<const int64_t *>(in_ptr(r, 0))[r.nseg];
+      const int64_t *doff = reinterpret_cast<const int64_t *>(dins[j]);
+      for (int h = 0; h < 2; ++h) {
+        AggJob a{};
+        a.kind = AGG_SEGSUM;
+        a.out = h ? D.c2 : D.c1;
+        a.src = h ? Sx.c2 : Sx.c1;
+        a.off = doff;
+        a.nseg = r.nseg;
+        a.n = n_el;
+        jobs.push_back(a);
+      }
+    }
+    const size_t pts = agg_batch_scratch_points(jobs.data(), (int)jobs.size());
     // scratch after the inputs; the slot is idle so it may be reallocated
     char *scr = nullptr;
     if (hipMallocAsync(reinterpret_cast<void **>(&scr), pts * 128, sl.st) != hipSuccess) {
       e = hipErrorOutOfMemory;
     } else {
-      PtView sv{reinterpret_cast<uint32_t *>(scr), pts};
-      const int64_t *doff = reinterpret_cast<const int64_t *>(sl.dev);
-      e = segsum_launch(D.c1, doff, r.nseg, n_el, nullptr, Sx.c1, 0, sv, sl.st);
-      if (e == hipSuccess) e = segsum_launch(D.c2, doff, r.nseg, n_el, nullptr, Sx.c2, 0, sv, sl.st);
+      e = agg_batch_launch(jobs.data(), (int)jobs.size(), PtView{reinterpret_cast<uint32_t *>(scr), pts}, sl.st);
       (void)hipFreeAsync(scr, sl.st);
     }
   }
@@ -479,7 +499,8 @@ void scheduler(ftx_server_s *S) {
       if (ready > 0) {
         if (batch.empty()) {
           take = true;
-        } else if (r.kind == kind && r.kind != K_SEGSUM && elems + r.n <= S->max_batch) {
+        } else if (r.kind == kind && elems + r.n <= S->max_batch &&
+                   (r.kind != K_SEGSUM || batch.size() < SEGSUM_BATCH_MAX)) {
           take = true;
         }
       }
@@ -509,7 +530,7 @@ void scheduler(ftx_server_s *S) {
         if (r.src) in_batch.insert(r.src);
         batch.push_back(*it);
         it = S->queue.erase(it);
-        if (kind == K_SEGSUM) break;
+        if (kind == K_SEGSUM && batch.size() == SEGSUM_BATCH_MAX) break;
       } else {
         if (r.dst) blocked.insert(r.dst);
         if (r.src) blocked.insert(r.src);

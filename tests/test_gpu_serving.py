@@ -93,3 +93,39 @@ def test_bad_requests(key):
         srv.wait(q3)
     assert e.value.code == ft.FT_ERR_OUT_OF_MEMORY
     srv.close()
+
+
+def test_segsum_requests_batched(key):
+    """24 segmented-sum requests over 24 distinct resident vectors, queued
+    together: the scheduler runs them as jobs of shared persistent launches
+    (fewer batches than requests), and each result equals the one-shot
+    ftx_segmented_sum of the same ciphertexts."""
+    k, _ = key
+    rng = np.random.default_rng(2024)
+    srv = SV.Server(k, budget_bytes=1 << 30, n_streams=2)
+    srcs = []
+    for i in range(24):
+        n = int(rng.integers(300, 6000))
+        m = rng.integers(0, 256, size=n, dtype=np.uint64)
+        r = ft.random_scalars(n, 900 + i, i)
+        q, h = srv.encrypt(m, r, 8)
+        srv.wait(q)
+        off = SV.lognormal_segments(max(1, n // 16), n, 31 + i)
+        srcs.append((h, off, ft.encrypt(k, m, r, msg_bits=8)))
+    # two large encryptions occupy both streams, so the segmented sums
+    # submitted next are all queued when a stream frees up
+    big = 1 << 20
+    busy = [srv.encrypt(np.zeros(big, dtype=np.uint64), ft.random_scalars(big, 77, j), 1)[0] for j in range(2)]
+    before = srv.stats()["batches"]
+    reqs = [srv.segsum(h, off) for h, off, _ in srcs]
+    for q in busy:
+        srv.wait(q)
+    for q, _ in reqs:
+        srv.wait(q)
+    batches = srv.stats()["batches"] - before
+    assert batches <= 4, batches
+    for (q, h), (_, off, (c1, c2)) in zip(reqs, srcs):
+        a, b = srv.read(h, len(off) - 1)
+        assert np.array_equal(a, ft.segmented_sum(off, c1).to_bytes_folded())
+        assert np.array_equal(b, ft.segmented_sum(off, c2).to_bytes_folded())
+    srv.close()
