@@ -403,7 +403,12 @@ class AggJob(ctypes.Structure):
 
 
 def agg_job(kind, out, src=None, d_offsets=None, n_seg=0, n_elems=0, seg_len=0):
-    return AggJob(kind, out.h, src.h if src is not None else None, d_offsets, n_seg, n_elems, seg_len)
+    """One job; it holds references to its arrays, so an array made inline
+    (e.g. a get_range) lives as long as the job does, not just until this
+    call returns (the structure itself keeps only the raw handles)."""
+    j = AggJob(kind, out.h, src.h if src is not None else None, d_offsets, n_seg, n_elems, seg_len)
+    j._arrays = (out, src)
+    return j
 
 
 def aggregate_batch(jobs, scratch=None, stream=None):

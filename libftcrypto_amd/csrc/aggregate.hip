@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <mutex>
 
 #include "kernels.h"
 
@@ -811,6 +812,68 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
 
 }  // namespace
 
+// Fork of the scans' dependent work onto library-owned streams (default
+// priority, created on first use) and its join back into the caller's
+// stream.  Held under a mutex from fork to join, so calls from several host
+// threads enqueue their record / wait pairs in order.  While s is being
+// captured into a graph, next() returns s itself (no fork).
+constexpr int AUX_STREAMS = 4;
+
+class ScanFork {
+ public:
+  explicit ScanFork(hipStream_t s) : s_(s), lk_(mu(), std::defer_lock) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    lk_.lock();
+    if (!ready() || hipEventRecord(ev_fork(), s) != hipSuccess) {
+      lk_.unlock();
+      return;
+    }
+    on_ = true;
+  }
+  ~ScanFork() { (void)join(); }
+  // stream for the next scan
+  hipStream_t next() {
+    if (!on_) return s_;
+    if (used_ == AUX_STREAMS) return aux()[rr_++ % AUX_STREAMS];   // all forked: round robin
+    if (hipStreamWaitEvent(aux()[used_], ev_fork(), 0) != hipSuccess) return s_;
+    return aux()[used_++];
+  }
+  hipError_t join() {
+    if (!on_) return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < used_; ++k) {
+      hipError_t r = hipEventRecord(ev_join()[k], aux()[k]);
+      if (r == hipSuccess) r = hipStreamWaitEvent(s_, ev_join()[k], 0);
+      if (r != hipSuccess && e == hipSuccess) e = r;
+    }
+    on_ = false;
+    lk_.unlock();
+    return e;
+  }
+
+ private:
+  static std::mutex &mu() { static std::mutex m; return m; }
+  static hipStream_t *aux() { static hipStream_t a[AUX_STREAMS]; return a; }
+  static hipEvent_t &ev_fork() { static hipEvent_t e; return e; }
+  static hipEvent_t *ev_join() { static hipEvent_t e[AUX_STREAMS]; return e; }
+  static bool ready() {            // under mu()
+    static bool done = false;
+    if (done) return true;
+    for (int k = 0; k < AUX_STREAMS; ++k)
+      if (hipStreamCreateWithFlags(&aux()[k], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&ev_join()[k], hipEventDisableTiming) != hipSuccess)
+        return false;
+    if (hipEventCreateWithFlags(&ev_fork(), hipEventDisableTiming) != hipSuccess) return false;
+    done = true;
+    return true;
+  }
+  hipStream_t s_;
+  std::unique_lock<std::mutex> lk_;
+  bool on_ = false;
+  int used_ = 0, rr_ = 0;
+};
+
 // resident blocks of k_agg_batch<CH> on the whole GPU (queried once)
 template <int CH>
 unsigned agg_resident_blocks() {
@@ -879,6 +942,18 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  // the scans' upper levels and second walks go to auxiliary streams forked
+  // from s here (eagerly; inside a graph capture they stay on s), so that
+  // the two halves of a ciphertext scan, the run-tree passes and the reduce
+  // tails, all latency-bound at their small top levels, fill each other's
+  // gaps; each scan has its own scratch region
+  ScanFork fk(s);
+  for (int i = 0; i < J && e == hipSuccess; ++i) {
+    if (!scan_kind(jobs[i].kind)) continue;
+    e = scan_launch_ch(jobs[i].out, jobs[i].n, jobs[i].kind == AGG_XSCAN, at(p.scan_at[i]), p.scan_ch[i],
+                       fk.next(), jobs[i].n > BSCAN_MAX);
+  }
+  if (e != hipSuccess) return e;
   for (int q = 0; q < p.B.J; ++q) {
     const AggJob &a = jobs[p.from[q]];
     if (a.kind != AGG_REDUCE) continue;
@@ -893,13 +968,7 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
     }
     if (e != hipSuccess) return e;
   }
-  for (int i = 0; i < J; ++i) {
-    if (!scan_kind(jobs[i].kind)) continue;
-    e = scan_launch_ch(jobs[i].out, jobs[i].n, jobs[i].kind == AGG_XSCAN, at(p.scan_at[i]), p.scan_ch[i], s,
-                       jobs[i].n > BSCAN_MAX);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return fk.join();
 }
 
 }  // namespace ft

@@ -161,23 +161,37 @@ def test_batch_all_job_slots():
 
 
 def test_batch_graph_replays():
-    """A batch captured in a hipGraph with caller scratch, replayed three
-    times with eager work in between: every replay equals the eager call."""
+    """A batch captured in a hipGraph with caller scratch (the scanned array
+    regenerated from device scalars first, so every replay starts from the
+    same input), replayed three times with eager work in between: every
+    replay equals the eager call, whose scans ran on forked streams."""
     rng = np.random.default_rng(3)
     _, X = points(100000, 6)
     off = ragged_offsets(rng, 15000, extra_long=[(10, 20000)])
     assert off[-1] <= 100000
     d_off = DevBuf(off)
-    seg, red = ft.GeArray.alloc(15000), ft.GeArray.alloc(1)
-    jobs = [ft.agg_job(ft.AGG_SEGSUM, seg, X, d_off.p, 15000, int(off[-1])), ft.agg_job(ft.AGG_REDUCE, red, X)]
+    ks = ft.random_scalars(80000, 61, 6)
+    d_ks = DevBuf(ks)
+    seg, red, scan = ft.GeArray.alloc(15000), ft.GeArray.alloc(1), ft.GeArray.alloc(80000)
+    jobs = [ft.agg_job(ft.AGG_SEGSUM, seg, X, d_off.p, 15000, int(off[-1])), ft.agg_job(ft.AGG_REDUCE, red, X),
+            ft.agg_job(ft.AGG_SCAN, scan)]
     scratch = ft.GeArray.alloc(ft.aggregate_batch_scratch_points(jobs))
     st = ft.Stream()
-    ft.aggregate_batch(jobs, scratch, st)
-    st.synchronize()
-    want_seg, want_red = enc(seg), enc(red)
-    assert np.array_equal(want_seg, enc(ft.segmented_sum(off, X)))
-    with ft.Graph(st) as g:
+    Lb = ft.lib()
+
+    def step():
+        ft.check(Lb.ftx_from_scalars_dev(scan.h, d_ks.p, st.h))
         ft.aggregate_batch(jobs, scratch, st)
+
+    step()
+    st.synchronize()
+    want_seg, want_red, want_scan = enc(seg), enc(red), enc(scan)
+    assert np.array_equal(want_seg, enc(ft.segmented_sum(off, X)))
+    o = O.Array.from_scalars(ks)
+    o.scan()
+    assert np.array_equal(want_scan, o.to_bytes())
+    with ft.Graph(st) as g:
+        step()
     junk = ft.GeArray.from_scalars(ft.random_scalars(15000, 9, 9))
     for _ in range(3):
         # eager work that overwrites both outputs before each replay
@@ -188,6 +202,7 @@ def test_batch_graph_replays():
         st.synchronize()
         assert np.array_equal(enc(seg), want_seg)
         assert np.array_equal(enc(red), want_red)
+        assert np.array_equal(enc(scan), want_scan)
 
 
 def test_batch_refusals():
@@ -218,3 +233,50 @@ def test_batch_refusals():
     assert rc([seg], ft.GeArray.alloc(1)) == INSUFFICIENT_SPACE              # scratch too small
     assert np.array_equal(enc(out), before)                                  # nothing written
     assert rc([]) == 0
+
+
+def test_batch_from_several_threads():
+    """Four host threads, each on its own stream and arrays, call batches
+    whose scans fork onto the library's auxiliary streams: every result
+    equals the same batch run alone."""
+    import threading
+    rng = np.random.default_rng(44)
+    cases = []
+    for t in range(4):
+        ks = ft.random_scalars(70000 + 1000 * t, 71, t)
+        _, X = points(20000, 20 + t)
+        off = ragged_offsets(rng, 3000)
+        assert off[-1] <= 20000
+        cases.append((ks, X, off, DevBuf(ks), DevBuf(off)))
+    Lb = ft.lib()
+
+    def run(case, st):
+        ks, X, off, d_ks, d_off = case
+        scan, seg = ft.GeArray.alloc(len(ks)), ft.GeArray.alloc(len(off) - 1)
+        short = X.get_range(0, 100, 1)
+        ft.check(Lb.ftx_from_scalars_dev(scan.h, d_ks.p, st.h))
+        ft.aggregate_batch([ft.agg_job(ft.AGG_SCAN, scan), ft.agg_job(ft.AGG_XSCAN, short),
+                            ft.agg_job(ft.AGG_SEGSUM, seg, X, d_off.p, len(off) - 1, int(off[-1]))], None, st)
+        st.synchronize()
+        return enc(scan), enc(seg)
+
+    st0 = ft.Stream()
+    want = [run(c, st0) for c in cases]
+    got = [None] * 4
+    errs = []
+
+    def worker(t):
+        try:
+            st = ft.Stream()
+            for _ in range(3):
+                got[t] = run(cases[t], st)
+                assert np.array_equal(got[t][0], want[t][0]) and np.array_equal(got[t][1], want[t][1])
+        except Exception as e:          # reported below
+            errs.append((t, e))
+
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errs, errs
