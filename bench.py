@@ -235,6 +235,10 @@ def run(args):
     if "cfg4" in sections:
         configs["cfg4_serving"] = bench_cfg4(ft, key, seed, args)
     detail["configs"] = configs
+    if not args.no_cpu and configs:
+        log("[bench] CPU port on samples of configs 0, 2 and 3")
+        H = bytes(ft.GeArray.from_scalars(x).to_bytes_folded())
+        detail["cpu_port_by_config"] = cpu_config_rates(H, seed, configs)
     detail["aggregation_schedules"] = schedules
     return {
         "metric": "scalar-mults/s (fixed-base, variable-base, ciphertext x u32) on 1 MI355X",
@@ -623,6 +627,75 @@ def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
             "seconds": round(total, 2),
             "one_thread": {"value": 4 * s1 / one, "sample": "%d of each kind" % s1, "seconds": round(one, 2)},
             "libsodium_one_thread": libsodium_rates(k_fixed, k_var)}
+
+
+def cpu_config_rates(H, seed, configs):
+    """The oracle (a C port of the reference CPU path's algorithm, all host
+    cores) on bounded samples of configs 0, 2 and 3, in the units of the GPU
+    lines, with the GPU / CPU ratio where the GPU line was measured."""
+    from oracle import oracle as O
+    from libftcrypto_amd import workloads as W
+    import libftcrypto_amd as ft
+    nt = O.threads()
+    rng = np.random.default_rng(seed + 9)
+
+    def timed(fn):
+        t0 = time.time()
+        fn()
+        return time.time() - t0
+
+    res = {"cores": nt, "kind": "port"}
+    # config 0: encrypt, ciphertext add, reduce
+    n0 = 1 << 15
+    m = rng.integers(0, 2, size=n0, dtype=np.uint64)
+    r = ft.random_scalars(n0, seed, 40)
+    box = {}
+    t_enc = timed(lambda: box.update(a=O.encrypt(H, m, r, nt=nt)))
+    a1, a2 = box["a"]
+    b1, b2 = O.encrypt(H, m, r, nt=nt)
+    t_add = timed(lambda: (a1.add(b1, nt=nt), a2.add(b2, nt=nt)))
+    t_red = timed(lambda: (a1.reduce(nt=nt), a2.reduce(nt=nt)))
+    c0 = {"sample": "%d ciphertexts" % n0, "encrypt_ciphertexts_per_s": n0 / t_enc,
+          "ct_add_point_adds_per_s": 2 * n0 / t_add, "reduce_point_adds_per_s": 2 * (n0 - 1) / t_red}
+    g0 = configs.get("cfg0_encrypt_add_reduce_n65536")
+    if g0:
+        c0["gpu_over_cpu"] = {"encrypt": g0["encrypt"]["ciphertexts_per_s"] / c0["encrypt_ciphertexts_per_s"],
+                              "ct_add": g0["ct_add"]["point_adds_per_s"] / c0["ct_add_point_adds_per_s"],
+                              "reduce": g0["reduce"]["point_adds_per_s"] / c0["reduce_point_adds_per_s"]}
+    res["cfg0"] = c0
+    # config 2: reduce, inclusive scan, segmented sum (lognormal, median 16)
+    n2 = 1 << 20
+    P = O.Array.from_scalars(ft.random_scalars(n2, seed, 41), nt=nt)
+    off = W.lognormal_segments(n2 >> 5, n2, seed)
+    t_r = timed(lambda: P.reduce(nt=nt))
+    t_g = timed(lambda: O.segmented_sum(off, P, nt=nt))
+    Q = P.copy()
+    t_s = timed(lambda: Q.scan(nt=nt))
+    c2 = {"sample": "2^20 points (one ciphertext half); segmented sum in 2^15 lognormal segments",
+          "reduce_point_adds_per_s": (n2 - 1) / t_r, "segmented_sum_point_adds_per_s": (n2 - (n2 >> 5)) / t_g,
+          "scan_point_adds_per_s": (n2 - 1) / t_s}
+    g2 = configs.get("cfg2_aggregation")
+    if g2:
+        c2["gpu_over_cpu"] = {"reduce": g2["reduce"]["point_adds_per_s"] / c2["reduce_point_adds_per_s"],
+                              "segmented_sum": g2["segmented_sum"]["point_adds_per_s"]
+                              / c2["segmented_sum_point_adds_per_s"],
+                              "scan": g2["scan"]["point_adds_per_s"] / c2["scan_point_adds_per_s"]}
+    res["cfg2"] = c2
+    # config 3: gather-sum of in-neighbours (both halves) and rerandomisation,
+    # on a graph of the same shape (power law 2.1, 8 edges per node)
+    nn, ne = 1 << 16, 1 << 19
+    row_off, cols, _ = W.power_law_graph(nn, ne, seed + 3)
+    S = O.Array.from_scalars(ft.random_scalars(nn, seed, 42), nt=nt)
+    t_gs = 2 * timed(lambda: O.segmented_sum(row_off, S, idx=cols, nt=nt))
+    e1, e2 = O.encrypt(H, np.zeros(nn, dtype=np.uint64), ft.random_scalars(nn, seed, 43), nt=nt)
+    t_rr = timed(lambda: O.rerandomize(e1, e2, H, ft.random_scalars(nn, seed, 44), nt=nt))
+    c3 = {"sample": "2^16 nodes, 2^19 edges", "nodes_per_s": nn / (t_gs + t_rr),
+          "gather_s": round(t_gs, 3), "rerandomize_s": round(t_rr, 3)}
+    g3 = configs.get("cfg3_sparse_step_graph")
+    if g3:
+        c3["gpu_over_cpu"] = ((1 << 22) / (g3["graph_ms_per_step"] * 1e-3)) / c3["nodes_per_s"]
+    res["cfg3"] = c3
+    return res
 
 
 def libsodium_rates(k_fixed, k_var, count=2048):
