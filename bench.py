@@ -646,7 +646,7 @@ def cpu_config_rates(H, seed, configs):
 
     res = {"cores": nt, "kind": "port"}
     # config 0: encrypt, ciphertext add, reduce
-    n0 = 1 << 15
+    n0 = 1 << 16
     m = rng.integers(0, 2, size=n0, dtype=np.uint64)
     r = ft.random_scalars(n0, seed, 40)
     box = {}
@@ -664,14 +664,14 @@ def cpu_config_rates(H, seed, configs):
                               "reduce": g0["reduce"]["point_adds_per_s"] / c0["reduce_point_adds_per_s"]}
     res["cfg0"] = c0
     # config 2: reduce, inclusive scan, segmented sum (lognormal, median 16)
-    n2 = 1 << 20
+    n2 = 1 << 21
     P = O.Array.from_scalars(ft.random_scalars(n2, seed, 41), nt=nt)
     off = W.lognormal_segments(n2 >> 5, n2, seed)
     t_r = timed(lambda: P.reduce(nt=nt))
     t_g = timed(lambda: O.segmented_sum(off, P, nt=nt))
     Q = P.copy()
     t_s = timed(lambda: Q.scan(nt=nt))
-    c2 = {"sample": "2^20 points (one ciphertext half); segmented sum in 2^15 lognormal segments",
+    c2 = {"sample": "2^21 points (one ciphertext half); segmented sum in 2^16 lognormal segments",
           "reduce_point_adds_per_s": (n2 - 1) / t_r, "segmented_sum_point_adds_per_s": (n2 - (n2 >> 5)) / t_g,
           "scan_point_adds_per_s": (n2 - 1) / t_s}
     g2 = configs.get("cfg2_aggregation")
@@ -683,13 +683,13 @@ def cpu_config_rates(H, seed, configs):
     res["cfg2"] = c2
     # config 3: gather-sum of in-neighbours (both halves) and rerandomisation,
     # on a graph of the same shape (power law 2.1, 8 edges per node)
-    nn, ne = 1 << 16, 1 << 19
+    nn, ne = 1 << 17, 1 << 20
     row_off, cols, _ = W.power_law_graph(nn, ne, seed + 3)
     S = O.Array.from_scalars(ft.random_scalars(nn, seed, 42), nt=nt)
     t_gs = 2 * timed(lambda: O.segmented_sum(row_off, S, idx=cols, nt=nt))
     e1, e2 = O.encrypt(H, np.zeros(nn, dtype=np.uint64), ft.random_scalars(nn, seed, 43), nt=nt)
     t_rr = timed(lambda: O.rerandomize(e1, e2, H, ft.random_scalars(nn, seed, 44), nt=nt))
-    c3 = {"sample": "2^16 nodes, 2^19 edges", "nodes_per_s": nn / (t_gs + t_rr),
+    c3 = {"sample": "2^17 nodes, 2^20 edges", "nodes_per_s": nn / (t_gs + t_rr),
           "gather_s": round(t_gs, 3), "rerandomize_s": round(t_rr, 3)}
     g3 = configs.get("cfg3_sparse_step_graph")
     if g3:
