@@ -117,10 +117,12 @@ def run(args):
     schedules = None
     if not args.no_autotune:
         t0 = time.time()
-        table = ft.autotune(3)
+        tab = ft.autotune(3)
+        lanes = ft.ct_lanes_table()
+        table = {c: tab[c] + (lanes[c],) for c in tab}
         schedules = {"picked_by": "ftx_autotune on this GPU", "seconds": round(time.time() - t0, 2),
                      "classes": {"n<2^14": table[0], "n<2^18": table[1], "n<2^22": table[2], "n>=2^22": table[3]},
-                     "fields": "(seg1, scan: log2 elements per lane; segu: log2 radix of the segmented-sum run tree; reduce: max blocks; window width of ct x u32 and of variable-base)"}
+                     "fields": "(seg1, scan: log2 elements per lane; segu: log2 radix of the segmented-sum run tree; reduce: max blocks; window width of ct x u32 and of variable-base; lanes per ciphertext of an encryption)"}
         log("[bench] autotuned aggregation schedules in %.1fs: %s" % (schedules["seconds"], table))
 
     log("[bench] generating %d x 4 inputs" % n)

@@ -214,10 +214,11 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
  * segments spanning several level-1 chunks; reduce_blocks (1..4096) bounds
  * the first reduce pass; u32_window
  * (ciphertext x u32) and varbase_window (ft_mul, ft_scale) are 4 (radix-16
- * digits, 8-entry table) or 3 (radix 8, 4 entries).  Results are the same
+ * digits, 8-entry table) or 3 (radix 8, 4 entries); lanes per ciphertext of
+ * an encryption (ftx_tune_get_ct_lanes) is 1 or 2.  Results are the same
  * under every schedule.
  * ftx_autotune times every candidate for classes 0..max_class on the device
- * (class 3 at n = 2^24, about 2.6 GB while it runs), keeps the fastest and
+ * (class 3 at n = 2^24, about 3.8 GB while it runs), keeps the fastest and
  * writes the table to path unless it is NULL; ft_crypto_init loads the file
  * named by the FTX_TUNE_FILE environment variable when set. */
 ft_error ftx_autotune(int max_class, const char *path);
@@ -227,6 +228,12 @@ ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *redu
                       int *varbase_window);
 ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window,
                       int varbase_window);
+/* Lanes per ciphertext of an encryption or rerandomisation in a size class:
+ * 1, or 2 (C1 and C2 on separate lanes: twice the waves of a small batch).
+ * A field element's limbs always stay in one lane.  Tuned by ftx_autotune,
+ * saved and loaded with the table (eighth column). */
+ft_error ftx_tune_get_ct_lanes(int size_class, int *lanes);
+ft_error ftx_tune_set_ct_lanes(int size_class, int lanes);
 int ftx_tune_size_class(size_t n);
 
 /* ---------------------------------------------------------- inputs */

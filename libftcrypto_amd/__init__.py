@@ -121,6 +121,8 @@ _SIGS = {
     "ftx_tune_get": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "ftx_tune_set": (i32, [i32, i32, i32, i32, i32, i32, i32]),
     "ftx_tune_size_class": (i32, [sz]),
+    "ftx_tune_get_ct_lanes": (i32, [i32, vp]),
+    "ftx_tune_set_ct_lanes": (i32, [i32, i32]),
     "ftx_random_scalars": (None, [vp, sz, u64, u64]),
     "ftx_scalar_reduce64": (None, [vp, vp, sz]),
 }
@@ -473,6 +475,20 @@ def tune_table():
 def tune_set(size_class, seg1, segu, scan, reduce_blocks, u32_window=3, varbase_window=3):
     check(lib().ftx_tune_set(size_class, seg1, segu, scan, reduce_blocks, u32_window, varbase_window),
           "ftx_tune_set")
+
+
+def ct_lanes_table():
+    """{class: lanes per ciphertext of an encryption (1 or 2)}"""
+    out = {}
+    for c in range(4):
+        v = ctypes.c_int()
+        check(lib().ftx_tune_get_ct_lanes(c, ctypes.byref(v)), "ftx_tune_get_ct_lanes")
+        out[c] = v.value
+    return out
+
+
+def tune_set_ct_lanes(size_class, lanes):
+    check(lib().ftx_tune_set_ct_lanes(size_class, lanes), "ftx_tune_set_ct_lanes")
 
 
 def tune_load(path):

@@ -21,6 +21,9 @@ hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_b
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
                           hipStream_t s);
+hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
+                            const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
+                            hipStream_t s);
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s);
 hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long long *bad, hipStream_t s);
 unsigned reduce_grid(size_t n, unsigned max_blocks);
@@ -38,6 +41,7 @@ struct AggTune {
   int reduce;        // most blocks of 256 lanes in the first reduce pass
   int u32w;          // window width of the 32-bit-multiplier kernels (3 or 4)
   int vbw;           // window width of the 256-bit variable-base kernel (3 or 4)
+  int encl;          // lanes per ciphertext of an encryption (1, or 2: C1 and C2 apart)
 };
 constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
 constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed run-tree radix log2 range

@@ -368,30 +368,39 @@ hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_b
   FT_LAUNCH_CHECK();
 }
 
-hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
-                          const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
-                          hipStream_t s) {
+// lanes: 1 (one lane per ciphertext) or 2 (C1 and C2 on separate lanes)
+hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
+                            const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
+                            hipStream_t s) {
   if (!n) return hipSuccess;
-  if (n >= ENC_WIDE_MIN) {
+  if (lanes == 1 && n >= ENC_WIDE_MIN) {
     const dim3 grid(grid_for(n, ETPB));
     if (accumulate)
       hipLaunchKernelGGL(k_encrypt_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
     else
       hipLaunchKernelGGL(k_encrypt_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
-  } else if (n >= ENC_SPLIT_MAX) {
+  } else if (lanes == 1) {
     const dim3 grid(grid_for(n));
     if (accumulate)
       hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
     else
       hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
-  } else {
+  } else if (lanes == 2) {
     const dim3 grid(grid_for(2 * n));
     if (accumulate)
       hipLaunchKernelGGL(k_encrypt_split<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
     else
       hipLaunchKernelGGL(k_encrypt_split<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  } else {
+    return hipErrorInvalidValue;
   }
   FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
+                          const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
+                          hipStream_t s) {
+  return launch_encrypt_l(c1, c2, btab, htab, m, nd, r, n, accumulate, tune_for(n).encl, s);
 }
 
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s) {

@@ -15,9 +15,9 @@ constexpr int TPB = 256;
 // to fill 256 CUs, and 256-lane blocks are used.
 constexpr int ETPB = 512;
 constexpr size_t ENC_WIDE_MIN = (size_t)ETPB * 512;
-// below 2^17 ciphertexts one lane per ciphertext leaves at most two waves per
-// SIMD; two lanes per ciphertext (one per half) double that
-constexpr size_t ENC_SPLIT_MAX = (size_t)1 << 17;
+// Small batches leave few waves per SIMD with one lane per ciphertext; two
+// lanes per ciphertext (one per half) double that.  Which of the two a size
+// class uses is a tuned schedule (AggTune::encl).
 
 // ------------------------------------------------------------ digits
 //

@@ -377,10 +377,11 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
   const unsigned grid = (unsigned)((start + TPB - 1) / TPB);
   if (e == hipSuccess && start > 0) {
     if (kind == K_ENCRYPT) {
-      if (start >= ENC_WIDE_MIN)
+      const bool split = tune_for(start).encl == 2;
+      if (!split && start >= ENC_WIDE_MIN)
         hipLaunchKernelGGL(kb_encrypt_wide, dim3((unsigned)((start + ETPB - 1) / ETPB)), dim3(ETPB), 0, sl.st,
                            ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
-      else if (start >= ENC_SPLIT_MAX)
+      else if (!split)
         hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
                            key_table(S->key), nd);
       else

@@ -12,6 +12,11 @@
 //           8-entry table, 9 windows) or 3 (radix 8, 4-entry table, 11)
 //   vbw     window width of the 256-bit variable-base mult: 4 (radix 16,
 //           64 windows) or 3 (radix 8, 86 windows)
+//   encl    lanes per ciphertext of an encryption or rerandomisation: 1, or
+//           2 (C1 on one lane, C2 on the next; twice the waves of a small
+//           batch, each lane with one comb pass).  The nine limbs of a field
+//           element always stay in one lane: splitting them would exchange
+//           partial column sums between lanes inside every multiply.
 // Every schedule gives the same group element (point addition is
 // associative and commutative), so the choice only moves time.  Scratch
 // sizes are those of the schedule needing the most, so buffers sized by
@@ -42,13 +47,13 @@ namespace {
 
 // Defaults per size class: what the tuner picks on MI355X (elements per lane
 // 2^4 below 2^22 and 2^6 above, radix-2 then radix-4 run trees, radix-16
-// windows for the variable base below 2^18), so an untuned process is close
-// to a tuned one.
+// windows for the variable base below 2^18, two lanes per ciphertext below
+// 2^22), so an untuned process is close to a tuned one.
 constexpr AggTune DEFAULT_TUNE[TUNE_CLASSES] = {
-    {4, 1, 4, 2048, 3, 4}, {4, 1, 4, 512, 3, 4}, {4, 1, 4, 512, 3, 3}, {6, 2, 4, 2048, 3, 3}};
+    {4, 1, 4, 2048, 3, 4, 2}, {4, 1, 4, 512, 3, 4, 2}, {4, 1, 4, 512, 3, 3, 2}, {6, 2, 4, 2048, 3, 3, 1}};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
-    T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES];
+    T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES], T_encl[TUNE_CLASSES];
 std::atomic<bool> T_ready{false};
 
 void tune_defaults() {
@@ -59,6 +64,7 @@ void tune_defaults() {
     T_reduce[c] = DEFAULT_TUNE[c].reduce;
     T_u32w[c] = DEFAULT_TUNE[c].u32w;
     T_vbw[c] = DEFAULT_TUNE[c].vbw;
+    T_encl[c] = DEFAULT_TUNE[c].encl;
   }
   T_ready = true;
 }
@@ -67,7 +73,7 @@ bool valid(const AggTune &t) {
   auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
   return ch_ok(t.seg1) && t.segu >= TREE_LR_MIN && t.segu <= TREE_LR_MAX && ch_ok(t.scan) && t.reduce >= 1 &&
          (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4) &&
-         (t.vbw == 3 || t.vbw == 4);
+         (t.vbw == 3 || t.vbw == 4) && (t.encl == 1 || t.encl == 2);
 }
 
 void tune_store(int c, const AggTune &t) {
@@ -77,6 +83,7 @@ void tune_store(int c, const AggTune &t) {
   T_reduce[c] = t.reduce;
   T_u32w[c] = t.u32w;
   T_vbw[c] = t.vbw;
+  T_encl[c] = t.encl;
 }
 
 }  // namespace
@@ -91,7 +98,7 @@ int tune_class(size_t n) {
 AggTune tune_for(size_t n) {
   if (!T_ready) tune_defaults();
   const int c = tune_class(n);
-  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c], T_vbw[c]};
+  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c], T_vbw[c], T_encl[c]};
 }
 
 }  // namespace ft
@@ -99,6 +106,10 @@ AggTune tune_for(size_t n) {
 using namespace ft;
 
 namespace {
+
+// encryptions are timed at the class's own size up to 2^22 (about 20 ms per
+// launch there): the lane split that wins at 2^18 need not win at 2^22
+constexpr size_t ENC_TUNE_MAX = (size_t)1 << 22;
 
 #define TCK(x)                                              \
   do {                                                      \
@@ -167,25 +178,41 @@ ft_error ftx_tune_get(int size_class, int *seg1, int *segu, int *scan, int *redu
 ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_blocks, int u32_window,
                       int varbase_window) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
-  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window, varbase_window};
-  if (!valid(t)) return FT_ERR_BAD_SIZE;
   if (!T_ready) tune_defaults();
+  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window, varbase_window, T_encl[size_class]};
+  if (!valid(t)) return FT_ERR_BAD_SIZE;
   tune_store(size_class, t);
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_tune_get_ct_lanes(int size_class, int *lanes) {
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  if (!T_ready) tune_defaults();
+  *lanes = T_encl[size_class];
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_tune_set_ct_lanes(int size_class, int lanes) {
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  if (lanes != 1 && lanes != 2) return FT_ERR_BAD_SIZE;
+  if (!T_ready) tune_defaults();
+  T_encl[size_class] = lanes;
   return FT_ERR_NO_ERROR;
 }
 
 int ftx_tune_size_class(size_t n) { return tune_class(n); }
 
 // Text format: a comment line, then one line per class:
-// "class seg1 segu scan reduce u32w vbw".
+// "class seg1 segu scan reduce u32w vbw encl" (files without the last column,
+// written before it existed, load with the default encl of each class).
 ft_error ftx_tune_save(const char *path) {
   FILE *f = std::fopen(path, "w");
   if (!f) return FT_ERR_INTERNAL_LIBRARY_ERROR;
   if (!T_ready) tune_defaults();
-  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w vbw\n");
+  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w vbw encl\n");
   for (int c = 0; c < TUNE_CLASSES; ++c)
-    std::fprintf(f, "%d %d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c], (int)T_reduce[c],
-                 (int)T_u32w[c], (int)T_vbw[c]);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c],
+                 (int)T_reduce[c], (int)T_u32w[c], (int)T_vbw[c], (int)T_encl[c]);
   return std::fclose(f) == 0 ? FT_ERR_NO_ERROR : FT_ERR_INTERNAL_LIBRARY_ERROR;
 }
 
@@ -198,13 +225,13 @@ ft_error ftx_tune_load(const char *path) {
   ft_error err = FT_ERR_NO_ERROR;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#' || line[0] == '\n') continue;
-    int c, a, b, d, r, w, v;
-    if (std::sscanf(line, "%d %d %d %d %d %d %d", &c, &a, &b, &d, &r, &w, &v) != 7 || c < 0 ||
-        c >= TUNE_CLASSES) {
+    int c, a, b, d, r, w, v, l;
+    const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d", &c, &a, &b, &d, &r, &w, &v, &l);
+    if ((got != 7 && got != 8) || c < 0 || c >= TUNE_CLASSES) {
       err = FT_ERR_BAD_SIZE;
       break;
     }
-    t[c] = AggTune{a, b, d, r, w, v};
+    t[c] = AggTune{a, b, d, r, w, v, got == 8 ? l : DEFAULT_TUNE[c].encl};
     if (!valid(t[c])) { err = FT_ERR_BAD_SIZE; break; }
     seen[c] = true;
   }
@@ -219,8 +246,8 @@ ft_error ftx_tune_load(const char *path) {
 
 // Measure every candidate schedule of every size class (classes up to
 // max_class, 0..3) and keep the fastest; write the table to path when it is
-// not NULL.  Needs ft_crypto_init; allocates up to ~2.6 GB of device memory
-// while it runs (class 3 is timed at n = 2^24).
+// not NULL.  Needs ft_crypto_init; allocates up to ~3.8 GB of device memory
+// while it runs (class 3 is timed at n = 2^24, its encryptions at 2^22).
 ft_error ftx_autotune(int max_class, const char *path) {
   if (max_class < 0 || max_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
   if (!base_table()) return FT_ERR_INSUFFICIENT_DEVICE_OR_DRIVER;
@@ -266,6 +293,14 @@ ft_error ftx_autotune(int max_class, const char *path) {
   TCK(hipMemcpyAsync(cpy.p, pts.p, nmax * 128, hipMemcpyDeviceToDevice, st.s));
   const PtView C{(uint32_t *)cpy.p, nmax};
   const PtView S{(uint32_t *)scr.p, scr_pts};
+  // encryption outputs and 32-byte r scalars for at most ENC_TUNE_MAX ciphertexts
+  const size_t nenc = std::min(nmax, ENC_TUNE_MAX);
+  DevMem e1, e2, er;
+  TCK(hipMalloc(&e1.p, nenc * 128));
+  TCK(hipMalloc(&e2.p, nenc * 128));
+  TCK(hipMalloc(&er.p, nenc * 32));
+  TCK(hipMemcpyAsync(er.p, pts.p, nenc * 32, hipMemcpyDeviceToDevice, st.s));
+  const PtView E1{(uint32_t *)e1.p, nenc}, E2{(uint32_t *)e2.p, nenc};
   for (int c = 0; c <= max_class; ++c) {
     const size_t n = (size_t)1 << LOGN[c];
     const int reps = c < 2 ? 10 : 3;
@@ -328,6 +363,19 @@ ft_error ftx_autotune(int max_class, const char *path) {
       });
       if (e != FT_ERR_NO_ERROR) return e;
       if (t < bt) { bt = t; best.vbw = w; }
+    }
+    // encryption with one lane per ciphertext against two (C1 and C2 on
+    // separate lanes); B's comb table for both keys and arbitrary r (any
+    // scalars time the same: the combs are constant-time)
+    bt = INFINITY;
+    for (int l = 1; l <= 2; ++l) {
+      const size_t ne = std::min(n, ENC_TUNE_MAX);
+      ft_error e = time_ms(&t, ev, st.s, c < 2 ? 3 : 1, [&] {
+        return launch_encrypt_l(E1, E2, base_table(), base_table(), (const uint64_t *)sc.p, 1,
+                                (const uint32_t *)er.p, ne, false, l, st.s);
+      });
+      if (e != FT_ERR_NO_ERROR) return e;
+      if (t < bt) { bt = t; best.encl = l; }
     }
     tune_store(c, best);
   }

@@ -190,3 +190,33 @@ def test_autotune_keeps_results(tmp_path):
     g.scan()
     o.scan()
     same(g, o)
+
+
+@pytest.mark.parametrize("n", [1000, 300000])
+def test_every_ct_lanes(n):
+    """Encryption and rerandomisation with one lane per ciphertext and with
+    two (C1 and C2 on separate lanes) against the oracle; at 300,000 the
+    one-lane form is the 512-lane-block kernel"""
+    c = ft.size_class(n)
+    before = ft.ct_lanes_table()[c]
+    x = ft.random_scalars(1, 4800, 0)
+    pk = ft.GeArray.from_scalars(x)
+    H = bytes(pk.to_bytes_folded())
+    key = ft.Key(pk)
+    rng = np.random.default_rng(n)
+    m = rng.integers(0, 256, size=n, dtype=np.uint64)
+    r, rho = ft.random_scalars(n, 4800 + n, 1), ft.random_scalars(n, 4800 + n, 2)
+    o1, o2 = O.encrypt(H, m, r)
+    q1, q2 = o1.copy(), o2.copy()
+    O.rerandomize(q1, q2, H, rho)
+    try:
+        for lanes in (1, 2):
+            ft.tune_set_ct_lanes(c, lanes)
+            g1, g2 = ft.encrypt(key, m, r, msg_bits=8)
+            same(g1, o1)
+            same(g2, o2)
+            ft.rerandomize(g1, g2, key, rho)
+            same(g1, q1)
+            same(g2, q2)
+    finally:
+        ft.tune_set_ct_lanes(c, before)

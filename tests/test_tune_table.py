@@ -58,3 +58,29 @@ def test_load_rejects_malformed(tmp_path, text):
         ft.tune_load(str(p))
     assert e.value.code == ft.FT_ERR_BAD_SIZE
     assert ft.tune_table() == before
+
+
+def test_ct_lanes_set_get_save_load(tmp_path):
+    before = ft.ct_lanes_table()
+    try:
+        ft.tune_set_ct_lanes(0, 1)
+        ft.tune_set_ct_lanes(3, 2)
+        assert ft.ct_lanes_table()[0] == 1 and ft.ct_lanes_table()[3] == 2
+        p = str(tmp_path / "t.txt")
+        ft.tune_save(p)
+        assert open(p).read().splitlines()[1].split()[-1] == "1"
+        ft.tune_set_ct_lanes(0, 2)
+        ft.tune_load(p)
+        assert ft.ct_lanes_table()[0] == 1 and ft.ct_lanes_table()[3] == 2
+        for bad in (0, 3):
+            with pytest.raises(ft.FtError) as e:
+                ft.tune_set_ct_lanes(1, bad)
+            assert e.value.code == ft.FT_ERR_BAD_SIZE
+        # a file without the lanes column (the older format) loads with the defaults
+        q = tmp_path / "old.txt"
+        q.write_text("".join("%d 4 1 4 512 3 4\n" % c for c in range(4)))
+        ft.tune_load(str(q))
+        assert ft.ct_lanes_table() == {0: 2, 1: 2, 2: 2, 3: 1}
+    finally:
+        for c, v in before.items():
+            ft.tune_set_ct_lanes(c, v)
