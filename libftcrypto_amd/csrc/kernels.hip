@@ -171,6 +171,25 @@ __global__ void __launch_bounds__(TPB) k_encrypt_split(PtView c1, PtView c2, con
   encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
 }
 
+// the same in 512-lane blocks at four waves per SIMD: two blocks (two comb
+// table pairs, 112 KB of LDS) fit a CU, so a launch running beside another
+// one (the two arrays of a ciphertext batch on two streams) reaches four
+// waves per SIMD where 256-lane blocks stop at two
+template <bool ACC>
+__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_encrypt_split_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
+                     const uint32_t *r, size_t n) {
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const size_t t = blockIdx.x * (size_t)ETPB + threadIdx.x;
+  const size_t i = t >> 1;
+  if (i >= n) return;
+  encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
+}
+
 template <bool ACC>
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_encrypt_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
@@ -385,6 +404,12 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
       hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
     else
       hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+  } else if (lanes == 2 && 2 * n >= ENC_SPLIT_WIDE_MIN) {
+    const dim3 grid(grid_for(2 * n, ETPB));
+    if (accumulate)
+      hipLaunchKernelGGL(k_encrypt_split_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+    else
+      hipLaunchKernelGGL(k_encrypt_split_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
   } else if (lanes == 2) {
     const dim3 grid(grid_for(2 * n));
     if (accumulate)

@@ -17,7 +17,9 @@ constexpr int ETPB = 512;
 constexpr size_t ENC_WIDE_MIN = (size_t)ETPB * 512;
 // Small batches leave few waves per SIMD with one lane per ciphertext; two
 // lanes per ciphertext (one per half) double that.  Which of the two a size
-// class uses is a tuned schedule (AggTune::encl).
+// class uses is a tuned schedule (AggTune::encl).  Two-lane launches of at
+// least one 512-lane block per CU use 512-lane blocks.
+constexpr size_t ENC_SPLIT_WIDE_MIN = (size_t)ETPB * 256;
 
 // ------------------------------------------------------------ digits
 //

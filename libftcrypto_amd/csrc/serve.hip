@@ -81,19 +81,32 @@ __global__ void __launch_bounds__(TPB) kb_encrypt(const BItem *items, int cnt, u
 }
 
 // small batches: two lanes per ciphertext (as k_encrypt_split)
-__global__ void __launch_bounds__(TPB) kb_encrypt_split(const BItem *items, int cnt, uint64_t total,
-                                                        const uint32_t *btab, const uint32_t *htab, int nd) {
+template <int BS>
+__device__ __forceinline__ void kb_encrypt_split_body(const BItem *items, int cnt, uint64_t total,
+                                                      const uint32_t *btab, const uint32_t *htab, int nd) {
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
   lds_load_table(th, htab);
   __syncthreads();
-  const uint64_t t2 = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  const uint64_t t2 = blockIdx.x * (uint64_t)BS + threadIdx.x;
   const uint64_t t = t2 >> 1;
   if (t >= total) return;
   const BItem &it = items[find_item(items, cnt, t)];
   const uint64_t i = t - it.start;
   encrypt_half<false>(it.a1, it.a2, i, (uint32_t)(t2 & 1), tb, th, it.m[i], nd, it.r + i * 8);
+}
+
+__global__ void __launch_bounds__(TPB) kb_encrypt_split(const BItem *items, int cnt, uint64_t total,
+                                                        const uint32_t *btab, const uint32_t *htab, int nd) {
+  kb_encrypt_split_body<TPB>(items, cnt, total, btab, htab, nd);
+}
+
+// two lanes per ciphertext in 512-lane blocks at four waves per SIMD
+__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+kb_encrypt_split_wide(const BItem *items, int cnt, uint64_t total, const uint32_t *btab, const uint32_t *htab,
+                      int nd) {
+  kb_encrypt_split_body<ETPB>(items, cnt, total, btab, htab, nd);
 }
 
 // large batches: 512-lane blocks at four waves per SIMD (as k_encrypt_wide)
@@ -384,6 +397,9 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
       else if (!split)
         hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
                            key_table(S->key), nd);
+      else if (2 * start >= ENC_SPLIT_WIDE_MIN)
+        hipLaunchKernelGGL(kb_encrypt_split_wide, dim3((unsigned)((2 * start + ETPB - 1) / ETPB)), dim3(ETPB), 0,
+                           sl.st, ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
       else
         hipLaunchKernelGGL(kb_encrypt_split, dim3((unsigned)((2 * start + TPB - 1) / TPB)), dim3(TPB), 0, sl.st,
                            ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);

@@ -47,10 +47,10 @@ namespace {
 
 // Defaults per size class: what the tuner picks on MI355X (elements per lane
 // 2^4 below 2^22 and 2^6 above, radix-2 then radix-4 run trees, radix-16
-// windows for the variable base below 2^18, two lanes per ciphertext below
-// 2^22), so an untuned process is close to a tuned one.
+// windows for the variable base below 2^18, two lanes per ciphertext of an
+// encryption), so an untuned process is close to a tuned one.
 constexpr AggTune DEFAULT_TUNE[TUNE_CLASSES] = {
-    {4, 1, 4, 2048, 3, 4, 2}, {4, 1, 4, 512, 3, 4, 2}, {4, 1, 4, 512, 3, 3, 2}, {6, 2, 4, 2048, 3, 3, 1}};
+    {4, 1, 4, 2048, 3, 4, 2}, {4, 1, 4, 512, 3, 4, 2}, {4, 1, 4, 512, 3, 3, 2}, {6, 2, 4, 2048, 3, 3, 2}};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
     T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES], T_encl[TUNE_CLASSES];

@@ -100,7 +100,7 @@ def test_segsum_requests_batched(key):
     together: the scheduler runs them as jobs of shared persistent launches
     (fewer batches than requests), and each result equals the one-shot
     ftx_segmented_sum of the same ciphertexts."""
-    k, _ = key
+    k, H = key
     rng = np.random.default_rng(2024)
     srv = SV.Server(k, budget_bytes=1 << 30, n_streams=2)
     srcs = []
@@ -115,7 +115,9 @@ def test_segsum_requests_batched(key):
     # two large encryptions occupy both streams, so the segmented sums
     # submitted next are all queued when a stream frees up
     big = 1 << 20
-    busy = [srv.encrypt(np.zeros(big, dtype=np.uint64), ft.random_scalars(big, 77, j), 1)[0] for j in range(2)]
+    big_r = [ft.random_scalars(big, 77, j) for j in range(2)]
+    busy_q = [srv.encrypt(np.zeros(big, dtype=np.uint64), big_r[j], 1) for j in range(2)]
+    busy = [q for q, _ in busy_q]
     before = srv.stats()["batches"]
     reqs = [srv.segsum(h, off) for h, off, _ in srcs]
     for q in busy:
@@ -128,4 +130,9 @@ def test_segsum_requests_batched(key):
         a, b = srv.read(h, len(off) - 1)
         assert np.array_equal(a, ft.segmented_sum(off, c1).to_bytes_folded())
         assert np.array_equal(b, ft.segmented_sum(off, c2).to_bytes_folded())
+    # a large served encryption (two lanes per ciphertext, 512-lane blocks
+    # under the default schedule) equals the oracle's
+    a, b = srv.read(busy_q[0][1], big)
+    o1, o2 = O.encrypt(H, np.zeros(big, dtype=np.uint64), big_r[0])
+    assert np.array_equal(a, o1.to_bytes()) and np.array_equal(b, o2.to_bytes())
     srv.close()

@@ -80,7 +80,7 @@ def test_ct_lanes_set_get_save_load(tmp_path):
         q = tmp_path / "old.txt"
         q.write_text("".join("%d 4 1 4 512 3 4\n" % c for c in range(4)))
         ft.tune_load(str(q))
-        assert ft.ct_lanes_table() == {0: 2, 1: 2, 2: 2, 3: 1}
+        assert ft.ct_lanes_table() == {0: 2, 1: 2, 2: 2, 3: 2}
     finally:
         for c, v in before.items():
             ft.tune_set_ct_lanes(c, v)
