@@ -136,3 +136,29 @@ def test_segsum_requests_batched(key):
     o1, o2 = O.encrypt(H, np.zeros(big, dtype=np.uint64), big_r[0])
     assert np.array_equal(a, o1.to_bytes()) and np.array_equal(b, o2.to_bytes())
     srv.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_served_encrypt_both_lane_schedules(key, lanes):
+    """Served encryptions of 1,000 and 300,000 ciphertexts (one batch each)
+    with one lane per ciphertext (256- and 512-lane-block batch kernels) and
+    with two, against the oracle"""
+    k, H = key
+    before = ft.ct_lanes_table()
+    try:
+        for c in range(4):
+            ft.tune_set_ct_lanes(c, lanes)
+        srv = SV.Server(k, budget_bytes=1 << 30, n_streams=2)
+        rng = np.random.default_rng(lanes)
+        for n in (1000, 300000):
+            m = rng.integers(0, 256, size=n, dtype=np.uint64)
+            r = ft.random_scalars(n, 930 + lanes, n)
+            q, h = srv.encrypt(m, r, 8)
+            srv.wait(q)
+            a, b = srv.read(h, n)
+            o1, o2 = O.encrypt(H, m, r)
+            assert np.array_equal(a, o1.to_bytes()) and np.array_equal(b, o2.to_bytes()), n
+        srv.close()
+    finally:
+        for c, v in before.items():
+            ft.tune_set_ct_lanes(c, v)
