@@ -63,9 +63,9 @@ def enc(arr, n=None):
 
 def test_batch_mixed_jobs_match_oracle():
     """Two CSR segmented sums (one with a 3000-element segment spanning many
-    chunks), a uniform one, two reduces, an inclusive scan above the
-    block-scan limit (its first level runs in the persistent launch) and a
-    short exclusive scan, in one call."""
+    chunks), a uniform one, two reduces, an inclusive and an exclusive scan
+    above the block-scan limit (their first levels run in the persistent
+    launch) and a short exclusive scan, in one call."""
     rng = np.random.default_rng(11)
     sA, A = points(9000, 1)
     sB, B = points(70001, 2)
@@ -80,6 +80,7 @@ def test_batch_mixed_jobs_match_oracle():
     uni = ft.GeArray.alloc((5000 + 36) // 37)
     redA, redB = ft.GeArray.alloc(1), ft.GeArray.alloc(1)
     scanB = B.get_range(0, len(B), 1)
+    xscanB = B.get_range(0, 66000, 1)
     xscanC = C.get_range(0, 3000, 1)
     jobs = [
         ft.agg_job(ft.AGG_SEGSUM, segA, A, dA.p, 900, int(offA[-1])),
@@ -89,6 +90,7 @@ def test_batch_mixed_jobs_match_oracle():
         ft.agg_job(ft.AGG_SEGSUM, uni, C, None, (5000 + 36) // 37, 5000, 37),
         ft.agg_job(ft.AGG_REDUCE, redB, B),
         ft.agg_job(ft.AGG_XSCAN, xscanC),
+        ft.agg_job(ft.AGG_XSCAN, xscanB),
     ]
     ft.aggregate_batch(jobs)
     assert np.array_equal(enc(segA), O.segmented_sum(offA, oA).to_bytes())
@@ -103,6 +105,9 @@ def test_batch_mixed_jobs_match_oracle():
     o = O.Array.from_scalars(sC[:3000])
     o.scan(exclusive=True)
     assert np.array_equal(enc(xscanC), o.to_bytes())
+    o = O.Array.from_scalars(sB[:66000])
+    o.scan(exclusive=True)
+    assert np.array_equal(enc(xscanB), o.to_bytes())
 
 
 def test_batch_full_size_matches_single_jobs():
