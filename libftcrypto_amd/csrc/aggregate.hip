@@ -633,7 +633,7 @@ __device__ __forceinline__ void reduce_tile(PtView part, size_t slot, PtView v, 
   PW nw;
   ge_p3_0(acc);
   const size_t end0 = base + ((size_t)64 << RCH);
-  const size_t end = end0 < n + 63 ? end0 : n + 63;    // lanes stop at their last i < n
+  const size_t end = end0 < n ? end0 : n;
   size_t i = base;
   if (i < n) { load_p3(acc, v, i); i += 64; }
   if (i < n && i < end) load_p3(q, v, i);
@@ -683,10 +683,11 @@ k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ct
   }
 }
 
-// job owning global index g of the chunk (seg = false) or segment ranges;
-// ranges start at multiples of 64, so the answer is the same for a wave
+// segments of a job with CSR offsets (uniform segments are never empty)
 __device__ __forceinline__ size_t csr_segs(const BJob &b) { return b.off ? (size_t)b.nseg : 0; }
 
+// job owning global index g of the chunk (seg = false) or segment ranges;
+// ranges start at multiples of 64, so the answer is the same for a wave
 __device__ __forceinline__ int batch_job_of(const BBatch &B, size_t g, bool seg) {
   int j = -1;
 #pragma unroll 1
