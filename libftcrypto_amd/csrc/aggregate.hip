@@ -590,8 +590,9 @@ hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStre
 // Several reduce / segmented-sum / scan jobs in ONE persistent launch.  The
 // element work of every job is cut into wave tiles: 64 level-1 chunks of a
 // segmented sum (a scan's first level is a segmented sum with uniform
-// segments of 2^ch), or 64 lanes x 2^RCH elements of a reduce, which each
-// lane walks with a stride of 64 (coalesced) and leaves as one partial.  The
+// segments of 2^ch), or 64 lanes x 2^r elements of a reduce, which each
+// lane walks with a stride of 64 (coalesced) and leaves as one partial (r
+// from 0 to RCH_MAX, so that a reduce has about REDUCE_TILES tiles).  The
 // grid holds as many waves as the GPU keeps resident; each wave takes the
 // next tile from an atomic counter until none is left, so short and long
 // jobs, and tiles that flush more or fewer segments, balance over the whole
@@ -600,7 +601,15 @@ hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStre
 // sum of each reduce's partials, and the upper scan levels with the second
 // walk.
 
-constexpr int RCH = 8;                     // reduce: 2^RCH elements per lane and tile
+constexpr int RCH_MAX = 8;                 // reduce: at most 2^8 elements per lane and tile
+constexpr size_t REDUCE_TILES = 4096;      // twice the waves the GPU keeps resident
+
+// log2 elements per lane of a reduce tile for n elements
+static int reduce_rch(size_t n) {
+  int r = 0;
+  while (r < RCH_MAX && (n >> (6 + r)) > REDUCE_TILES) ++r;
+  return r;
+}
 
 // One job as the kernels see it: 88 bytes, so that a table of AGG_BATCH_MAX
 // jobs passes by value as a kernel argument (well under the 4 KiB limit) and
@@ -609,7 +618,7 @@ constexpr int RCH = 8;                     // reduce: 2^RCH elements per lane an
 struct BJob {
   PtView out, src;
   const int64_t *off;                      // CSR offsets; null: uniform segments of len
-  int64_t len;
+  int64_t len;                             // reduce: log2 elements per lane of its tiles
   uint64_t nseg, n;
   uint32_t tb;                             // first global tile
   uint32_t cb, cn;                         // chunk-kind jobs: first global chunk (multiple of 64), chunks
@@ -626,13 +635,13 @@ struct BBatch {
 };
 static_assert(sizeof(BBatch) + 48 <= 3072, "job table must stay well inside the kernel-argument limit");
 
-// reduce tile: lane walks elements base, base + 64, ... (2^RCH of them, below
+// reduce tile: lane walks elements base, base + 64, ... (2^rch of them, below
 // n) with the next point's words in flight during each add
-__device__ __forceinline__ void reduce_tile(PtView part, size_t slot, PtView v, size_t n, size_t base) {
+__device__ __forceinline__ void reduce_tile(PtView part, size_t slot, PtView v, size_t n, size_t base, int rch) {
   ge_p3 acc, q;
   PW nw;
   ge_p3_0(acc);
-  const size_t end0 = base + ((size_t)64 << RCH);
+  const size_t end0 = base + ((size_t)64 << rch);
   const size_t end = end0 < n ? end0 : n;
   size_t i = base;
   if (i < n) { load_p3(acc, v, i); i += 64; }
@@ -667,7 +676,9 @@ k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ct
       if (B.job[q].tb <= tile) j = q;
     const size_t lt = (size_t)(tile - B.job[j].tb);
     if (B.job[j].kind == 1) {
-      reduce_tile(part, (size_t)B.job[j].pb + lt * 64 + lane, B.job[j].src, B.job[j].n, (lt << (6 + RCH)) + lane);
+      const int rch = (int)B.job[j].len;
+      reduce_tile(part, (size_t)B.job[j].pb + lt * 64 + lane, B.job[j].src, B.job[j].n, (lt << (6 + rch)) + lane,
+                  rch);
     } else {
       const PtView r{rec.w + (size_t)B.job[j].cb * 2 * 8, rec.cap};
       int64_t *k = keys + 2 * (size_t)B.job[j].cb;
@@ -757,8 +768,10 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
       BJob &b = p.B.job[q++];
       b.tb = (uint32_t)nt;
       if (red) {
-        const size_t t = (a.n + ((size_t)64 << RCH) - 1) >> (6 + RCH);
+        const int rch = reduce_rch(a.n);
+        const size_t t = (a.n + ((size_t)64 << rch) - 1) >> (6 + rch);
         b.kind = 1;
+        b.len = rch;
         b.src = a.src;
         b.n = a.n;
         b.pb = (uint32_t)p.P;
@@ -962,7 +975,8 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
       e = launch_set_identity(a.out, 0, 1, s);
     } else {
       const BJob &b = p.B.job[q];
-      const size_t np = 64 * ((a.n + ((size_t)64 << RCH) - 1) >> (6 + RCH));
+      const int rch = (int)b.len;
+      const size_t np = 64 * ((a.n + ((size_t)64 << rch) - 1) >> (6 + rch));
       const unsigned g1 = reduce_grid(np, REDUCE_BLOCKS_MAX);
       e = launch_reduce_blocks(at(p.red_base), at(p.part_base + b.pb), np, g1, s);
       if (e == hipSuccess) e = launch_reduce_blocks(a.out, at(p.red_base), g1, 1, s);
