@@ -280,7 +280,8 @@ def adds_entry(ms, adds, bytes_per_add=None):
 
 def bench_cfg0(ft, hip, key, seed):
     """Encrypt 2 x 65,536 bits, add the two arrays, reduce both halves of the
-    sum: on one stream, and with the independent halves on two streams."""
+    sum: on one stream, with the independent halves on two streams, and
+    batched (both encryptions in one launch, both reduces in one batch)."""
     L = ft.lib()
     stream = L.ftx_default_stream()
     T = Timer(hip, stream)
@@ -330,6 +331,21 @@ def bench_cfg0(ft, hip, key, seed):
     two_ms = T(step2)
     ft.check(L.ftx_synchronize(None))
     L.ftx_stream_destroy(s2)
+    # the same step batched: A's and B's encryptions in one launch
+    # (ftx_encrypt_batch_dev), the adds, both halves' reduces as one
+    # aggregation batch (ftx_aggregate_batch_dev)
+    ejobs = (ft.EncJob * 2)(ft.enc_job(a1, a2, m, r), ft.enc_job(b1, b2, m, r))
+    rjobs = (ft.AggJob * 2)(ft.agg_job(ft.AGG_REDUCE, out1, a1), ft.agg_job(ft.AGG_REDUCE, out2, a2))
+    bscr = ft.GeArray.alloc(L.ftx_aggregate_batch_scratch_points(rjobs, 2))
+
+    def step_batched():
+        ft.check(L.ftx_encrypt_batch_dev(ejobs, 2, key.h, 1, stream))
+        ft.check(L.ftx_add_dev(a1.h, b1.h, stream))
+        ft.check(L.ftx_add_dev(a2.h, b2.h, stream))
+        ft.check(L.ftx_aggregate_batch_dev(rjobs, 2, bscr.h, stream))
+
+    batched_ms = T(step_batched)
+    enc_batched_ms = T(lambda: ft.check(L.ftx_encrypt_batch_dev(ejobs, 2, key.h, 1, stream)))
     hip.L.hipFree(m)
     hip.L.hipFree(r)
     # elementwise ciphertext add at 2^22 (bandwidth-sized: 2 x 384 B per ct)
@@ -347,7 +363,10 @@ def bench_cfg0(ft, hip, key, seed):
             "reduce": adds_entry(red_ms, 2 * (n - 1), POINT_BYTES),
             "step_ms_one_stream": round(enc_ms + add_ms + red_ms, 4),
             "step_ms_two_streams": round(two_ms, 4),
-            "ciphertexts_per_s_two_streams": 2 * n / (two_ms * 1e-3)}
+            "ciphertexts_per_s_two_streams": 2 * n / (two_ms * 1e-3),
+            "step_ms_batched": round(batched_ms, 4),
+            "encrypt_batched": {"ms": round(enc_batched_ms, 4), "ciphertexts_per_s": 2 * n / (enc_batched_ms * 1e-3)},
+            "ciphertexts_per_s_batched": 2 * n / (batched_ms * 1e-3)}
 
 
 def bench_cfg2(ft, hip, key, seed, args):

@@ -73,6 +73,21 @@ ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                          const void *d_m, const void *d_r, int msg_bits, void *stream);
 ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                              const void *d_r, void *stream);
+/* Several encryptions under one key in ONE launch (no counterpart in the
+ * reference, whose caller composes each encryption from separate calls):
+ * for every job, c1[i] = r_i B and c2[i] = m_i B + r_i H for i < len(c1),
+ * with d_m (uint64) and d_r (32-byte scalars) in device memory as for
+ * ftx_encrypt_dev; msg_bits bounds every job's messages.  Two lanes per
+ * ciphertext, the jobs' ciphertexts laid end to end, so small arrays fill
+ * the GPU together.  At most FTX_ENC_MAX_JOBS jobs; the jobs' output arrays
+ * must be pairwise distinct (FT_ERR_BAD_SIZE otherwise). */
+#define FTX_ENC_MAX_JOBS 16
+typedef struct {
+  ft_ge25519_array c1, c2;
+  const void *d_m, *d_r;
+} ftx_enc_job;
+ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key key, int msg_bits,
+                               void *stream);
 /* out must have n_seg elements; d_gather may be NULL (plain CSR), otherwise
  * n_elems int32 source indices, each below len(src) (device data: the caller
  * guarantees this; ftx_gather_sum checks it on the host); d_scratch must hold
@@ -161,7 +176,7 @@ ft_error ftx_stream_create(void **stream);
 void ftx_stream_destroy(void *stream);
 /* Capture the _dev calls issued on `stream` (not the library stream) between
  * begin and end into a graph.  Capturable: ftx_from_scalars_dev, ftx_mul_dev,
- * ftx_mul_u32_dev, ftx_add_dev, ftx_encrypt_dev, ftx_rerandomize_dev,
+ * ftx_mul_u32_dev, ftx_add_dev, ftx_encrypt_dev, ftx_encrypt_batch_dev, ftx_rerandomize_dev,
  * ftx_encode_dev, and ftx_segmented_sum_dev / ftx_reduce_dev / ftx_scan_dev /
  * ftx_reduce_isum_dev / ftx_aggregate_batch_dev with caller-provided scratch
  * (and work memory). */

@@ -97,6 +97,7 @@ _SIGS = {
     "ftx_rerandomize_dev": (i32, [vp, vp, vp, vp, vp]),
     "ftx_segsum_scratch_points": (sz, [sz]),
     "ftx_segmented_sum_dev": (i32, [vp, vp, sz, sz, vp, vp, vp, vp]),
+    "ftx_encrypt_batch_dev": (i32, [vp, sz, vp, i32, vp]),
     "ftx_aggregate_batch_scratch_points": (sz, [vp, sz]),
     "ftx_aggregate_batch_dev": (i32, [vp, sz, vp, vp]),
     "ftx_reduce_isum_scratch_points": (sz, [sz, sz]),
@@ -391,6 +392,33 @@ def segmented_sum(offsets, src):
     out = ctypes.c_void_p()
     check(lib().ftx_segmented_sum(ctypes.byref(out), _p(off), len(off) - 1, src.h), "ftx_segmented_sum")
     return GeArray(out)
+
+
+ENC_MAX_JOBS = 16
+
+
+class EncJob(ctypes.Structure):
+    """ftx_enc_job: one encryption of ftx_encrypt_batch_dev."""
+    _fields_ = [("c1", ctypes.c_void_p), ("c2", ctypes.c_void_p), ("d_m", ctypes.c_void_p),
+                ("d_r", ctypes.c_void_p)]
+
+
+def enc_job(c1, c2, d_m, d_r):
+    """One job; it holds references to its arrays (the structure keeps only
+    the raw handles)."""
+    j = EncJob(c1.h, c2.h, d_m, d_r)
+    j._arrays = (c1, c2)
+    return j
+
+
+def encrypt_batch(key, jobs, msg_bits, stream=None):
+    """All jobs (EncJob list) in one launch on `stream` (default: the library
+    stream, then synchronised)."""
+    arr = (EncJob * max(1, len(jobs)))(*jobs)
+    st = stream.h if stream is not None else None
+    check(lib().ftx_encrypt_batch_dev(arr, len(jobs), key.h, msg_bits, st), "ftx_encrypt_batch_dev")
+    if stream is None:
+        check(lib().ftx_synchronize(None), "ftx_synchronize")
 
 
 AGG_SEGSUM, AGG_REDUCE, AGG_SCAN, AGG_XSCAN = 0, 1, 2, 3

@@ -592,7 +592,9 @@ hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStre
 // segmented sum (a scan's first level is a segmented sum with uniform
 // segments of 2^ch), or 64 lanes x 2^r elements of a reduce, which each
 // lane walks with a stride of 64 (coalesced) and leaves as one partial (r
-// from 0 to RCH_MAX, so that a reduce has about REDUCE_TILES tiles).  The
+// from 0 to RCH_MAX, so that a reduce has about REDUCE_TILES tiles; a reduce
+// of at most 64 REDUCE_TILES elements has no tiles and is summed straight
+// from its source afterwards).  The
 // grid holds as many waves as the GPU keeps resident; each wave takes the
 // next tile from an atomic counter until none is left, so short and long
 // jobs, and tiles that flush more or fewer segments, balance over the whole
@@ -603,6 +605,9 @@ hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStre
 
 constexpr int RCH_MAX = 8;                 // reduce: at most 2^8 elements per lane and tile
 constexpr size_t REDUCE_TILES = 4096;      // twice the waves the GPU keeps resident
+
+// reduces this short are summed straight from their source after the launch
+constexpr size_t REDUCE_DIRECT_MAX = 64 * REDUCE_TILES;
 
 // log2 elements per lane of a reduce tile for n elements
 static int reduce_rch(size_t n) {
@@ -618,7 +623,7 @@ static int reduce_rch(size_t n) {
 struct BJob {
   PtView out, src;
   const int64_t *off;                      // CSR offsets; null: uniform segments of len
-  int64_t len;                             // reduce: log2 elements per lane of its tiles
+  int64_t len;                             // reduce: log2 elements per lane of its tiles, -1: no tiles
   uint64_t nseg, n;
   uint32_t tb;                             // first global tile
   uint32_t cb, cn;                         // chunk-kind jobs: first global chunk (multiple of 64), chunks
@@ -740,6 +745,7 @@ struct BatchPlan {
   BBatch B;
   size_t C = 0, Sg = 0, P = 0;             // global chunks, CSR segments, reduce partials
   size_t maxT = 0;                         // most chunks of one job
+  int nred = 0;                            // reduce jobs (one block-sum region each)
   size_t rec_pts = 0, part_base = 0, red_base = 0, scan_base = 0, total = 0;
   size_t scan_at[AGG_BATCH_MAX];           // scan jobs (by caller's index): their scratch region
   int scan_ch[AGG_BATCH_MAX];
@@ -769,9 +775,10 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
       b.tb = (uint32_t)nt;
       if (red) {
         const int rch = reduce_rch(a.n);
-        const size_t t = (a.n + ((size_t)64 << rch) - 1) >> (6 + rch);
+        const size_t t = a.n <= REDUCE_DIRECT_MAX ? 0 : (a.n + ((size_t)64 << rch) - 1) >> (6 + rch);
         b.kind = 1;
-        b.len = rch;
+        b.len = t ? rch : -1;
+        ++p.nred;
         b.src = a.src;
         b.n = a.n;
         b.pb = (uint32_t)p.P;
@@ -813,7 +820,7 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
   p.rec_pts = 2 * p.C + (2 * p.C * 8 + 31) / 32 + 1;
   p.part_base = p.rec_pts;
   p.red_base = p.part_base + p.P;
-  p.scan_base = p.red_base + REDUCE_BLOCKS_MAX;
+  p.scan_base = p.red_base + REDUCE_BLOCKS_MAX * (size_t)p.nred;
   size_t at = p.scan_base;
   for (int i = 0; i < J; ++i) {
     p.scan_at[i] = at;
@@ -826,16 +833,16 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
 
 }  // namespace
 
-// Fork of the scans' dependent work onto library-owned streams (default
-// priority, created on first use) and its join back into the caller's
-// stream.  Held under a mutex from fork to join, so calls from several host
+// Fork of the dependent tails (scans' upper levels and second walks, reduces'
+// block sums) onto library-owned streams (default priority, created on first
+// use) and its join back into the caller's stream.  Held under a mutex from fork to join, so calls from several host
 // threads enqueue their record / wait pairs in order.  While s is being
 // captured into a graph, next() returns s itself (no fork).
 constexpr int AUX_STREAMS = 4;
 
-class ScanFork {
+class TailFork {
  public:
-  explicit ScanFork(hipStream_t s) : s_(s), lk_(mu(), std::defer_lock) {
+  explicit TailFork(hipStream_t s) : s_(s), lk_(mu(), std::defer_lock) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     lk_.lock();
@@ -845,8 +852,8 @@ class ScanFork {
     }
     on_ = true;
   }
-  ~ScanFork() { (void)join(); }
-  // stream for the next scan
+  ~TailFork() { (void)join(); }
+  // stream for the next tail
   hipStream_t next() {
     if (!on_) return s_;
     if (used_ == AUX_STREAMS) return aux()[rr_++ % AUX_STREAMS];   // all forked: round robin
@@ -956,30 +963,35 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  // the scans' upper levels and second walks go to auxiliary streams forked
-  // from s here (eagerly; inside a graph capture they stay on s), so that
-  // the two halves of a ciphertext scan, the run-tree passes and the reduce
-  // tails, all latency-bound at their small top levels, fill each other's
-  // gaps; each scan has its own scratch region
-  ScanFork fk(s);
+  // the scans' upper levels and second walks and the reduces' block sums go
+  // to auxiliary streams forked from s here (eagerly; inside a graph capture
+  // they stay on s), so that these tails, latency-bound at their small top
+  // levels, fill each other's gaps; each has its own scratch region
+  TailFork fk(s);
   for (int i = 0; i < J && e == hipSuccess; ++i) {
     if (!scan_kind(jobs[i].kind)) continue;
     e = scan_launch_ch(jobs[i].out, jobs[i].n, jobs[i].kind == AGG_XSCAN, at(p.scan_at[i]), p.scan_ch[i],
                        fk.next(), jobs[i].n > BSCAN_MAX);
   }
   if (e != hipSuccess) return e;
+  int kred = 0;
   for (int q = 0; q < p.B.J; ++q) {
     const AggJob &a = jobs[p.from[q]];
     if (a.kind != AGG_REDUCE) continue;
+    const hipStream_t rs = fk.next();
+    const PtView red = at(p.red_base + REDUCE_BLOCKS_MAX * (size_t)kred++);
+    const BJob &b = p.B.job[q];
     if (!a.n) {
-      e = launch_set_identity(a.out, 0, 1, s);
+      e = launch_set_identity(a.out, 0, 1, rs);
     } else {
-      const BJob &b = p.B.job[q];
-      const int rch = (int)b.len;
-      const size_t np = 64 * ((a.n + ((size_t)64 << rch) - 1) >> (6 + rch));
+      // from the tiles' partials, or straight from the source (no tiles)
+      const bool tiled = b.len >= 0;
+      const int rch = tiled ? (int)b.len : 0;
+      const size_t np = tiled ? 64 * ((a.n + ((size_t)64 << rch) - 1) >> (6 + rch)) : a.n;
+      const PtView from = tiled ? at(p.part_base + b.pb) : a.src;
       const unsigned g1 = reduce_grid(np, REDUCE_BLOCKS_MAX);
-      e = launch_reduce_blocks(at(p.red_base), at(p.part_base + b.pb), np, g1, s);
-      if (e == hipSuccess) e = launch_reduce_blocks(a.out, at(p.red_base), g1, 1, s);
+      e = launch_reduce_blocks(red, from, np, g1, rs);
+      if (e == hipSuccess) e = launch_reduce_blocks(a.out, red, g1, 1, rs);
     }
     if (e != hipSuccess) return e;
   }

@@ -922,6 +922,26 @@ ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key, 
   return FT_ERR_NO_ERROR;
 }
 
+ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key key, int msg_bits, void *stream) {
+  if (n_jobs > (size_t)FTX_ENC_MAX_JOBS) return FT_ERR_BAD_SIZE;
+  if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
+  EncBatch B;
+  std::memset(&B, 0, sizeof B);
+  B.J = (int)n_jobs;
+  for (size_t q = 0; q < n_jobs; ++q) {
+    const ftx_enc_job &a = jobs[q];
+    if (!a.c1 || !a.c2 || a.c1->d == a.c2->d) return FT_ERR_BAD_SIZE;
+    if (a.c1->n != a.c2->n) return FT_ERR_SIZE_MISMATCH;
+    for (size_t o = 0; o < q; ++o)      // jobs write concurrently: outputs pairwise distinct
+      if (jobs[o].c1->d == a.c1->d || jobs[o].c1->d == a.c2->d || jobs[o].c2->d == a.c1->d ||
+          jobs[o].c2->d == a.c2->d)
+        return FT_ERR_BAD_SIZE;
+    B.job[q] = EncJob{view(a.c1), view(a.c2), (const uint64_t *)a.d_m, (const uint32_t *)a.d_r, 0, a.c1->n};
+  }
+  CK(launch_encrypt_batch(B, G.btab, key->table, msg_digits(msg_bits), S(stream)));
+  return FT_ERR_NO_ERROR;
+}
+
 ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key, const uint64_t *m,
                      const uint8_t *r, size_t n, int msg_bits) {
   if (msg_bits <= 0) {

@@ -24,6 +24,21 @@ hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
                             hipStream_t s);
+// several ciphertext arrays encrypted in one launch (two lanes per ciphertext);
+// the table passes by value as a kernel argument; bases (multiples of 32
+// ciphertexts) are filled in
+constexpr int ENC_BATCH_MAX = 16;
+struct EncJob {
+  PtView c1, c2;
+  const uint64_t *m;
+  const uint32_t *r;
+  uint64_t base, n;
+};
+struct EncBatch {
+  int J;
+  EncJob job[ENC_BATCH_MAX];
+};
+hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s);
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s);
 hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long long *bad, hipStream_t s);
 unsigned reduce_grid(size_t n, unsigned max_blocks);

@@ -190,6 +190,41 @@ k_encrypt_split_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t 
   encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
 }
 
+// Several ciphertext arrays encrypted in one launch, two lanes per
+// ciphertext, the jobs' ciphertexts laid end to end from multiples of 32 (a
+// wave's 32 ciphertexts), so that the job of a wave is one: lane pair t finds
+// it in the table (a kernel argument), as a scalar, and runs encrypt_half.
+template <int BS>
+__device__ __forceinline__ void encrypt_batch_body(const EncBatch &B, const uint32_t *btab, const uint32_t *htab,
+                                                   int nd, uint64_t total) {
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const uint64_t t2 = blockIdx.x * (uint64_t)BS + threadIdx.x;
+  const uint64_t t = t2 >> 1;
+  if (t >= total) return;
+  int j = 0;
+  for (int q = 1; q < B.J; ++q)
+    if (B.job[q].base <= t) j = q;
+  j = __builtin_amdgcn_readfirstlane(j);          // the same for every lane of the wave
+  const uint64_t i = t - B.job[j].base;
+  if (i >= B.job[j].n) return;                    // padding up to the next job's start
+  encrypt_half<false>(B.job[j].c1, B.job[j].c2, i, (uint32_t)(t2 & 1), tb, th, B.job[j].m[i], nd,
+                      B.job[j].r + i * 8);
+}
+
+__global__ void __launch_bounds__(TPB) k_encrypt_batch(const EncBatch B, const uint32_t *btab,
+                                                       const uint32_t *htab, int nd, uint64_t total) {
+  encrypt_batch_body<TPB>(B, btab, htab, nd, total);
+}
+
+__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_encrypt_batch_wide(const EncBatch B, const uint32_t *btab, const uint32_t *htab, int nd, uint64_t total) {
+  encrypt_batch_body<ETPB>(B, btab, htab, nd, total);
+}
+
 template <bool ACC>
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_encrypt_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
@@ -419,6 +454,23 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
   } else {
     return hipErrorInvalidValue;
   }
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s) {
+  if (B.J < 0 || B.J > ENC_BATCH_MAX) return hipErrorInvalidValue;
+  uint64_t total = 0;
+  for (int q = 0; q < B.J; ++q) {
+    total = (total + 31) & ~(uint64_t)31;           // a wave holds 32 ciphertexts
+    B.job[q].base = total;
+    total += B.job[q].n;
+  }
+  if (!total) return hipSuccess;
+  if (2 * total >= ENC_SPLIT_WIDE_MIN)
+    hipLaunchKernelGGL(k_encrypt_batch_wide, dim3(grid_for(2 * total, ETPB)), dim3(ETPB), 0, s, B, btab, htab, nd,
+                       total);
+  else
+    hipLaunchKernelGGL(k_encrypt_batch, dim3(grid_for(2 * total)), dim3(TPB), 0, s, B, btab, htab, nd, total);
   FT_LAUNCH_CHECK();
 }
 
