@@ -201,7 +201,8 @@ def run(args):
     host_rates = {"fixed_base_mults_per_s": n / t_fixed_host, "variable_base_mults_per_s": n / t_var_host,
                   "note": "ft_array_from_scalars / ft_mul with host scalar buffers, PCIe and sync included"}
 
-    detail = {"host_buffer_calls": host_rates}
+    detail = {"host_buffer_calls": host_rates,
+              "arithmetic": "exact GF(2^255-19): nine 29-bit limbs in u32, 32x32->64 products into u64 column sums"}
     for k in ops:
         rate = mults[k] / (per[k] * 1e-3)
         ach = rate * PRODUCTS[k]
@@ -246,7 +247,7 @@ def run(args):
         "metric": "scalar-mults/s (fixed-base, variable-base, ciphertext x u32) on 1 MI355X",
         "value": value, "unit": "scalar-mults/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": step_ms, "wall_ms_per_step": wall_ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32 limbs (exact GF(2^255-19))", "data": "synthetic (seeded SplitMix64 scalars mod l)",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded SplitMix64 scalars mod l)",
         "config": {"workload": "scalar-mult arrays N=2^%d each: k*B, k*P, (a*C1, a*C2) with a u32" % args.n,
                    "n": n, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "detail": detail,
