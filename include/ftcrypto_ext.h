@@ -88,7 +88,8 @@ typedef struct {
 } ftx_enc_job;
 ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key key, int msg_bits,
                                void *stream);
-/* out must have n_seg elements; d_gather may be NULL (plain CSR), otherwise
+/* out must have n_seg elements and be another array than src
+ * (FT_ERR_BAD_SIZE); d_gather may be NULL (plain CSR), otherwise
  * n_elems int32 source indices, each below len(src) (device data: the caller
  * guarantees this; ftx_gather_sum checks it on the host); d_scratch must hold
  * ftx_segsum_scratch_points(n_elems) points (NULL: library-owned).

@@ -1074,6 +1074,7 @@ ft_error ftx_segmented_sum_dev(ft_ge25519_array out, const void *d_offsets, size
                                const void *d_gather, const_ft_ge25519_array src, ft_ge25519_array scratch,
                                void *stream) {
   if (out->n < n_seg) return FT_ERR_INSUFFICIENT_SPACE;
+  if (out->d == src->d) return FT_ERR_BAD_SIZE;   // sums are written while other lanes read src
   if (!d_gather && n_elems > src->n) return FT_ERR_OUT_OF_BOUNDS;
   const size_t need = seg_scratch_points(n_elems);
   ArrGuard own;

@@ -41,6 +41,9 @@ def test_undersized_output_buffers():
     assert hip.hipMalloc(ctypes.byref(d_off), off.nbytes) == 0
     assert hip.hipMemcpy(d_off, _p(off), off.nbytes, 1) == 0
     assert L.ftx_segmented_sum_dev(out.h, d_off, 4, 10, None, g.h, None, None) == ft.FT_ERR_INSUFFICIENT_SPACE
+    before = g.to_bytes_folded()
+    assert L.ftx_segmented_sum_dev(g.h, d_off, 4, 10, None, g.h, None, None) == ft.FT_ERR_BAD_SIZE   # in place
+    assert np.array_equal(g.to_bytes_folded(), before)
     hip.hipFree(d_off)
 
 
