@@ -49,7 +49,9 @@ ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key,
 /* c1[i] += r_i B, c2[i] += r_i H */
 ft_error ftx_rerandomize(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                          const uint8_t *r);
-/* c1[i] = k_i c1[i] and c2[i] = k_i c2[i] (c2 may be NULL) for uint32 k */
+/* c1[i] = k_i c1[i] and c2[i] = k_i c2[i] (c2 may be NULL) for uint32 k.
+ * In this and every other call taking a ciphertext's two halves, c1 and c2
+ * must be different arrays (FT_ERR_BAD_SIZE otherwise). */
 ft_error ftx_mul_u32(ft_ge25519_array c1, ft_ge25519_array c2, const uint32_t *k, size_t n);
 
 /* ---------------------------------------------------------- aggregation */

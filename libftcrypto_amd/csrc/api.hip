@@ -916,6 +916,7 @@ void ftx_key_free(ftx_key key) {
 ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key, const void *d_m,
                          const void *d_r, int msg_bits, void *stream) {
   if (c1->n != c2->n) return FT_ERR_SIZE_MISMATCH;
+  if (c1->d == c2->d) return FT_ERR_BAD_SIZE;
   if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
   CK(launch_encrypt(view(c1), view(c2), G.btab, key->table, (const uint64_t *)d_m, msg_digits(msg_bits),
                     (const uint32_t *)d_r, c1->n, false, S(stream)));
@@ -969,6 +970,7 @@ ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key, co
 ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key, const void *d_r,
                              void *stream) {
   if (c1->n != c2->n) return FT_ERR_SIZE_MISMATCH;
+  if (c1->d == c2->d) return FT_ERR_BAD_SIZE;
   CK(launch_encrypt(view(c1), view(c2), G.btab, key->table, nullptr, 0, (const uint32_t *)d_r, c1->n, true,
                     S(stream)));
   return FT_ERR_NO_ERROR;
@@ -984,6 +986,7 @@ ft_error ftx_rerandomize(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key, 
 
 ft_error ftx_mul_u32_dev(ft_ge25519_array c1, ft_ge25519_array c2, const void *d_k, void *stream) {
   if (c2 && c2->n != c1->n) return FT_ERR_SIZE_MISMATCH;
+  if (c2 && c2->d == c1->d) return FT_ERR_BAD_SIZE;   // the halves' lanes run at once
   CK(launch_mul_u32(view(c1), c2 ? view(c2) : view(c1), c2 != nullptr, (const uint32_t *)d_k, c1->n,
                     S(stream)));
   return FT_ERR_NO_ERROR;

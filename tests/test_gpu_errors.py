@@ -86,3 +86,30 @@ def test_device_memory_exhaustion_is_reported_and_recovered():
     g.reduce()
     assert bytes(g.to_bytes_folded()) == bytes(ft.GeArray.from_small_scalars(np.array([10], dtype=np.uint64))
                                                .to_bytes_folded())
+
+
+def test_ciphertext_halves_must_differ():
+    """encrypt / rerandomise / x u32 write both halves at once: the same
+    array as both is refused and left unchanged"""
+    L = ft.lib()
+    x = ft.random_scalars(1, 44, 0)
+    key = ft.Key(ft.GeArray.from_scalars(x))
+    g = ft.GeArray.from_small_scalars(np.arange(1, 9, dtype=np.uint64))
+    before = g.to_bytes_folded()
+    k = np.arange(8, dtype=np.uint32)
+    r = ft.random_scalars(8, 44, 1)
+    m = np.zeros(8, dtype=np.uint64)
+    assert L.ftx_mul_u32(g.h, g.h, _p(k), 8) == ft.FT_ERR_BAD_SIZE
+    assert L.ftx_rerandomize(g.h, g.h, key.h, _p(r)) == ft.FT_ERR_BAD_SIZE
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    dm, dr = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dm), m.nbytes) == 0 and hip.hipMalloc(ctypes.byref(dr), r.nbytes) == 0
+    assert hip.hipMemcpy(dm, _p(m), m.nbytes, 1) == 0 and hip.hipMemcpy(dr, _p(r), r.nbytes, 1) == 0
+    assert L.ftx_encrypt_dev(g.h, g.h, key.h, dm, dr, 1, None) == ft.FT_ERR_BAD_SIZE
+    L.ftx_synchronize(None)
+    hip.hipFree(dm)
+    hip.hipFree(dr)
+    assert np.array_equal(g.to_bytes_folded(), before)
