@@ -590,12 +590,23 @@ def bench_cfg4(ft, key, seed, args):
     n_req = sum(1 for op in ops if op[0] != SV.FREE)
     elems = SV.trace_elements(ops)
     t1, vec = SV.run_oneshot(key, ops, size_of)
-    del vec
     log("[bench] cfg4: one-shot %.3f s; served burst" % t1)
     srv = SV.Server(key, budget_bytes=args.cfg4_budget_gib << 30, n_streams=args.cfg4_streams, borrow=True)
-    t2, lat_b, _ = SV.run_served(srv, ops)
+    t2, lat_b, hmap = SV.run_served(srv, ops)
     st = srv.stats()
+    # byte-exact check of the served vectors left at the end against the
+    # one-shot results: every surviving vector of at most 2^18 ciphertexts
+    # (up to 48 of them) and the largest one
+    freed = {op[1] for op in ops if op[0] == SV.FREE}
+    live = sorted((size_of[v], v) for v in hmap if v not in freed)
+    sample = [v for n, v in live if n <= 1 << 18][:48] + ([live[-1][1]] if live else [])
+    same = 0
+    for v in dict.fromkeys(sample):
+        a, b = srv.read(hmap[v], size_of[v])
+        same += int(np.array_equal(a, vec[v][0].to_bytes_folded()) and np.array_equal(b, vec[v][1].to_bytes_folded()))
+    checked = len(dict.fromkeys(sample))
     srv.close()
+    del vec
     cap = n_req / t1
     rate = args.cfg4_load * cap
     log("[bench] cfg4: served burst %.3f s; Poisson at %.0f req/s" % (t2, rate))
@@ -608,6 +619,7 @@ def bench_cfg4(ft, key, seed, args):
             "oneshot_s": round(t1, 4), "oneshot_req_per_s": cap,
             "served_saturation_s": round(t2, 4), "served_vs_oneshot_throughput": t1 / t2,
             "batches_at_saturation": st["batches"],
+            "byte_exact_vs_oneshot": {"vectors_checked": checked, "equal": same},
             "fixed_load": {"offered_req_per_s": rate, "load_vs_oneshot": args.cfg4_load,
                            "wall_s": round(t3, 4), "achieved_req_per_s": n_req / t3,
                            "p50_ms": float(np.percentile(lat, 50)) / 1e3, "p99_ms": float(np.percentile(lat, 99)) / 1e3,
