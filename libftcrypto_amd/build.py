@@ -64,5 +64,23 @@ def build(force=True, verbose=False):
     return OUT
 
 
+REF_CALLER_SRC = os.path.join(HERE, "..", "tests", "native", "ref_caller.cc")
+REF_CALLER = os.path.join(HERE, "..", "tests", "native", "ref_caller")
+
+
+def build_ref_caller(verbose=False):
+    """Compile the reference-style C++ caller (tests/native/ref_caller.cc) with
+    g++ against include/ftcrypto.h, linked to the in-tree libftcrypto.so by a
+    path relative to the binary, so it runs wherever the tree is copied."""
+    cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"), REF_CALLER_SRC,
+           "-o", REF_CALLER, "-L" + HERE, "-l:libftcrypto.so", "-Wl,-rpath,$ORIGIN/../../libftcrypto_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for ref_caller.cc:\n" + r.stderr[-4000:])
+    if verbose:
+        print("built", os.path.normpath(REF_CALLER))
+    return REF_CALLER
+
+
 if __name__ == "__main__":
     build(force="--force" in sys.argv, verbose=True)
