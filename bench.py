@@ -294,8 +294,10 @@ def bench_cfg0(ft, hip, key, seed):
     out1, out2 = ft.GeArray.alloc(1), ft.GeArray.alloc(1)
     rscr1 = ft.GeArray.alloc(L.ftx_reduce_scratch_points(n))
     rscr2 = ft.GeArray.alloc(L.ftx_reduce_scratch_points(n))
-    enc_ms = T(lambda: (ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, stream)),
-                        ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, stream))))
+    # message-bound fail words of the bounded encryptions (one per stream)
+    fa, fb = ft.DeviceWord(), ft.DeviceWord()
+    enc_ms = T(lambda: (ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, fa.p, stream)),
+                        ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, fa.p, stream))))
     add_ms = T(lambda: (ft.check(L.ftx_add_dev(a1.h, b1.h, stream)), ft.check(L.ftx_add_dev(a2.h, b2.h, stream))))
     red_ms = T(lambda: (ft.check(L.ftx_reduce_dev(out1.h, a1.h, rscr1.h, stream)),
                         ft.check(L.ftx_reduce_dev(out2.h, a2.h, rscr2.h, stream))))
@@ -319,8 +321,8 @@ def bench_cfg0(ft, hip, key, seed):
 
     def step2():
         fork()
-        ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, stream))
-        ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, s2))
+        ft.check(L.ftx_encrypt_dev(a1.h, a2.h, key.h, m, r, 1, fa.p, stream))
+        ft.check(L.ftx_encrypt_dev(b1.h, b2.h, key.h, m, r, 1, fb.p, s2))
         join()
         fork()
         ft.check(L.ftx_add_dev(a1.h, b1.h, stream))
@@ -340,13 +342,15 @@ def bench_cfg0(ft, hip, key, seed):
     bscr = ft.GeArray.alloc(L.ftx_aggregate_batch_scratch_points(rjobs, 2))
 
     def step_batched():
-        ft.check(L.ftx_encrypt_batch_dev(ejobs, 2, key.h, 1, stream))
+        ft.check(L.ftx_encrypt_batch_dev(ejobs, 2, key.h, 1, fa.p, stream))
         ft.check(L.ftx_add_dev(a1.h, b1.h, stream))
         ft.check(L.ftx_add_dev(a2.h, b2.h, stream))
         ft.check(L.ftx_aggregate_batch_dev(rjobs, 2, bscr.h, stream))
 
     batched_ms = T(step_batched)
-    enc_batched_ms = T(lambda: ft.check(L.ftx_encrypt_batch_dev(ejobs, 2, key.h, 1, stream)))
+    enc_batched_ms = T(lambda: ft.check(L.ftx_encrypt_batch_dev(ejobs, 2, key.h, 1, fa.p, stream)))
+    ft.check(L.ftx_synchronize(None))
+    assert fa.read() == 0 and fb.read() == 0, "config 0: messages above their 1-bit bound"
     hip.L.hipFree(m)
     hip.L.hipFree(r)
     # elementwise ciphertext add at 2^22 (bandwidth-sized: 2 x 384 B per ct)
@@ -387,8 +391,10 @@ def bench_cfg2(ft, hip, key, seed, args):
         c1, c2 = ft.GeArray.alloc(n), ft.GeArray.alloc(n)
         m = hip.upload(rng.integers(0, 256, size=n, dtype=np.uint64))
         r = hip.upload(ft.random_scalars(n, seed, sid))
-        ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, stream))
+        fail = ft.DeviceWord()
+        ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 8, fail.p, stream))
         ft.check(L.ftx_synchronize(stream))
+        assert fail.read() == 0
         hip.L.hipFree(m)
         hip.L.hipFree(r)
         return c1, c2
@@ -553,7 +559,10 @@ def bench_cfg3(ft, hip, key, seed, args):
     rho = hip.upload(ft.random_scalars(nn, seed, 31))
     d_off, d_cols = hip.upload(row_off), hip.upload(cols)
     c1, c2, o1, o2 = (ft.GeArray.alloc(nn) for _ in range(4))
-    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 1, st.h))
+    fail = ft.DeviceWord()
+    ft.check(L.ftx_encrypt_dev(c1.h, c2.h, key.h, m, r, 1, fail.p, st.h))
+    st.synchronize()
+    assert fail.read() == 0
     # scratch with room for the point-contiguous copy of the source
     scr = ft.GeArray.alloc(L.ftx_gather_scratch_points(ne, nn))
     scr_planes = ft.GeArray.alloc(L.ftx_segsum_scratch_points(ne))

@@ -42,8 +42,11 @@ ft_error ftx_key_create(ftx_key *key, const_ft_ge25519_array pk, size_t pk_idx);
 void ftx_key_free(ftx_key key);
 
 /* c1[i] = r_i B, c2[i] = m_i B + r_i H for host m (uint64) and r (32-byte
- * scalars).  msg_bits bounds the bit length of every m_i (0: derive it from
- * the largest m_i); it fixes the number of message windows. */
+ * scalars).  msg_bits is a public bound on the bit length of every m_i: it
+ * fixes the number of message windows, so the time taken depends on it and
+ * not on the messages.  0 or 64: any uint64 (17 windows).  With a bound
+ * 0 < msg_bits < 64, a message >= 2^msg_bits gives FT_ERR_OUT_OF_BOUNDS and
+ * nothing is allocated. */
 ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key,
                      const uint64_t *m, const uint8_t *r, size_t n, int msg_bits);
 /* c1[i] += r_i B, c2[i] += r_i H */
@@ -71,15 +74,23 @@ ft_error ftx_from_scalars_dev(ft_ge25519_array out, const void *d_scalars, void 
 ft_error ftx_mul_dev(ft_ge25519_array array, const void *d_scalars, void *stream);
 ft_error ftx_mul_u32_dev(ft_ge25519_array c1, ft_ge25519_array c2, const void *d_k, void *stream);
 ft_error ftx_add_dev(ft_ge25519_array res, const_ft_ge25519_array summand, void *stream);
+/* Device form of ftx_encrypt (d_m: uint64 messages, d_r: 32-byte scalars).
+ * A bound 0 < msg_bits < 64 is checked on the device: the device uint64 at
+ * d_fail becomes 1 + the largest index whose message is >= 2^msg_bits (0 when
+ * none), and when it is nonzero the encryption writes nothing (c1 and c2 keep
+ * their contents).  d_fail is required with such a bound (FT_ERR_BAD_SIZE
+ * otherwise); with msg_bits 0 or 64 it may be NULL, and is set to 0 if not. */
 ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
-                         const void *d_m, const void *d_r, int msg_bits, void *stream);
+                         const void *d_m, const void *d_r, int msg_bits, void *d_fail, void *stream);
 ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key,
                              const void *d_r, void *stream);
 /* Several encryptions under one key in ONE launch (no counterpart in the
  * reference, whose caller composes each encryption from separate calls):
  * for every job, c1[i] = r_i B and c2[i] = m_i B + r_i H for i < len(c1),
  * with d_m (uint64) and d_r (32-byte scalars) in device memory as for
- * ftx_encrypt_dev; msg_bits bounds every job's messages.  Two lanes per
+ * ftx_encrypt_dev; msg_bits bounds every job's messages, checked on the
+ * device as there: bit j of the device uint64 at d_fail is set when a message
+ * of job j is >= 2^msg_bits, and then no job is written.  Two lanes per
  * ciphertext, the jobs' ciphertexts laid end to end, so small arrays fill
  * the GPU together.  At most FTX_ENC_MAX_JOBS jobs; the jobs' output arrays
  * must be pairwise distinct (FT_ERR_BAD_SIZE otherwise). */
@@ -89,7 +100,7 @@ typedef struct {
   const void *d_m, *d_r;
 } ftx_enc_job;
 ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key key, int msg_bits,
-                               void *stream);
+                               void *d_fail, void *stream);
 /* out must have n_seg elements and be another array than src
  * (FT_ERR_BAD_SIZE); d_gather may be NULL (plain CSR), otherwise
  * n_elems int32 source indices, each below len(src) (device data: the caller

@@ -93,11 +93,11 @@ _SIGS = {
     "ftx_mul_dev": (i32, [vp, vp, vp]),
     "ftx_mul_u32_dev": (i32, [vp, vp, vp, vp]),
     "ftx_add_dev": (i32, [vp, vp, vp]),
-    "ftx_encrypt_dev": (i32, [vp, vp, vp, vp, vp, i32, vp]),
+    "ftx_encrypt_dev": (i32, [vp, vp, vp, vp, vp, i32, vp, vp]),
     "ftx_rerandomize_dev": (i32, [vp, vp, vp, vp, vp]),
     "ftx_segsum_scratch_points": (sz, [sz]),
     "ftx_segmented_sum_dev": (i32, [vp, vp, sz, sz, vp, vp, vp, vp]),
-    "ftx_encrypt_batch_dev": (i32, [vp, sz, vp, i32, vp]),
+    "ftx_encrypt_batch_dev": (i32, [vp, sz, vp, i32, vp, vp]),
     "ftx_aggregate_batch_scratch_points": (sz, [vp, sz]),
     "ftx_aggregate_batch_dev": (i32, [vp, sz, vp, vp]),
     "ftx_reduce_isum_scratch_points": (sz, [sz, sz]),
@@ -411,14 +411,54 @@ def enc_job(c1, c2, d_m, d_r):
     return j
 
 
-def encrypt_batch(key, jobs, msg_bits, stream=None):
+class DeviceWord:
+    """One device uint64: the fail word of the bounded device encryptions
+    (ftx_encrypt_dev, ftx_encrypt_batch_dev)."""
+
+    def __init__(self):
+        self._hip = ctypes.CDLL("libamdhip64.so")
+        self._hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self._hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self._hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.p = ctypes.c_void_p()
+        if self._hip.hipMalloc(ctypes.byref(self.p), 8) != 0:
+            raise FtError(FT_ERR_OUT_OF_MEMORY, "hipMalloc of a device word")
+
+    def write(self, value):
+        v = ctypes.c_uint64(value)
+        if self._hip.hipMemcpy(self.p, ctypes.byref(v), 8, 1) != 0:
+            raise FtError(FT_ERR_INTERNAL_LIBRARY_ERROR, "hipMemcpy of a device word")
+
+    def read(self):
+        """The word's value (the stream that wrote it must be synchronised)."""
+        v = ctypes.c_uint64()
+        if self._hip.hipMemcpy(ctypes.byref(v), self.p, 8, 2) != 0:
+            raise FtError(FT_ERR_INTERNAL_LIBRARY_ERROR, "hipMemcpy of a device word")
+        return v.value
+
+    def __del__(self):
+        if getattr(self, "p", None) is not None and self.p.value:
+            self._hip.hipFree(self.p)
+            self.p = None
+
+
+def encrypt_batch(key, jobs, msg_bits, stream=None, fail=None):
     """All jobs (EncJob list) in one launch on `stream` (default: the library
-    stream, then synchronised)."""
+    stream, then synchronised).  A bound 0 < msg_bits < 64 is checked on the
+    device into `fail` (a DeviceWord; one is made when none is given): bit j
+    set when a message of job j is out of range, and then nothing is written.
+    Without a stream, a nonzero fail word raises FT_ERR_OUT_OF_BOUNDS.
+    Returns the fail word."""
     arr = (EncJob * max(1, len(jobs)))(*jobs)
     st = stream.h if stream is not None else None
-    check(lib().ftx_encrypt_batch_dev(arr, len(jobs), key.h, msg_bits, st), "ftx_encrypt_batch_dev")
+    if fail is None:
+        fail = DeviceWord()
+    check(lib().ftx_encrypt_batch_dev(arr, len(jobs), key.h, msg_bits, fail.p, st), "ftx_encrypt_batch_dev")
     if stream is None:
         check(lib().ftx_synchronize(None), "ftx_synchronize")
+        if fail.read():
+            raise FtError(FT_ERR_OUT_OF_BOUNDS, "ftx_encrypt_batch_dev: message above the bound")
+    return fail
 
 
 AGG_SEGSUM, AGG_REDUCE, AGG_SCAN, AGG_XSCAN = 0, 1, 2, 3

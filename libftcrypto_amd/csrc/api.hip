@@ -355,7 +355,9 @@ void words_to_bytes(uint8_t *o, const uint32_t *w, int nw) {
     for (int j = 0; j < 4; ++j) o[4 * k + j] = (uint8_t)(w[k] >> (8 * j));
 }
 
-int msg_digits(int bits) { return bits <= 0 ? 1 : (bits + 3) / 4 + 1; }
+// radix-16 message digits for a public bound of `bits` bits (0 or 64: any
+// uint64, 17 digits); the top digit keeps the final carry
+int msg_digits(int bits) { return bits <= 0 || bits >= 64 ? 17 : (bits + 3) / 4 + 1; }
 
 }  // namespace
 
@@ -913,19 +915,32 @@ void ftx_key_free(ftx_key key) {
   delete key;
 }
 
+// A bound 0 < msg_bits < 64 is checked on the device before the encryption:
+// the check writes the fail word, and the encryption launch is gated by it
+// (writes nothing when it is nonzero).  Without a bound (0 or 64) the fail
+// word, when given, is cleared.
+static bool msg_bounded(int msg_bits) { return msg_bits > 0 && msg_bits < 64; }
+
 ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key, const void *d_m,
-                         const void *d_r, int msg_bits, void *stream) {
+                         const void *d_r, int msg_bits, void *d_fail, void *stream) {
   if (c1->n != c2->n) return FT_ERR_SIZE_MISMATCH;
   if (c1->d == c2->d) return FT_ERR_BAD_SIZE;
   if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
+  const bool bounded = msg_bounded(msg_bits);
+  if (bounded && !d_fail) return FT_ERR_BAD_SIZE;
+  auto *fail = static_cast<unsigned long long *>(d_fail);
+  if (fail) CK(launch_msg_bound((const uint64_t *)d_m, bounded ? c1->n : 0, msg_bits, fail, S(stream)));
   CK(launch_encrypt(view(c1), view(c2), G.btab, key->table, (const uint64_t *)d_m, msg_digits(msg_bits),
-                    (const uint32_t *)d_r, c1->n, false, S(stream)));
+                    (const uint32_t *)d_r, c1->n, false, S(stream), bounded ? fail : nullptr));
   return FT_ERR_NO_ERROR;
 }
 
-ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key key, int msg_bits, void *stream) {
+ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key key, int msg_bits, void *d_fail,
+                               void *stream) {
   if (n_jobs > (size_t)FTX_ENC_MAX_JOBS) return FT_ERR_BAD_SIZE;
   if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
+  const bool bounded = msg_bounded(msg_bits);
+  if (bounded && !d_fail) return FT_ERR_BAD_SIZE;
   EncBatch B;
   std::memset(&B, 0, sizeof B);
   B.J = (int)n_jobs;
@@ -939,28 +954,32 @@ ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key k
         return FT_ERR_BAD_SIZE;
     B.job[q] = EncJob{view(a.c1), view(a.c2), (const uint64_t *)a.d_m, (const uint32_t *)a.d_r, 0, a.c1->n};
   }
-  CK(launch_encrypt_batch(B, G.btab, key->table, msg_digits(msg_bits), S(stream)));
+  auto *fail = static_cast<unsigned long long *>(d_fail);
+  if (fail) {
+    EncBatch C = B;                       // no message is read without a bound
+    if (!bounded)
+      for (int q = 0; q < C.J; ++q) C.job[q].n = 0;
+    CK(launch_msg_bound_batch(C, bounded ? msg_bits : 1, fail, S(stream)));
+  }
+  CK(launch_encrypt_batch(B, G.btab, key->table, msg_digits(msg_bits), S(stream), bounded ? fail : nullptr));
   return FT_ERR_NO_ERROR;
 }
 
 ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key, const uint64_t *m,
                      const uint8_t *r, size_t n, int msg_bits) {
-  if (msg_bits <= 0) {
-    uint64_t mx = 0;
-    for (size_t i = 0; i < n; ++i) mx |= m[i];
-    msg_bits = 0;
-    while (msg_bits < 64 && (mx >> msg_bits)) ++msg_bits;
-  } else if (msg_bits < 64) {
+  if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
+  // the bound is public and checked here, before anything is allocated
+  if (msg_bounded(msg_bits))
     for (size_t i = 0; i < n; ++i)
       if (m[i] >> msg_bits) return FT_ERR_OUT_OF_BOUNDS;
-  }
   ArrGuard a1, a2;
   CKE(alloc_array(&a1.a, n));
   CKE(alloc_array(&a2.a, n));
   DevBuf dm, dr;
   CKE(dm.upload(m, n * 8));
   CKE(dr.upload(r, n * 32));
-  CKE(ftx_encrypt_dev(a1.a, a2.a, key, dm.p, dr.p, msg_bits, G.stream));
+  CK(launch_encrypt(view(a1.a), view(a2.a), G.btab, key->table, (const uint64_t *)dm.p, msg_digits(msg_bits),
+                    (const uint32_t *)dr.p, n, false, G.stream));
   CK(hipStreamSynchronize(G.stream));
   *c1 = a1.release();
   *c2 = a2.release();

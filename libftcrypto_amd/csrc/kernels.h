@@ -18,12 +18,16 @@ hipError_t launch_varbase_w(PtView pts, const uint32_t *sc, size_t sc_stride_wor
 hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s);
 hipError_t launch_mul_u32_w(PtView a, PtView b, bool two, const uint32_t *k, size_t n, int w, hipStream_t s);
 hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
+// gate: a device word; when it is nonzero at launch time the launch writes
+// nothing (the message-bound check of bounded encryptions), null: ungated
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
-                          hipStream_t s);
+                          hipStream_t s, const unsigned long long *gate = nullptr);
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
-                            hipStream_t s);
+                            hipStream_t s, const unsigned long long *gate = nullptr);
+// fail = 1 + the largest index i with m[i] >= 2^bits, 0 when none (0 < bits < 64)
+hipError_t launch_msg_bound(const uint64_t *m, size_t n, int bits, unsigned long long *fail, hipStream_t s);
 // several ciphertext arrays encrypted in one launch (two lanes per ciphertext);
 // the table passes by value as a kernel argument; bases (multiples of 32
 // ciphertexts) are filled in
@@ -38,7 +42,10 @@ struct EncBatch {
   int J;
   EncJob job[ENC_BATCH_MAX];
 };
-hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s);
+hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s,
+                                const unsigned long long *gate = nullptr);
+// bit j of fail set when a message of job j is >= 2^bits, 0 when none
+hipError_t launch_msg_bound_batch(EncBatch &B, int bits, unsigned long long *fail, hipStream_t s);
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s);
 hipError_t launch_decode(PtView v, const uint32_t *in, size_t n, unsigned long long *bad, hipStream_t s);
 unsigned reduce_grid(size_t n, unsigned max_blocks);

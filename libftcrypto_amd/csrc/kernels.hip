@@ -131,13 +131,35 @@ __global__ void __launch_bounds__(TPB) k_build_comb(uint32_t *table, PtView src,
 
 // ------------------------------------------------------ ElGamal encryption
 
+// A launch gated by a device word (the message-bound check of the bounded
+// encryptions): when it is nonzero the whole launch writes nothing.  The word
+// is the same for every lane, so whole blocks return together, before any
+// barrier.
+__device__ __forceinline__ bool gated(const unsigned long long *gate) { return gate && *gate != 0ull; }
+
+// Message bound of an encryption (a public bit count 0 < bits < 64): fail
+// becomes 1 + the largest index whose message has a bit at or above `bits`
+// (fail must hold 0 before; a wave-wide max, then one atomic per wave).
+__global__ void __launch_bounds__(TPB) k_msg_bound(const uint64_t *m, size_t n, int bits, unsigned long long *fail) {
+  const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  long long f = -1;
+  if (i < n && (m[i] >> bits) != 0ull) f = (long long)i;
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(f, off);
+    f = o > f ? o : f;
+  }
+  if ((threadIdx.x & 63) == 0 && f >= 0) atomicMax(fail, (unsigned long long)f + 1ull);
+}
+
 // c1[i] = r_i B, c2[i] = m_i B + r_i H (or, with ACC, c1 += r_i B, c2 += r_i H:
 // rerandomisation).  The comb runs for both tables at once so the four
 // middle doublings of r H are shared with m B; the message uses ND digits
 // (ND = 0 for rerandomisation), a public bound on its bit length.
 template <bool ACC, int BS>
 __device__ __forceinline__ void encrypt_block(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
-                                              const uint64_t *m, int nd, const uint32_t *r, size_t n) {
+                                              const uint64_t *m, int nd, const uint32_t *r, size_t n,
+                                              const unsigned long long *gate) {
+  if (gated(gate)) return;
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
@@ -151,15 +173,16 @@ __device__ __forceinline__ void encrypt_block(PtView c1, PtView c2, const uint32
 template <bool ACC>
 __global__ void __launch_bounds__(TPB) k_encrypt(PtView c1, PtView c2, const uint32_t *btab,
                                                  const uint32_t *htab, const uint64_t *m, int nd,
-                                                 const uint32_t *r, size_t n) {
-  encrypt_block<ACC, TPB>(c1, c2, btab, htab, m, nd, r, n);
+                                                 const uint32_t *r, size_t n, const unsigned long long *gate) {
+  encrypt_block<ACC, TPB>(c1, c2, btab, htab, m, nd, r, n, gate);
 }
 
 // small batches: two lanes per ciphertext (encrypt_half)
 template <bool ACC>
 __global__ void __launch_bounds__(TPB) k_encrypt_split(PtView c1, PtView c2, const uint32_t *btab,
                                                        const uint32_t *htab, const uint64_t *m, int nd,
-                                                       const uint32_t *r, size_t n) {
+                                                       const uint32_t *r, size_t n, const unsigned long long *gate) {
+  if (gated(gate)) return;
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
@@ -178,7 +201,8 @@ __global__ void __launch_bounds__(TPB) k_encrypt_split(PtView c1, PtView c2, con
 template <bool ACC>
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_encrypt_split_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
-                     const uint32_t *r, size_t n) {
+                     const uint32_t *r, size_t n, const unsigned long long *gate) {
+  if (gated(gate)) return;
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
@@ -196,7 +220,8 @@ k_encrypt_split_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t 
 // it in the table (a kernel argument), as a scalar, and runs encrypt_half.
 template <int BS>
 __device__ __forceinline__ void encrypt_batch_body(const EncBatch &B, const uint32_t *btab, const uint32_t *htab,
-                                                   int nd, uint64_t total) {
+                                                   int nd, uint64_t total, const unsigned long long *gate) {
+  if (gated(gate)) return;
   __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
   __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
   lds_load_table(tb, btab);
@@ -216,20 +241,35 @@ __device__ __forceinline__ void encrypt_batch_body(const EncBatch &B, const uint
 }
 
 __global__ void __launch_bounds__(TPB) k_encrypt_batch(const EncBatch B, const uint32_t *btab,
-                                                       const uint32_t *htab, int nd, uint64_t total) {
-  encrypt_batch_body<TPB>(B, btab, htab, nd, total);
+                                                       const uint32_t *htab, int nd, uint64_t total,
+                                                       const unsigned long long *gate) {
+  encrypt_batch_body<TPB>(B, btab, htab, nd, total, gate);
 }
 
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_encrypt_batch_wide(const EncBatch B, const uint32_t *btab, const uint32_t *htab, int nd, uint64_t total) {
-  encrypt_batch_body<ETPB>(B, btab, htab, nd, total);
+k_encrypt_batch_wide(const EncBatch B, const uint32_t *btab, const uint32_t *htab, int nd, uint64_t total,
+                     const unsigned long long *gate) {
+  encrypt_batch_body<ETPB>(B, btab, htab, nd, total, gate);
+}
+
+// message bound of a batch: bit j of fail is set when a message of job j has
+// a bit at or above `bits`
+__global__ void __launch_bounds__(TPB) k_msg_bound_batch(const EncBatch B, int bits, uint64_t total,
+                                                         unsigned long long *fail) {
+  const uint64_t t = blockIdx.x * (uint64_t)TPB + threadIdx.x;
+  if (t >= total) return;
+  int j = 0;
+  for (int q = 1; q < B.J; ++q)
+    if (B.job[q].base <= t) j = q;
+  const uint64_t i = t - B.job[j].base;
+  if (i < B.job[j].n && (B.job[j].m[i] >> bits) != 0ull) atomicOr(fail, 1ull << j);
 }
 
 template <bool ACC>
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_encrypt_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
-               const uint32_t *r, size_t n) {
-  encrypt_block<ACC, ETPB>(c1, c2, btab, htab, m, nd, r, n);
+               const uint32_t *r, size_t n, const unsigned long long *gate) {
+  encrypt_block<ACC, ETPB>(c1, c2, btab, htab, m, nd, r, n, gate);
 }
 
 // ------------------------------------------------------ encode / decode
@@ -425,59 +465,80 @@ hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_b
 // lanes: 1 (one lane per ciphertext) or 2 (C1 and C2 on separate lanes)
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
-                            hipStream_t s) {
+                            hipStream_t s, const unsigned long long *gate) {
   if (!n) return hipSuccess;
   if (lanes == 1 && n >= ENC_WIDE_MIN) {
     const dim3 grid(grid_for(n, ETPB));
     if (accumulate)
-      hipLaunchKernelGGL(k_encrypt_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+      hipLaunchKernelGGL(k_encrypt_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
     else
-      hipLaunchKernelGGL(k_encrypt_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+      hipLaunchKernelGGL(k_encrypt_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else if (lanes == 1) {
     const dim3 grid(grid_for(n));
     if (accumulate)
-      hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+      hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
     else
-      hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+      hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else if (lanes == 2 && 2 * n >= ENC_SPLIT_WIDE_MIN) {
     const dim3 grid(grid_for(2 * n, ETPB));
     if (accumulate)
-      hipLaunchKernelGGL(k_encrypt_split_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+      hipLaunchKernelGGL(k_encrypt_split_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
     else
-      hipLaunchKernelGGL(k_encrypt_split_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+      hipLaunchKernelGGL(k_encrypt_split_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else if (lanes == 2) {
     const dim3 grid(grid_for(2 * n));
     if (accumulate)
-      hipLaunchKernelGGL(k_encrypt_split<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n);
+      hipLaunchKernelGGL(k_encrypt_split<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
     else
-      hipLaunchKernelGGL(k_encrypt_split<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n);
+      hipLaunchKernelGGL(k_encrypt_split<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else {
     return hipErrorInvalidValue;
   }
   FT_LAUNCH_CHECK();
 }
 
-hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s) {
-  if (B.J < 0 || B.J > ENC_BATCH_MAX) return hipErrorInvalidValue;
+// bases of the batch's jobs (multiples of 32 ciphertexts) and their total
+static uint64_t batch_bases(EncBatch &B) {
   uint64_t total = 0;
   for (int q = 0; q < B.J; ++q) {
-    total = (total + 31) & ~(uint64_t)31;           // a wave holds 32 ciphertexts
+    total = (total + 31) & ~(uint64_t)31;
     B.job[q].base = total;
     total += B.job[q].n;
   }
+  return total;
+}
+
+hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s,
+                                const unsigned long long *gate) {
+  if (B.J < 0 || B.J > ENC_BATCH_MAX) return hipErrorInvalidValue;
+  const uint64_t total = batch_bases(B);            // a wave holds 32 ciphertexts
   if (!total) return hipSuccess;
   if (2 * total >= ENC_SPLIT_WIDE_MIN)
     hipLaunchKernelGGL(k_encrypt_batch_wide, dim3(grid_for(2 * total, ETPB)), dim3(ETPB), 0, s, B, btab, htab, nd,
-                       total);
+                       total, gate);
   else
-    hipLaunchKernelGGL(k_encrypt_batch, dim3(grid_for(2 * total)), dim3(TPB), 0, s, B, btab, htab, nd, total);
+    hipLaunchKernelGGL(k_encrypt_batch, dim3(grid_for(2 * total)), dim3(TPB), 0, s, B, btab, htab, nd, total, gate);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_msg_bound(const uint64_t *m, size_t n, int bits, unsigned long long *fail, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, fail, 0ull);
+  if (n) hipLaunchKernelGGL(k_msg_bound, dim3(grid_for(n)), dim3(TPB), 0, s, m, n, bits, fail);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_msg_bound_batch(EncBatch &B, int bits, unsigned long long *fail, hipStream_t s) {
+  if (B.J < 0 || B.J > ENC_BATCH_MAX) return hipErrorInvalidValue;
+  const uint64_t total = batch_bases(B);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, fail, 0ull);
+  if (total) hipLaunchKernelGGL(k_msg_bound_batch, dim3(grid_for(total)), dim3(TPB), 0, s, B, bits, total, fail);
   FT_LAUNCH_CHECK();
 }
 
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
-                          hipStream_t s) {
-  return launch_encrypt_l(c1, c2, btab, htab, m, nd, r, n, accumulate, tune_for(n).encl, s);
+                          hipStream_t s, const unsigned long long *gate) {
+  return launch_encrypt_l(c1, c2, btab, htab, m, nd, r, n, accumulate, tune_for(n).encl, s, gate);
 }
 
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s) {

@@ -164,6 +164,7 @@ struct Handle {
   PtView c1{nullptr, 0}, c2{nullptr, 0};
   int inflight = 0;         // launched batches not yet complete
   int queued = 0;           // queued requests referencing it
+  int reading = 0;          // ftx_server_read calls encoding it (writers wait)
   bool released = false;
   bool failed = false;      // its creating request failed
   int last_slot = -1;       // stream of the last batch that touched it
@@ -216,6 +217,7 @@ struct ftx_server_s {
   uint64_t next_req = 1, next_handle = 1;
   uint64_t n_batches = 0, n_done = 0;
   bool stop = false;
+  bool draining = false;    // destroy called: queued requests memory cannot admit fail
   std::thread th;
   ft_error fatal = FT_ERR_NO_ERROR;
 };
@@ -237,7 +239,15 @@ bool ensure_staging(Slot &s, size_t dev_need, size_t host_need) {
   return true;
 }
 
-size_t in_bytes(const Request &r) { return r.bp[0] ? r.bn[0] + r.bn[1] : r.in.size(); }
+// A request's inputs are one or two parts (m and r of an encryption; the
+// multipliers; the offsets).  On the device the second part starts at the
+// next multiple of 256 bytes after the first, so that 32-byte scalars are
+// read by aligned 16-byte loads whatever n is.
+size_t up256(size_t x) { return (x + 255) / 256 * 256; }
+size_t part_off(const Request &r, int q) { return q ? up256(r.bn[0]) : 0; }
+size_t in_bytes(const Request &r) { return r.bn[1] ? up256(r.bn[0]) + r.bn[1] : r.bn[0]; }
+// host address of input part q: borrowed client memory, or the request's copy
+// (parts back to back)
 const uint8_t *in_ptr(const Request &r, int q) { return r.bp[0] ? r.bp[q] : r.in.data() + (q ? r.bn[0] : 0); }
 
 void maybe_free(ftx_server_s *S, uint64_t h) {
@@ -301,7 +311,8 @@ void wait_for(ftx_server_s *S, Slot &sl, int slot_idx, const Handle &H) {
   }
 }
 
-// 1: ready, 0: waits for a vector still being created, -1: a vector it needs failed
+// 1: ready, 0: waits (for a vector still being created, or for a read of the
+// vector it writes), -1: a vector it needs failed
 int request_ready(ftx_server_s *S, const Request &r) {
   const bool dst_in = (r.kind == K_ADD || r.kind == K_MUL);     // dst must already exist
   const bool src_in = (r.kind == K_ADD || r.kind == K_SEGSUM);
@@ -311,6 +322,7 @@ int request_ready(ftx_server_s *S, const Request &r) {
     auto it = S->handles.find(q == 0 ? r.dst : r.src);
     if (it == S->handles.end() || it->second.failed) return -1;
     if (!it->second.allocated) return 0;
+    if (q == 0 && it->second.reading > 0) return 0;
   }
   return 1;
 }
@@ -321,7 +333,7 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
   const Kind kind = batch[0]->kind;
   // staging sizes
   size_t in_total = 0;
-  for (auto &r : batch) in_total += (in_bytes(*r) + 255) / 256 * 256;
+  for (auto &r : batch) in_total += up256(in_bytes(*r));
   const size_t desc = (batch.size() * sizeof(BItem) + 255) / 256 * 256;
   if (!ensure_staging(sl, in_total + desc, in_total + desc)) {
     for (auto &r : batch) fail_request(S, r, FT_ERR_OUT_OF_MEMORY);
@@ -349,13 +361,14 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
     dins[j] = din;
     if (kind == K_ENCRYPT) {
       it.m = reinterpret_cast<const uint64_t *>(din);
-      it.r = reinterpret_cast<const uint32_t *>(din + r.n * 8);
-      const int d = r.msg_bits <= 0 ? 1 : (r.msg_bits + 3) / 4 + 1;
+      it.r = reinterpret_cast<const uint32_t *>(din + part_off(r, 1));
+      // radix-16 message digits of the public bound (0 or 64: any uint64)
+      const int d = (r.msg_bits <= 0 || r.msg_bits >= 64) ? 17 : (r.msg_bits + 3) / 4 + 1;
       nd = std::max(nd, d);
     } else if (kind == K_MUL) {
       it.k = reinterpret_cast<const uint32_t *>(din);
     }
-    off += (in_bytes(r) + 255) / 256 * 256;
+    off += up256(in_bytes(r));
     it.start = start;
     it.n = r.n;
     start += r.n;
@@ -367,14 +380,14 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
   size_t o2 = 0;
   for (auto &rp : batch) {
     const Request &r = *rp;
-    const size_t b0 = r.bp[0] ? r.bn[0] : in_bytes(r), b1 = r.bp[0] ? r.bn[1] : 0;
-    if (S->borrow) {
-      (void)hipMemcpyAsync(sl.dev + o2, in_ptr(r, 0), b0, hipMemcpyHostToDevice, sl.st);
-      if (b1) (void)hipMemcpyAsync(sl.dev + o2 + b0, in_ptr(r, 1), b1, hipMemcpyHostToDevice, sl.st);
-    } else {
-      std::memcpy(sl.host + o2, r.in.data(), r.in.size());
+    for (int q = 0; q < 2; ++q) {
+      if (!r.bn[q]) continue;
+      if (S->borrow)
+        (void)hipMemcpyAsync(sl.dev + o2 + part_off(r, q), in_ptr(r, q), r.bn[q], hipMemcpyHostToDevice, sl.st);
+      else
+        std::memcpy(sl.host + o2 + part_off(r, q), in_ptr(r, q), r.bn[q]);
     }
-    o2 += (in_bytes(r) + 255) / 256 * 256;
+    o2 += up256(in_bytes(r));
   }
   // ordering against other streams
   for (auto &r : batch) {
@@ -529,7 +542,9 @@ void scheduler(ftx_server_s *S) {
             take = false;                                 // admission: wait for memory
             bool any_inflight = false;
             for (auto &sl2 : S->slots) any_inflight |= sl2.busy;
-            if (batch.empty() && !any_inflight && D.n * 256 > S->arena.size) {   // can never fit
+            // never fits; or the server is being destroyed and nothing in
+            // flight can free memory (the client releases nothing any more)
+            if (batch.empty() && !any_inflight && (D.n * 256 > S->arena.size || S->draining)) {
               auto rr = *it;
               it = S->queue.erase(it);
               fail_request(S, rr, FT_ERR_OUT_OF_MEMORY);
@@ -614,7 +629,10 @@ void ftx_server_destroy(ftx_server S) {
   if (!S) return;
   {
     std::unique_lock<std::mutex> lk(S->mu);
-    // drain: wait for queued and in-flight work
+    // drain: wait for queued and in-flight work; queued requests that memory
+    // can no longer admit fail with FT_ERR_OUT_OF_MEMORY instead of waiting
+    S->draining = true;
+    S->cv_work.notify_all();
     S->cv_done.wait(lk, [&] {
       bool busy = false;
       for (auto &sl : S->slots) busy |= sl.busy;
@@ -638,6 +656,9 @@ void ftx_server_destroy(ftx_server S) {
 ft_error ftx_server_encrypt(ftx_server S, const uint64_t *m, const uint8_t *r, size_t n, int msg_bits,
                             uint64_t *req, uint64_t *handle) {
   if (msg_bits < 0 || msg_bits > 64) return FT_ERR_BAD_SIZE;
+  if (msg_bits > 0 && msg_bits < 64)          // a public bound, checked before queueing (as ftx_encrypt)
+    for (size_t i = 0; i < n; ++i)
+      if (m[i] >> msg_bits) return FT_ERR_OUT_OF_BOUNDS;
   auto q = std::make_shared<Request>();
   q->kind = K_ENCRYPT;
   q->n = n;
@@ -753,6 +774,7 @@ ft_error ftx_server_read(ftx_server S, uint64_t h, uint8_t *c1_enc, uint8_t *c2_
     it = S->handles.find(h);
     if (it == S->handles.end() || !it->second.allocated) return FT_ERR_OUT_OF_BOUNDS;
     it->second.queued++;                 // pin while reading
+    it->second.reading++;                // writers of h stay queued until the read ends
     v1 = it->second.c1;
     v2 = it->second.c2;
     n = it->second.n;
@@ -770,9 +792,12 @@ ft_error ftx_server_read(ftx_server S, uint64_t h, uint8_t *c1_enc, uint8_t *c2_
   if (d) (void)hipFree(d);
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    S->handles[h].queued--;
+    Handle &H = S->handles[h];
+    H.queued--;
+    H.reading--;
     maybe_free(S, h);
   }
+  S->cv_work.notify_one();
   return err;
 }
 

@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import FtError, GeArray, check, init, lib, random_scalars
+from . import FT_ERR_OUT_OF_BOUNDS, DeviceWord, FtError, GeArray, check, init, lib, random_scalars
 from .workloads import lognormal_segments
 
 ENC, ADD, MUL, SEG, FREE = range(5)
@@ -268,6 +268,7 @@ def run_oneshot(key, ops, size_of):
             dev.append(doff)
             scr = GeArray.alloc(L.ftx_segsum_scratch_points(int(op[2][-1])))
             prepared.append((SEG, op[1], doff, nseg, int(op[2][-1]), o1, o2, scr))
+    fail = DeviceWord()
     check(L.ftx_synchronize(None))
     e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
     hip.L.hipEventCreate(ctypes.byref(e0))
@@ -275,7 +276,7 @@ def run_oneshot(key, ops, size_of):
     hip.L.hipEventRecord(e0, st)
     for p in prepared:
         if p[0] == ENC:
-            check(L.ftx_encrypt_dev(p[1].h, p[2].h, key.h, p[3], p[4], p[5], st))
+            check(L.ftx_encrypt_dev(p[1].h, p[2].h, key.h, p[3], p[4], p[5], fail.p, st))
         elif p[0] == ADD:
             d, s = vec[p[1]], vec[p[2]]
             check(L.ftx_add_dev(d[0].h, s[0].h, st))
@@ -289,6 +290,8 @@ def run_oneshot(key, ops, size_of):
             check(L.ftx_segmented_sum_dev(p[6].h, p[2], p[3], p[4], None, a[1].h, p[7].h, st))
     hip.L.hipEventRecord(e1, st)
     hip.L.hipEventSynchronize(e1)
+    if fail.read():
+        raise FtError(FT_ERR_OUT_OF_BOUNDS, "one-shot encryption: message above its bound")
     ms = ctypes.c_float()
     hip.L.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
     for p in dev:

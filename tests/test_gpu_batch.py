@@ -16,6 +16,7 @@ import ctypes
 import numpy as np
 import pytest
 
+import edref
 import libftcrypto_amd as ft
 from oracle import oracle as O
 
@@ -108,6 +109,55 @@ def test_batch_mixed_jobs_match_oracle():
     o = O.Array.from_scalars(sB[:66000])
     o.scan(exclusive=True)
     assert np.array_equal(enc(xscanB), o.to_bytes())
+
+
+# element steps per lane r of a batched reduce's tiles: the smallest r <= 8
+# with n / 2^(6 + r) <= 4096 tiles; up to 2^18 elements no tiles (summed
+# straight from the source after the launch)
+REDUCE_DEPTH_SIZES = [(1 << 18) + 1, 1 << 19, (1 << 20) + 3, 1 << 21, (1 << 22) - 1, (1 << 23) + 5, 1 << 24,
+                      (1 << 25) - 64, 1 << 26]
+
+
+def reduce_depth(n):
+    if n <= 64 * 4096:
+        return None
+    r = 0
+    while r < 8 and (n >> (6 + r)) > 4096:
+        r += 1
+    return r
+
+
+def test_reduce_depth_sizes_cover_every_tile_depth():
+    assert sorted({reduce_depth(n) for n in REDUCE_DEPTH_SIZES}) == list(range(9))
+
+
+@pytest.mark.parametrize("n", REDUCE_DEPTH_SIZES)
+def test_batched_reduce_every_tile_depth(n):
+    """A batched reduce at every tile depth (r = 0 .. 8, ragged last tiles
+    included) against its closed form (sum k) B and the single-call reduce"""
+    # k < 2^32, so the sum of at most 2^26 of them is exact in uint64
+    k = (np.arange(n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(n)) % np.uint64(1 << 32)
+    X = ft.GeArray.from_small_scalars(k)
+    out = ft.GeArray.alloc(1)
+    ft.aggregate_batch([ft.agg_job(ft.AGG_REDUCE, out, X)])
+    want = O.scalarmult_base(int.to_bytes(int(k.sum(dtype=np.uint64)) % edref.L, 32, "little"))
+    assert bytes(out.to_bytes_folded()) == want, (n, reduce_depth(n))
+    X.reduce()
+    assert bytes(X.to_bytes_folded()) == want, n
+
+
+def test_batched_reduces_of_mixed_depths_in_one_launch():
+    """reduces of several tile depths and two direct ones share a launch"""
+    sizes = [100, (1 << 18) - 3, (1 << 19) + 11, (1 << 21) + 7, (1 << 23) + 5]
+    arrs, outs, wants = [], [], []
+    for q, n in enumerate(sizes):
+        k = (np.arange(n, dtype=np.uint64) * np.uint64(40503 + q) + np.uint64(q)) % np.uint64(1 << 32)
+        arrs.append(ft.GeArray.from_small_scalars(k))
+        outs.append(ft.GeArray.alloc(1))
+        wants.append(O.scalarmult_base(int.to_bytes(int(k.sum(dtype=np.uint64)) % edref.L, 32, "little")))
+    ft.aggregate_batch([ft.agg_job(ft.AGG_REDUCE, o, a) for o, a in zip(outs, arrs)])
+    for o, w, n in zip(outs, wants, sizes):
+        assert bytes(o.to_bytes_folded()) == w, n
 
 
 def test_batch_full_size_matches_single_jobs():

@@ -140,6 +140,61 @@ def test_config2_aggregation_full_size():
         assert bytes(e1[s]) == w1 and bytes(e2[s]) == w2, s
 
 
+def test_config2_one_persistent_launch_full_size():
+    """Config 2 as the benchmark runs it: both halves of the 2^26-ciphertext
+    reduce, the 2^24 inclusive scan and the 2^24 segmented sum over 2^19
+    lognormal CSR segments, six jobs in ONE ftx_aggregate_batch_dev call (the
+    reduces at eight element steps per lane and tile).  Each output is
+    checked against its closed form: the full sums, sampled scan prefixes
+    (n - 1 included), the longest and sampled segments."""
+    seed = W.SEED_BASE + 22
+    x, H, key = keypair(seed)
+    rng = np.random.default_rng(seed)
+    nr, ns = 1 << 26, 1 << 24
+    mr = rng.integers(0, 256, size=nr, dtype=np.uint64)
+    r1, r2 = ft.encrypt(key, mr, ft.random_scalars(nr, seed, 1), msg_bits=8)
+    ms = rng.integers(0, 256, size=ns, dtype=np.uint64)
+    rs = ft.random_scalars(ns, seed, 2)
+    s1, s2 = ft.encrypt(key, ms, rs, msg_bits=8)
+    mg = rng.integers(0, 256, size=ns, dtype=np.uint64)
+    rg = ft.random_scalars(ns, seed, 3)
+    g1, g2 = ft.encrypt(key, mg, rg, msg_bits=8)
+    off = W.lognormal_segments(1 << 19, ns, seed)
+    d_off = DevBuf(off)
+    nseg = len(off) - 1
+    red1, red2 = ft.GeArray.alloc(1), ft.GeArray.alloc(1)
+    seg1, seg2 = ft.GeArray.alloc(nseg), ft.GeArray.alloc(nseg)
+    jobs = [ft.agg_job(ft.AGG_REDUCE, red1, r1), ft.agg_job(ft.AGG_REDUCE, red2, r2),
+            ft.agg_job(ft.AGG_SCAN, s1), ft.agg_job(ft.AGG_SCAN, s2),
+            ft.agg_job(ft.AGG_SEGSUM, seg1, g1, d_off.p, nseg, ns),
+            ft.agg_job(ft.AGG_SEGSUM, seg2, g2, d_off.p, nseg, ns)]
+    scratch = ft.GeArray.alloc(ft.aggregate_batch_scratch_points(jobs))
+    ft.aggregate_batch(jobs, scratch)
+    # reduces
+    w1, w2 = enc_expected(H, int(mr.sum()), W.scalar_sum(ft.random_scalars(nr, seed, 1)))
+    assert bytes(red1.to_bytes_folded()) == w1 and bytes(red2.to_bytes_folded()) == w2
+    del r1, r2, mr, jobs
+    # scans
+    cm = np.cumsum(ms)
+    cr = np.cumsum(W.limb_columns(rs), axis=0, dtype=np.uint64)
+    e1, e2 = s1.to_bytes_folded().reshape(-1, 32), s2.to_bytes_folded().reshape(-1, 32)
+    for j in [0, 1, 63, 64, 4095, 4096, 65535, 65536, 1 << 20, (1 << 20) + 17, ns // 2 + 7, ns - 2, ns - 1]:
+        w1, w2 = enc_expected(H, int(cm[j]), W.limbs_to_int(cr[j]))
+        assert bytes(e1[j]) == w1 and bytes(e2[j]) == w2, j
+    del cr, e1, e2
+    # segmented sums
+    segm = np.add.reduceat(mg, off[:-1])
+    segr = np.add.reduceat(W.limb_columns(rg), off[:-1], axis=0)
+    lens = np.diff(off)
+    pick = list(np.argsort(-lens)[:4]) + list(np.random.default_rng(5).choice(nseg, 300, replace=False))
+    pick += [0, nseg - 1]
+    e1, e2 = seg1.to_bytes_folded().reshape(-1, 32), seg2.to_bytes_folded().reshape(-1, 32)
+    for sg in pick:
+        w1, w2 = enc_expected(H, int(segm[sg]), W.limbs_to_int(segr[sg]))
+        assert bytes(e1[sg]) == w1 and bytes(e2[sg]) == w2, sg
+    assert reduce_enc(seg1) == reduce_enc(g1) and reduce_enc(seg2) == reduce_enc(g2)
+
+
 def sparse_step(stream, out1, out2, c1, c2, d_off, n_nodes, n_edges, d_cols, key, d_rho, scratch):
     Lb = ft.lib()
     ft.check(Lb.ftx_segmented_sum_dev(out1.h, d_off, n_nodes, n_edges, d_cols, c1.h, scratch.h, stream.h))

@@ -84,9 +84,11 @@ def test_batch_refusals(key):
     short = ft.GeArray.alloc(4)
     Lb = ft.lib()
 
+    fail = ft.DeviceWord()
+
     def rc(jobs, bits=1):
         arr = (ft.EncJob * max(1, len(jobs)))(*jobs)
-        out = Lb.ftx_encrypt_batch_dev(arr, len(jobs), k.h, bits, None)
+        out = Lb.ftx_encrypt_batch_dev(arr, len(jobs), k.h, bits, fail.p, None)
         Lb.ftx_synchronize(None)
         return out
 
@@ -98,3 +100,8 @@ def test_batch_refusals(key):
     assert rc([ft.enc_job(c1, short, dm.p, dr.p)]) == SIZE_MISMATCH
     assert rc([j], bits=65) == BAD_SIZE
     assert rc([]) == 0
+    # a bound below 64 bits needs the fail word
+    arr = (ft.EncJob * 1)(j)
+    assert Lb.ftx_encrypt_batch_dev(arr, 1, k.h, 8, None, None) == BAD_SIZE
+    assert Lb.ftx_encrypt_batch_dev(arr, 1, k.h, 0, None, None) == 0
+    Lb.ftx_synchronize(None)

@@ -108,7 +108,7 @@ def test_ciphertext_halves_must_differ():
     dm, dr = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(dm), m.nbytes) == 0 and hip.hipMalloc(ctypes.byref(dr), r.nbytes) == 0
     assert hip.hipMemcpy(dm, _p(m), m.nbytes, 1) == 0 and hip.hipMemcpy(dr, _p(r), r.nbytes, 1) == 0
-    assert L.ftx_encrypt_dev(g.h, g.h, key.h, dm, dr, 1, None) == ft.FT_ERR_BAD_SIZE
+    assert L.ftx_encrypt_dev(g.h, g.h, key.h, dm, dr, 1, None, None) == ft.FT_ERR_BAD_SIZE
     L.ftx_synchronize(None)
     hip.hipFree(dm)
     hip.hipFree(dr)

@@ -195,6 +195,91 @@ def test_decode_dev_matches_host_decode():
     assert bad > 0
 
 
+def special_encodings():
+    """Encodings the reference's decoder treats specially (sodium_fe.cu:70-141,
+    sodium_ge.cu:137-180): y >= p (bit 255 ignored, y not reduced first), x = 0
+    with the sign bit set, y = 0, y = -1."""
+    P = 2**255 - 19
+
+    def enc(y, sign=0):
+        return (y | (sign << 255)).to_bytes(32, "little")
+
+    cases = {
+        "y=p+1 (identity)": enc(P + 1),
+        "y=p+1, sign set": enc(P + 1, 1),
+        "identity, bit 255 set": enc(1, 1),
+        "y=p-1=2^255-20 (order 2)": enc(2**255 - 20),
+        "y=p-1, sign set": enc(2**255 - 20, 1),
+        "y=p (y=0: order 4)": enc(P),
+        "y=p, sign set": enc(P, 1),
+        "y=0": enc(0),
+        "y=0, sign set": enc(0, 1),
+        "y=2^255-1": enc(2**255 - 1),
+        "y=2^255-1, sign set": enc(2**255 - 1, 1),
+        "y=p+2": enc(P + 2),
+        "y=p+3, sign set": enc(P + 3, 1),
+        "B": bytes.fromhex("58" + "66" * 31),
+        "B with y + p": enc(int.from_bytes(bytes.fromhex("58" + "66" * 31), "little") + P),
+    }
+    return cases
+
+
+def test_decode_special_encodings():
+    """Each special encoding decodes on the GPU (host and device entry points)
+    exactly as in the oracle: the same success or failure, the same canonical
+    re-encoding, and the same point (its sum with 3B and its double agree)."""
+    import ctypes
+    cases = special_encodings()
+    names = list(cases)
+    data = np.frombuffer(b"".join(cases[k] for k in names), dtype=np.uint8).copy()
+    three_b = O.Array.from_small_scalars(np.full(1, 3, dtype=np.uint64)).to_bytes()
+    L = ft.lib()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    d_in, d_fail = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d_in), len(data)) == 0 and hip.hipMalloc(ctypes.byref(d_fail), 8) == 0
+    for i, name in enumerate(names):
+        one = data[32 * i:32 * i + 32].copy()
+        rc, _ = O.decode_roundtrip(bytes(one))
+        g, bad = ft.GeArray.from_bytes_folded(one)
+        assert (bad == -1) == (rc == 0), name
+        assert hip.hipMemcpy(d_in, one.ctypes.data_as(ctypes.c_void_p), 32, 1) == 0
+        dv = ft.GeArray.alloc(1)
+        ft.check(L.ftx_decode_dev(dv.h, d_in, d_fail, None))
+        ft.check(L.ftx_synchronize(None))
+        fail = np.zeros(1, dtype=np.uint64)
+        assert hip.hipMemcpy(fail.ctypes.data_as(ctypes.c_void_p), d_fail, 8, 2) == 0
+        assert int(fail[0]) - 1 == bad, name
+        if rc != 0:
+            continue
+        o, obad = O.Array.from_bytes(one)
+        assert obad == -1, name
+        assert np.array_equal(g.to_bytes_folded(), o.to_bytes()), name
+        assert np.array_equal(dv.to_bytes_folded(), o.to_bytes()), name
+        # the decoded coordinates, not only the encoding: P + 3B and 2P
+        g.add(ft.GeArray.from_small_scalars(np.full(1, 3, dtype=np.uint64)))
+        o3 = O.Array.from_bytes(three_b)[0]
+        o.add(o3)
+        assert np.array_equal(g.to_bytes_folded(), o.to_bytes()), name
+        dv.add(dv)
+        o2 = O.Array.from_bytes(one)[0]
+        o2.add(O.Array.from_bytes(one)[0])
+        assert np.array_equal(dv.to_bytes_folded(), o2.to_bytes()), name
+    # all of them at once through the device entry point: the largest failing index
+    assert hip.hipMemcpy(d_in, data.ctypes.data_as(ctypes.c_void_p), len(data), 1) == 0
+    allv = ft.GeArray.alloc(len(names))
+    ft.check(L.ftx_decode_dev(allv.h, d_in, d_fail, None))
+    ft.check(L.ftx_synchronize(None))
+    fail = np.zeros(1, dtype=np.uint64)
+    assert hip.hipMemcpy(fail.ctypes.data_as(ctypes.c_void_p), d_fail, 8, 2) == 0
+    _, obad = O.Array.from_bytes(data)
+    assert int(fail[0]) - 1 == obad
+    hip.hipFree(d_in)
+    hip.hipFree(d_fail)
+
+
 def test_affine_and_extended_roundtrip_and_validate():
     s = scalars(300, 112)
     g = ft.GeArray.from_scalars(s)

@@ -162,3 +162,71 @@ def test_served_encrypt_both_lane_schedules(key, lanes):
     finally:
         for c, v in before.items():
             ft.tune_set_ct_lanes(c, v)
+
+
+def test_destroy_with_unadmittable_request(key):
+    """a create request that fits the budget but not the free arena while the
+    client keeps its vectors: destroy fails it instead of waiting forever"""
+    import threading
+    k, _ = key
+    srv = SV.Server(k, budget_bytes=1 << 20, n_streams=2)             # 4096 ciphertexts
+    q1, h1 = srv.encrypt(np.zeros(3000, dtype=np.uint64), ft.random_scalars(3000, 8, 1), 1)
+    srv.wait(q1)
+    srv.encrypt(np.zeros(2000, dtype=np.uint64), ft.random_scalars(2000, 8, 2), 1)   # cannot be admitted
+    done = threading.Event()
+
+    def close():
+        srv.close()
+        done.set()
+
+    t = threading.Thread(target=close, daemon=True)
+    t.start()
+    assert done.wait(30.0), "ftx_server_destroy did not return"
+    t.join()
+
+
+def test_read_ordered_against_writer(key):
+    """reads of a vector while another thread multiplies it by 2 request
+    after request: each read returns one whole state 2^j C (both halves the
+    same j, j never decreasing), never a mix of states"""
+    import threading
+    k, H = key
+    n, steps = 2048, 12
+    m = np.arange(n, dtype=np.uint64) % np.uint64(256)
+    r = ft.random_scalars(n, 8, 3)
+    srv = SV.Server(k, budget_bytes=1 << 26, n_streams=4)
+    q, h = srv.encrypt(m, r, 8)
+    srv.wait(q)
+    o1, o2 = O.encrypt(H, m, r)
+    states = []
+    for j in range(steps + 1):
+        kk = np.full(n, 1 << j, dtype=np.uint32)
+        a, b = o1.copy(), o2.copy()
+        a.mul_u32(kk)
+        b.mul_u32(kk)
+        states.append((bytes(a.to_bytes()), bytes(b.to_bytes())))
+    index = {s: j for j, s in enumerate(states)}
+    two = np.full(n, 2, dtype=np.uint32)
+    errors = []
+
+    def writer():
+        try:
+            for _ in range(steps):
+                srv.wait(srv.mul_u32(h, two))
+        except Exception as e:                       # reported by the main thread
+            errors.append(e)
+
+    t = threading.Thread(target=writer)
+    t.start()
+    seen = []
+    while t.is_alive() or not seen:
+        a, b = srv.read(h, n)
+        key_ab = (bytes(a), bytes(b))
+        assert key_ab in index, "a read returned a state that is no whole 2^j multiple"
+        seen.append(index[key_ab])
+    t.join()
+    assert not errors, errors
+    assert seen == sorted(seen)
+    a, b = srv.read(h, n)
+    assert index[(bytes(a), bytes(b))] == steps
+    srv.close()
