@@ -167,6 +167,18 @@ ft_error ftx_reduce_isum_dev(ft_ge25519_array res, const void *d_keys, const_ft_
 size_t ftx_reduce_scratch_points(size_t n);
 ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, ft_ge25519_array scratch,
                         void *stream);
+/* ftx_reduce_dev that allocates nothing (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  out[0] =
+ * sum of src (out needs 1 element), enqueued on `stream`; the block sums go
+ * to a static device array of FTX_REDUCE_NOALLOC_POINTS points (512 KiB, one
+ * per GPU, held until the process ends) instead of a scratch array.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from several threads) race, so serialise them, e.g.
+ * on one stream.  Any len(src) fits (the first pass uses at most 4096 blocks);
+ * a schedule needing more than the array holds is refused (FT_ERR_BAD_SIZE). */
+#define FTX_REDUCE_NOALLOC_POINTS 4096
+ft_error ftx_reduce_noalloc_dev(ft_ge25519_array out, const_ft_ge25519_array src, void *stream);
 /* in-place inclusive (exclusive = 0) or exclusive scan; scratch as above with
  * ftx_scan_scratch_points(len(array)) points */
 size_t ftx_scan_scratch_points(size_t n);

@@ -105,6 +105,7 @@ _SIGS = {
     "ftx_reduce_isum_dev": (i32, [vp, vp, vp, vp, vp, vp]),
     "ftx_reduce_scratch_points": (sz, [sz]),
     "ftx_reduce_dev": (i32, [vp, vp, vp, vp]),
+    "ftx_reduce_noalloc_dev": (i32, [vp, vp, vp]),
     "ftx_scan_scratch_points": (sz, [sz]),
     "ftx_scan_dev": (i32, [vp, i32, vp, vp]),
     "ftx_gather_scratch_points": (sz, [sz, sz]),

@@ -396,6 +396,7 @@ ft_error ft_crypto_init(void) {
   CK(hipMemcpyAsync(G.lnaf, naf, sizeof naf, hipMemcpyHostToDevice, G.stream));
   CK(launch_build_comb(G.btab, PtView{nullptr, 0}, 0, true, G.stream));
   CK(hipStreamSynchronize(G.stream));
+  CK(noalloc_resolve(G.device));
   tune_init_from_env();
   G.init = true;
   return FT_ERR_NO_ERROR;
@@ -1232,6 +1233,18 @@ ft_error ftx_reduce_dev(ft_ge25519_array out, const_ft_ge25519_array src, ft_ge2
   if (!scratch) return reduce_into(out, src, S(stream));
   if (scratch->cap < reduce_scratch(src->n)) return FT_ERR_INSUFFICIENT_SPACE;
   return reduce_run(out, src, view(scratch), S(stream));
+}
+
+static_assert(FTX_REDUCE_NOALLOC_POINTS == NOALLOC_POINTS, "static reduce workspace size");
+
+// ftx_reduce_dev with the library's static block-sum array as its scratch:
+// two kernel launches, nothing allocated, nothing synchronised
+ft_error ftx_reduce_noalloc_dev(ft_ge25519_array out, const_ft_ge25519_array src, void *stream) {
+  if (out->n < 1) return FT_ERR_INSUFFICIENT_SPACE;
+  if (reduce_scratch(src->n) > NOALLOC_POINTS) return FT_ERR_BAD_SIZE;   // beyond the static array
+  PtView part;
+  CK(noalloc_scratch(&part));
+  return reduce_run(out, src, part, S(stream));
 }
 
 ft_error ftx_scan_dev(ft_ge25519_array array, int exclusive, ft_ge25519_array scratch, void *stream) {

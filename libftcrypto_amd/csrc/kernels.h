@@ -114,6 +114,14 @@ hipError_t keys_out_of_range(unsigned long long *bad, const uint64_t *keys, size
 hipError_t csr_sort_keys(int64_t *off, const int32_t **order, const uint64_t *keys, size_t n, size_t m,
                          void *scratch, hipStream_t s);
 
+// noalloc.hip: the static block-sum workspace of the allocation-free reduce,
+// REDUCE_BLOCKS_MAX points in a file-scope device array (one copy per GPU).
+// noalloc_resolve takes its address on a device (once; ft_crypto_init does it
+// for the library's device), noalloc_scratch gives the current device's copy.
+constexpr size_t NOALLOC_POINTS = REDUCE_BLOCKS_MAX;
+hipError_t noalloc_resolve(int device);
+hipError_t noalloc_scratch(PtView *v);
+
 // api.hip: device tables owned by the library
 const uint32_t *base_table();
 const uint32_t *key_table(const struct ftx_key_s *key);
