@@ -49,17 +49,40 @@ PRODUCTS = {
 }
 POINT_BYTES = 128
 
-# HBM bytes per launch at N = 2^22 from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (one pass each), scaled by the same counters' reading for a
-# plain copy kernel of known size (x1.993 read, x0.994 write), and the VALU
-# busy fraction of the same run (profiles/scalarmult_n22_pmc_counters.md),
-# for the default (radix-8) variable-base kernels; their traffic includes
-# register spills.
-PMC_N22 = {
-    "fixed_base": {"read": 135.0e6, "write": 533.6e6, "valu_busy": 1.0},
-    "variable_base": {"read": 823.8e6, "write": 900.2e6, "valu_busy": 0.992},
-    "mul_u32": {"read": 1119.6e6, "write": 1334.0e6, "valu_busy": 0.978},
-}
+# HBM bytes per launch at N = 2^22 (rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, one pass each, scaled by the same counters' reading for a plain
+# copy kernel of known size) and VALU busy, read from the committed counter
+# summary below: its table has one row per kernel.
+PMC_PROFILE = os.path.join(ROOT, "profiles", "scalarmult_n22_pmc_counters.md")
+PMC_KERNEL = {"fixed_base": "k_fixed_base<0>", "variable_base": {3: "k_varbase_r8", 4: "k_varbase"},
+              "mul_u32": {3: "k_mul_u32_r8", 4: "k_mul_u32_r16"}}
+
+
+def pmc_row(kernel, path=PMC_PROFILE):
+    """{'read', 'write' (bytes per launch), 'valu_busy'} of `kernel` from the
+    profile's table, or None when the profile has no row for it."""
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    head = None
+    for ln in lines:
+        if not ln.startswith("|"):
+            continue
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if head is None and "kernel" in cells[0]:
+            head = cells
+            continue
+        if head and cells[0].strip("`") == kernel:
+            row = dict(zip(head, cells))
+            try:
+                return {"read": float(row["HBM read (MB)"]) * 1e6, "write": float(row["HBM write (MB)"]) * 1e6,
+                        "valu_busy": float(row["VALU busy"])}
+            except (KeyError, ValueError):
+                return None
+    return None
+
+
 ALG_BYTES = {"fixed_base": 32 + 128, "variable_base": 32 + 2 * 128, "mul_u32": 4 + 4 * 128}
 
 
@@ -214,10 +237,16 @@ def run(args):
                 "unit": "Tmul32/s", "frac": ach / PEAK_MUL32, "traffic": None,
                 "algorithmic_bytes": ALG_BYTES[dom] * (mults[dom] // (2 if dom == "mul_u32" else 1))}
     if n == 1 << 22:
-        pm = PMC_N22[dom]
-        roofline["traffic"] = pm["read"] + pm["write"]
-        roofline["valu_busy"] = pm["valu_busy"]
-        roofline["counters"] = "profiles/scalarmult_n22_pmc_counters.md"
+        kname = PMC_KERNEL[dom]
+        if isinstance(kname, dict):            # the window width the schedule table picks at this size
+            got = [ctypes.c_int() for _ in range(6)]
+            ft.check(L.ftx_tune_get(L.ftx_tune_size_class(n), *[ctypes.byref(g) for g in got]))
+            kname = kname[got[5].value if dom == "variable_base" else got[4].value]
+        pm = pmc_row(kname)
+        if pm:
+            roofline["traffic"] = pm["read"] + pm["write"]
+            roofline["valu_busy"] = pm["valu_busy"]
+            roofline["counters"] = "profiles/scalarmult_n22_pmc_counters.md (%s)" % kname
 
     cpu = None
     if not args.no_cpu:
@@ -639,6 +668,7 @@ def bench_cfg4(ft, key, seed, args):
 def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
     """The oracle on a bounded sample of the same step, all host cores."""
     from oracle import oracle as O
+    flags = O.use_native()                 # built for this host's CPU when gcc can
     nt = O.threads()
     samp = max(256, min(n, 1 << 19))
     t = {}
@@ -665,7 +695,7 @@ def cpu_baseline(n, k_fixed, k_var, s_var, a_u32):
     P.mul_u32(a_u32[:s1], nt=1)
     t3 = time.time()
     one = (t1 - t0) + (t2 - t1) + 2 * (t3 - t2)
-    return {"value": rate, "unit": "scalar-mults/s", "cores": nt, "kind": "port",
+    return {"value": rate, "unit": "scalar-mults/s", "cores": nt, "kind": "port", "flags": flags,
             "sample": "%d of each of the N=2^k fixed-base, variable-base and u32 mults (ct = 2 point mults)" % samp,
             "seconds": round(total, 2),
             "one_thread": {"value": 4 * s1 / one, "sample": "%d of each kind" % s1, "seconds": round(one, 2)},
@@ -687,7 +717,7 @@ def cpu_config_rates(H, seed, configs):
         fn()
         return time.time() - t0
 
-    res = {"cores": nt, "kind": "port"}
+    res = {"cores": nt, "kind": "port", "flags": O.flags()}
     # config 0: encrypt, ciphertext add, reduce
     n0 = 1 << 16
     m = rng.integers(0, 2, size=n0, dtype=np.uint64)

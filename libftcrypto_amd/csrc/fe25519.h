@@ -250,7 +250,7 @@ FT_HD uint32_t fe_iszero(const fe &a) {
 }
 
 // Constants in radix-2^29 limbs (class N), derived from their definitions and
-// checked by tests/test_native_host.py:
+// checked against them by tests/test_bounds.py:
 //   d = -121665/121666 mod p, 2d, sqrt(-1) = 2^((p-1)/4) mod p.
 constexpr fe FE_D = {{0x135978a3u, 0x0f5a6e50u, 0x10762addu, 0x00149a82u, 0x1e898007u,
                       0x003cbbbcu, 0x19ce331du, 0x1dc56dffu, 0x0052036cu}};

@@ -27,13 +27,43 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
+FLAGS_GENERIC = "gcc -O3 -fPIC -fopenmp -std=c11"
+FLAGS_NATIVE = FLAGS_GENERIC + " -march=native -mtune=native"
+_flags = FLAGS_GENERIC
+
+
+def use_native():
+    """Rebind the oracle to a build for this host's CPU (-march=native, made
+    here with make's `native` target) when that builds and loads; returns the
+    compile flags of the oracle now in use.  For timing the CPU baseline; the
+    generic build (the default) is what the tests check against."""
+    global _lib, _flags
+    import tempfile
+    so = os.path.join(tempfile.mkdtemp(prefix="ftc_oracle_native_"), "libftc_oracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native", "NATIVE_OUT=" + so], check=True, capture_output=True)
+        _lib = None
+        lib(so)
+        _flags = FLAGS_NATIVE
+    except (subprocess.CalledProcessError, OSError, RuntimeError):
+        _lib = None
+        lib()
+        _flags = FLAGS_GENERIC
+    return _flags
+
+
+def flags():
+    return _flags
+
+
+def lib(path=None):
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(_SO):
+    path = path or _SO
+    if not os.path.exists(path):
         build()
-    L = ctypes.CDLL(_SO)
+    L = ctypes.CDLL(path)
     vp, sz, i32, i64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int64
     sig = {
         "ftco_init": (i32, []),
