@@ -329,27 +329,42 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
     keys[2 * t + 1] = -1;
   }
   size_t s = seg_find1(off, 0, nseg, clo);
-  // [lo, hi) = segment s; nhi = end of segment s + 1, loaded one crossing
-  // ahead so that stepping to the next segment waits on no load
-  int64_t lo = 0, hi = 0, nhi = INT64_MAX;
-  if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
-  if (s + 1 < nseg) nhi = off[s + 2];
+  // segment s = [lo, bnd[0]); bnd[k] = end of segment s + k (INT64_MAX past
+  // the last segment).  The ends are reloaded at the start of every group of
+  // four elements, together with the group's points, so that no crossing
+  // waits on a load: on this GPU stores count in the same in-order memory
+  // counter as loads, and a boundary loaded at one crossing and read at the
+  // next would wait for every flush store issued in between.  Four crossings
+  // at most fit a group (one per non-empty segment), so five ends suffice;
+  // runs of empty segments take the search path.
+  constexpr int NB = 5;
+  int64_t lo = s < nseg ? off[s] : 0;
+  int64_t bnd[NB];
+  auto refill = [&]() {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const size_t j = s + 1 + (size_t)k;
+      const int64_t e = off[j <= nseg ? j : nseg];
+      bnd[k] = j <= nseg ? e : INT64_MAX;
+    }
+  };
   ge_p3 acc, x;
   ge_p3_0(acc);
   bool touched = false;
   // advance the segment cursor to element i; true when i lies in segment s
   auto take = [&](int64_t i) -> bool {
-    if (s < nseg && hi <= i) {
-      if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
+    if (s < nseg && bnd[0] <= i) {
+      if (touched) seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
       touched = false;
       ++s;
-      lo = hi;
-      hi = nhi;
-      if (s < nseg && hi <= i) {                 // empty segments: search
+      lo = bnd[0];
+#pragma unroll
+      for (int k = 0; k + 1 < NB; ++k) bnd[k] = bnd[k + 1];
+      if (s < nseg && bnd[0] <= i) {             // empty segments: search
         s = seg_find1(off, s, nseg, i);
-        if (s < nseg) { lo = off[s]; hi = off[s + 1]; }
+        if (s < nseg) lo = off[s];
+        refill();
       }
-      nhi = (s + 1 < nseg) ? off[s + 2] : INT64_MAX;
     }
     return s < nseg && lo <= i;
   };
@@ -366,6 +381,7 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
     const bool vec = ((uintptr_t)gather & 15) == 0;
 #pragma unroll 1
     for (int64_t g = clo; g < cend; g += 4) {
+      refill();
       int4 ix;
       if (vec && g + 4 <= cend) {
         ix = *reinterpret_cast<const int4 *>(gather + g);
@@ -400,6 +416,7 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
   } else {
 #pragma unroll 1
     for (int64_t g = clo; g < cend; g += 4) {
+      refill();
       PW q0, q1, q2, q3;
       FT_LOAD4(q0, q1, q2, q3, src, (size_t)g, (size_t)cend);
       if (take(g)) { un_pw(x, q0); accum(); }
@@ -408,7 +425,7 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
       if (g + 3 < cend && take(g + 3)) { un_pw(x, q3); accum(); }
     }
   }
-  if (touched) seg_flush(out, rec, keys, s, lo, hi, clo, chi, t, acc);
+  if (touched) seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
 }
 
 template <int CH, int MODE>

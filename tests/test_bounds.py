@@ -114,6 +114,9 @@ def test_canonical_store_and_unpack(H):
         if t < len(specials):
             x = specials[t]
             a = limbs_of(x) if x < 2**261 else None
+        elif t == len(specials):                   # every limb at the class-N maximum
+            a = [N] * 9
+            x = val(a)
         else:
             a = [rng.randrange(N + 1) for _ in range(9)]
             x = val(a)
@@ -122,6 +125,13 @@ def test_canonical_store_and_unpack(H):
         assert words_val(w) == x % P
         H.h_fe_store(w, arr(a))
         assert words_val(w) < 2**256 and words_val(w) % P == x % P
+        # the resident format: limb 8's bits from 2^255 on fold into limb 0
+        # (times 19), the rest is the plain integer sum of the limbs
+        v = list(a)
+        c = v[8] >> 23
+        v[8] &= (1 << 23) - 1
+        v[0] += 19 * c
+        assert words_val(w) == sum(vi << (29 * i) for i, vi in enumerate(v))
         r = (ctypes.c_uint32 * 9)()
         H.h_fe_from_words(r, w)
         assert val(r) == words_val(w) and max(r) <= N
