@@ -275,6 +275,15 @@ ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_b
  * saved and loaded with the table (eighth column). */
 ft_error ftx_tune_get_ct_lanes(int size_class, int *lanes);
 ft_error ftx_tune_set_ct_lanes(int size_class, int lanes);
+/* Scan form of a size class: 0, chunk levels (a walk summing chunks, the
+ * upper levels, a second walk from each chunk's prefix); 1, one pass whose
+ * 8,192-point tiles find their prefixes by a decoupled look-back over the
+ * tiles before them (no counterpart in the reference, which scans
+ * recursively: ftcrypto.cu:1474-1562).  Tuned by ftx_autotune, saved and
+ * loaded with the table (ninth column).  Batched scans
+ * (ftx_aggregate_batch_dev) always use the chunk levels. */
+ft_error ftx_tune_get_scan_lookback(int size_class, int *lookback);
+ft_error ftx_tune_set_scan_lookback(int size_class, int lookback);
 int ftx_tune_size_class(size_t n);
 
 /* ---------------------------------------------------------- inputs */

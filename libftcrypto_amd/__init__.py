@@ -125,6 +125,8 @@ _SIGS = {
     "ftx_tune_size_class": (i32, [sz]),
     "ftx_tune_get_ct_lanes": (i32, [i32, vp]),
     "ftx_tune_set_ct_lanes": (i32, [i32, i32]),
+    "ftx_tune_get_scan_lookback": (i32, [i32, vp]),
+    "ftx_tune_set_scan_lookback": (i32, [i32, i32]),
     "ftx_random_scalars": (None, [vp, sz, u64, u64]),
     "ftx_scalar_reduce64": (None, [vp, vp, sz]),
 }
@@ -558,6 +560,20 @@ def ct_lanes_table():
 
 def tune_set_ct_lanes(size_class, lanes):
     check(lib().ftx_tune_set_ct_lanes(size_class, lanes), "ftx_tune_set_ct_lanes")
+
+
+def scan_lookback_table():
+    """{class: scan form (0 chunk levels, 1 single-pass look-back)}"""
+    out = {}
+    for c in range(4):
+        v = ctypes.c_int()
+        check(lib().ftx_tune_get_scan_lookback(c, ctypes.byref(v)), "ftx_tune_get_scan_lookback")
+        out[c] = v.value
+    return out
+
+
+def tune_set_scan_lookback(size_class, lookback):
+    check(lib().ftx_tune_set_scan_lookback(size_class, lookback), "ftx_tune_set_scan_lookback")
 
 
 def tune_load(path):

@@ -64,6 +64,7 @@ struct AggTune {
   int u32w;          // window width of the 32-bit-multiplier kernels (3 or 4)
   int vbw;           // window width of the 256-bit variable-base kernel (3 or 4)
   int encl;          // lanes per ciphertext of an encryption (1, or 2: C1 and C2 apart)
+  int scanlb;        // scan: 0 chunk levels (two walks, upper levels between), 1 single-pass look-back
 };
 constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
 constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed run-tree radix log2 range
@@ -74,6 +75,8 @@ void tune_init_from_env();
 
 // aggregate.hip
 size_t scan_scratch_points(size_t n);             // enough for every schedule
+size_t lookback_scratch_points(size_t n);         // the single-pass look-back scan's
+hipError_t scan_lookback_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s);
 hipError_t scan_launch(PtView v, size_t n, bool exclusive, PtView scratch, hipStream_t s);
 hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, int ch, hipStream_t s,
                           bool level1_done = false);

@@ -12,6 +12,9 @@
 //           8-entry table, 9 windows) or 3 (radix 8, 4-entry table, 11)
 //   vbw     window width of the 256-bit variable-base mult: 4 (radix 16,
 //           64 windows) or 3 (radix 8, 86 windows)
+//   scanlb  scan form: 0 chunk levels (a walk summing chunks, upper levels,
+//           a second walk from each chunk's prefix), 1 single pass with a
+//           decoupled look-back over 8,192-point tiles
 //   encl    lanes per ciphertext of an encryption or rerandomisation: 1, or
 //           2 (C1 on one lane, C2 on the next; twice the waves of a small
 //           batch, each lane with one comb pass).  The nine limbs of a field
@@ -50,10 +53,10 @@ namespace {
 // windows for the variable base below 2^18, two lanes per ciphertext of an
 // encryption), so an untuned process is close to a tuned one.
 constexpr AggTune DEFAULT_TUNE[TUNE_CLASSES] = {
-    {4, 1, 4, 2048, 3, 4, 2}, {4, 1, 4, 512, 3, 4, 2}, {4, 1, 4, 512, 3, 3, 2}, {6, 2, 4, 2048, 3, 3, 2}};
+    {4, 1, 4, 2048, 3, 4, 2, 0}, {4, 1, 4, 512, 3, 4, 2, 0}, {4, 1, 4, 512, 3, 3, 2, 0}, {6, 2, 4, 2048, 3, 3, 2, 0}};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
-    T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES], T_encl[TUNE_CLASSES];
+    T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES], T_encl[TUNE_CLASSES], T_scanlb[TUNE_CLASSES];
 std::atomic<bool> T_ready{false};
 
 void tune_defaults() {
@@ -65,6 +68,7 @@ void tune_defaults() {
     T_u32w[c] = DEFAULT_TUNE[c].u32w;
     T_vbw[c] = DEFAULT_TUNE[c].vbw;
     T_encl[c] = DEFAULT_TUNE[c].encl;
+    T_scanlb[c] = DEFAULT_TUNE[c].scanlb;
   }
   T_ready = true;
 }
@@ -73,7 +77,7 @@ bool valid(const AggTune &t) {
   auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
   return ch_ok(t.seg1) && t.segu >= TREE_LR_MIN && t.segu <= TREE_LR_MAX && ch_ok(t.scan) && t.reduce >= 1 &&
          (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4) &&
-         (t.vbw == 3 || t.vbw == 4) && (t.encl == 1 || t.encl == 2);
+         (t.vbw == 3 || t.vbw == 4) && (t.encl == 1 || t.encl == 2) && (t.scanlb == 0 || t.scanlb == 1);
 }
 
 void tune_store(int c, const AggTune &t) {
@@ -84,6 +88,7 @@ void tune_store(int c, const AggTune &t) {
   T_u32w[c] = t.u32w;
   T_vbw[c] = t.vbw;
   T_encl[c] = t.encl;
+  T_scanlb[c] = t.scanlb;
 }
 
 }  // namespace
@@ -98,7 +103,7 @@ int tune_class(size_t n) {
 AggTune tune_for(size_t n) {
   if (!T_ready) tune_defaults();
   const int c = tune_class(n);
-  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c], T_vbw[c], T_encl[c]};
+  return AggTune{T_seg1[c], T_segu[c], T_scan[c], T_reduce[c], T_u32w[c], T_vbw[c], T_encl[c], T_scanlb[c]};
 }
 
 }  // namespace ft
@@ -179,7 +184,8 @@ ft_error ftx_tune_set(int size_class, int seg1, int segu, int scan, int reduce_b
                       int varbase_window) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
   if (!T_ready) tune_defaults();
-  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window, varbase_window, T_encl[size_class]};
+  const AggTune t{seg1, segu, scan, reduce_blocks, u32_window, varbase_window, T_encl[size_class],
+                  T_scanlb[size_class]};
   if (!valid(t)) return FT_ERR_BAD_SIZE;
   tune_store(size_class, t);
   return FT_ERR_NO_ERROR;
@@ -200,19 +206,34 @@ ft_error ftx_tune_set_ct_lanes(int size_class, int lanes) {
   return FT_ERR_NO_ERROR;
 }
 
+ft_error ftx_tune_get_scan_lookback(int size_class, int *lookback) {
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  if (!T_ready) tune_defaults();
+  *lookback = T_scanlb[size_class];
+  return FT_ERR_NO_ERROR;
+}
+
+ft_error ftx_tune_set_scan_lookback(int size_class, int lookback) {
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
+  if (lookback != 0 && lookback != 1) return FT_ERR_BAD_SIZE;
+  if (!T_ready) tune_defaults();
+  T_scanlb[size_class] = lookback;
+  return FT_ERR_NO_ERROR;
+}
+
 int ftx_tune_size_class(size_t n) { return tune_class(n); }
 
 // Text format: a comment line, then one line per class:
-// "class seg1 segu scan reduce u32w vbw encl" (files without the last column,
-// written before it existed, load with the default encl of each class).
+// "class seg1 segu scan reduce u32w vbw encl scanlb" (files without the last
+// columns, written before they existed, load with the defaults of each class).
 ft_error ftx_tune_save(const char *path) {
   FILE *f = std::fopen(path, "w");
   if (!f) return FT_ERR_INTERNAL_LIBRARY_ERROR;
   if (!T_ready) tune_defaults();
-  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w vbw encl\n");
+  std::fprintf(f, "# libftcrypto_amd launch schedules: class seg1 segu scan reduce u32w vbw encl scanlb\n");
   for (int c = 0; c < TUNE_CLASSES; ++c)
-    std::fprintf(f, "%d %d %d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c],
-                 (int)T_reduce[c], (int)T_u32w[c], (int)T_vbw[c], (int)T_encl[c]);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d\n", c, (int)T_seg1[c], (int)T_segu[c], (int)T_scan[c],
+                 (int)T_reduce[c], (int)T_u32w[c], (int)T_vbw[c], (int)T_encl[c], (int)T_scanlb[c]);
   return std::fclose(f) == 0 ? FT_ERR_NO_ERROR : FT_ERR_INTERNAL_LIBRARY_ERROR;
 }
 
@@ -225,13 +246,13 @@ ft_error ftx_tune_load(const char *path) {
   ft_error err = FT_ERR_NO_ERROR;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#' || line[0] == '\n') continue;
-    int c, a, b, d, r, w, v, l;
-    const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d", &c, &a, &b, &d, &r, &w, &v, &l);
-    if ((got != 7 && got != 8) || c < 0 || c >= TUNE_CLASSES) {
+    int c, a, b, d, r, w, v, l, k;
+    const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d %d", &c, &a, &b, &d, &r, &w, &v, &l, &k);
+    if (got < 7 || c < 0 || c >= TUNE_CLASSES) {
       err = FT_ERR_BAD_SIZE;
       break;
     }
-    t[c] = AggTune{a, b, d, r, w, v, got == 8 ? l : DEFAULT_TUNE[c].encl};
+    t[c] = AggTune{a, b, d, r, w, v, got >= 8 ? l : DEFAULT_TUNE[c].encl, got >= 9 ? k : DEFAULT_TUNE[c].scanlb};
     if (!valid(t[c])) { err = FT_ERR_BAD_SIZE; break; }
     seen[c] = true;
   }
@@ -328,7 +349,13 @@ ft_error ftx_autotune(int max_class, const char *path) {
       // in place on a copy of the points (any points time the same)
       ft_error e = time_ms(&t, ev, st.s, reps, [&] { return scan_launch_ch(C, n, false, S, a, st.s); });
       if (e != FT_ERR_NO_ERROR) return e;
-      if (t < bt) { bt = t; best.scan = a; }
+      if (t < bt) { bt = t; best.scan = a; best.scanlb = 0; }
+    }
+    {
+      // the single-pass look-back scan against the best chunk schedule
+      ft_error e = time_ms(&t, ev, st.s, reps, [&] { return scan_lookback_launch(C, n, false, S, st.s); });
+      if (e != FT_ERR_NO_ERROR) return e;
+      if (t < bt) { bt = t; best.scanlb = 1; }
     }
     bt = INFINITY;
     for (unsigned blocks = 512; blocks <= REDUCE_BLOCKS_MAX; blocks *= 2) {

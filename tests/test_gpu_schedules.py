@@ -11,8 +11,11 @@ import os
 import numpy as np
 import pytest
 
+import edref
 import libftcrypto_amd as ft
 from oracle import oracle as O
+
+L = edref.L
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +101,55 @@ def test_every_scan_schedule(n):
         g.prescan()
         same(g, o_exc)
     ft.tune_set(c, *base)
+
+
+@pytest.mark.parametrize("n", [1, 63, 5000, 8192, 8193, 65537, 70000])
+def test_lookback_scan_small(n):
+    """the single-pass look-back scan (8,192-point tiles): one tile, a tile
+    boundary, a few tiles; inclusive and exclusive against the oracle"""
+    c = ft.size_class(n)
+    before = ft.scan_lookback_table()[c]
+    s = ft.random_scalars(n, 4250 + n, 0)
+    o_inc, o_exc = O.Array.from_scalars(s), O.Array.from_scalars(s)
+    o_inc.scan()
+    o_exc.scan(exclusive=True)
+    try:
+        ft.tune_set_scan_lookback(c, 1)
+        g = ft.GeArray.from_scalars(s)
+        g.scan()
+        same(g, o_inc)
+        g = ft.GeArray.from_scalars(s)
+        g.prescan()
+        same(g, o_exc)
+    finally:
+        ft.tune_set_scan_lookback(c, before)
+
+
+@pytest.mark.parametrize("n", [(1 << 20) + 3, (1 << 22) + 7])
+def test_lookback_scan_many_tiles(n):
+    """129 and 513 tiles: look-backs that cross more than one window of 64
+    tiles; canonical outputs equal to the chunk-level scan's, and the last
+    prefix equal to the closed form (sum k) B"""
+    c = ft.size_class(n)
+    before = ft.scan_lookback_table()[c]
+    k = (np.arange(n, dtype=np.uint64) * np.uint64(40503) + np.uint64(7)) % np.uint64(1 << 30)
+    try:
+        ft.tune_set_scan_lookback(c, 0)
+        a = ft.GeArray.from_small_scalars(k)
+        a.scan()
+        want = a.to_bytes_folded()
+        ft.tune_set_scan_lookback(c, 1)
+        for rep in range(2):
+            g = ft.GeArray.from_small_scalars(k)
+            g.scan()
+            assert np.array_equal(g.to_bytes_folded(), want), rep
+        last = O.scalarmult_base(int.to_bytes(int(k.sum(dtype=np.uint64)) % L, 32, "little"))
+        assert bytes(want[-32:]) == last
+        g = ft.GeArray.from_small_scalars(k)
+        g.prescan()
+        assert np.array_equal(g.to_bytes_folded()[32:], want[:-32])
+    finally:
+        ft.tune_set_scan_lookback(c, before)
 
 
 @pytest.mark.parametrize("n", [1, 300, 70000])

@@ -68,7 +68,7 @@ def test_ct_lanes_set_get_save_load(tmp_path):
         assert ft.ct_lanes_table()[0] == 1 and ft.ct_lanes_table()[3] == 2
         p = str(tmp_path / "t.txt")
         ft.tune_save(p)
-        assert open(p).read().splitlines()[1].split()[-1] == "1"
+        assert open(p).read().splitlines()[1].split()[7] == "1"
         ft.tune_set_ct_lanes(0, 2)
         ft.tune_load(p)
         assert ft.ct_lanes_table()[0] == 1 and ft.ct_lanes_table()[3] == 2
@@ -84,3 +84,28 @@ def test_ct_lanes_set_get_save_load(tmp_path):
     finally:
         for c, v in before.items():
             ft.tune_set_ct_lanes(c, v)
+
+
+def test_scan_lookback_set_get_save_load(tmp_path):
+    before = ft.scan_lookback_table()
+    try:
+        ft.tune_set_scan_lookback(2, 1)
+        assert ft.scan_lookback_table()[2] == 1
+        p = str(tmp_path / "t.txt")
+        ft.tune_save(p)
+        rows = [ln.split() for ln in open(p).read().splitlines()[1:]]
+        assert all(len(r) == 9 for r in rows) and rows[2][8] == "1"
+        ft.tune_set_scan_lookback(2, 0)
+        ft.tune_load(p)
+        assert ft.scan_lookback_table()[2] == 1
+        with pytest.raises(ft.FtError) as e:
+            ft.tune_set_scan_lookback(1, 2)
+        assert e.value.code == ft.FT_ERR_BAD_SIZE
+        # files of eight columns (written before the scan form existed) load with the defaults
+        q = tmp_path / "eight.txt"
+        q.write_text("".join("%d 4 1 4 512 3 4 2\n" % c for c in range(4)))
+        ft.tune_load(str(q))
+        assert ft.scan_lookback_table() == {0: 0, 1: 0, 2: 0, 3: 0}
+    finally:
+        for c, v in before.items():
+            ft.tune_set_scan_lookback(c, v)
