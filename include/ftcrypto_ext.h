@@ -220,7 +220,12 @@ void ftx_graph_free(ftx_graph graph);
  * kind are batched into one kernel launch, batches run on `n_streams`
  * concurrent streams, and requests that create vectors are admitted only when
  * the `hbm_budget_bytes` arena can hold them.  Requests on one vector execute
- * in submission order. */
+ * in submission order.  A batch holds at most `max_batch_elems` ciphertexts
+ * (0: FTX_SERVER_DEFAULT_BATCH); an encryption or multiply longer than that
+ * runs as consecutive slices of that many ciphertexts, at most two in flight
+ * and slices together on at most half of the streams, so that shorter requests
+ * behind it are not held up for its whole length. */
+#define FTX_SERVER_DEFAULT_BATCH ((size_t)1 << 18)
 struct ftx_server_s;
 typedef struct ftx_server_s *ftx_server;
 
@@ -240,6 +245,9 @@ ft_error ftx_server_mul_u32(ftx_server server, uint64_t handle, const uint32_t *
 ft_error ftx_server_segsum(ftx_server server, uint64_t handle, const int64_t *offsets, size_t n_seg,
                            uint64_t *req, uint64_t *out_handle);
 ft_error ftx_server_wait(ftx_server server, uint64_t req, double *latency_us);
+/* as ftx_server_wait, and the part of the latency the request spent queued
+ * before its first batch launched (queued_us may be NULL) */
+ft_error ftx_server_wait_ex(ftx_server server, uint64_t req, double *latency_us, double *queued_us);
 ft_error ftx_server_release(ftx_server server, uint64_t handle);
 /* folded encodings of both halves of a vector (waits for its pending work) */
 ft_error ftx_server_read(ftx_server server, uint64_t handle, uint8_t *c1_enc, uint8_t *c2_enc);

@@ -14,9 +14,17 @@
 //   default-priority streams, so batches of different kinds overlap on the
 //   device; a batch touching a vector still in flight on another stream waits
 //   for that stream's event.
+// * Slicing: an encryption or multiply longer than the batch cap runs as
+//   slices of at most that many elements, each a batch of its own, at most
+//   two in flight (the next slice's inputs are copied while the previous one
+//   computes; its kernel waits for that slice's event), and slices of all
+//   long requests together hold at most half of the streams (one of one).
+//   Short requests queued behind a long one then run on the other streams
+//   instead of waiting for the whole request.
 // * Latency: submit time to the scheduler's observation of the batch's
 //   completion event (steady clock).
 #include <hip/hip_runtime.h>
+#include <sys/prctl.h>
 
 #include <algorithm>
 #include <atomic>
@@ -180,22 +188,38 @@ struct Request {
   std::vector<uint8_t> in;     // host inputs owned by the request: m|r, k, or offsets
   const uint8_t *bp[2] = {nullptr, nullptr};   // borrowed inputs (m, r) / (k) / (offsets)
   size_t bn[2] = {0, 0};
-  Clock::time_point t_submit, t_done;
-  int state = 0;               // 0 queued, 1 in flight, 2 done
+  Clock::time_point t_submit, t_launch, t_done;   // t_launch: its first batch
+  int state = 0;               // 0 queued, 1 launched (wholly or in part), 2 done
   ft_error err = FT_ERR_NO_ERROR;
+  size_t next = 0;             // sliced requests: elements launched so far
+  int slices_inflight = 0;
+};
+
+// slices of one long request in flight at once
+constexpr int SLICES_INFLIGHT = 2;
+
+// elements [s0, s0 + len) of a request, launched in one batch; `last` when the
+// request is complete with it; `slice` when it is part of a sliced request
+struct Piece {
+  std::shared_ptr<Request> r;
+  size_t s0 = 0, len = 0;
+  bool last = true;
+  bool slice = false;
 };
 
 struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
   bool busy = false;
+  bool slice = false;          // the batch in flight is a slice of a long request
+  size_t scr_off = 0, scr_bytes = 0;   // arena scratch of the batch in flight
   uint64_t seq = 0;            // sequence number of the batch in flight / last done
   uint64_t done_seq = 0;
   char *dev = nullptr;         // device staging (inputs, descriptors, scratch)
   size_t dev_bytes = 0;
   char *host = nullptr;        // pinned staging
   size_t host_bytes = 0;
-  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<Piece> pieces;
   std::vector<uint64_t> touched;
 };
 
@@ -244,11 +268,25 @@ bool ensure_staging(Slot &s, size_t dev_need, size_t host_need) {
 // next multiple of 256 bytes after the first, so that 32-byte scalars are
 // read by aligned 16-byte loads whatever n is.
 size_t up256(size_t x) { return (x + 255) / 256 * 256; }
-size_t part_off(const Request &r, int q) { return q ? up256(r.bn[0]) : 0; }
-size_t in_bytes(const Request &r) { return r.bn[1] ? up256(r.bn[0]) + r.bn[1] : r.bn[0]; }
 // host address of input part q: borrowed client memory, or the request's copy
 // (parts back to back)
 const uint8_t *in_ptr(const Request &r, int q) { return r.bp[0] ? r.bp[q] : r.in.data() + (q ? r.bn[0] : 0); }
+// a slice of an encryption or multiply reads the same element range of each
+// input part (8 + 32 or 4 bytes per element); a whole request reads all of it
+size_t piece_bytes(const Piece &p, int q) {
+  const Request &r = *p.r;
+  return p.len == r.n ? r.bn[q] : r.bn[q] / r.n * p.len;
+}
+const uint8_t *piece_ptr(const Piece &p, int q) {
+  const Request &r = *p.r;
+  return in_ptr(r, q) + (p.len == r.n ? 0 : r.bn[q] / r.n * p.s0);
+}
+size_t piece_off(const Piece &p, int q) { return q ? up256(piece_bytes(p, 0)) : 0; }
+size_t piece_in_bytes(const Piece &p) {
+  return piece_bytes(p, 1) ? up256(piece_bytes(p, 0)) + piece_bytes(p, 1) : piece_bytes(p, 0);
+}
+// elements [s0, ...) of a point array: the same planes, shifted
+PtView shifted(PtView v, size_t s0) { return PtView{v.w + 8 * s0, v.cap}; }
 
 void maybe_free(ftx_server_s *S, uint64_t h) {
   auto it = S->handles.find(h);
@@ -273,9 +311,11 @@ void place(ftx_server_s *S, Handle &H, size_t off) {
 
 void complete(ftx_server_s *S, Slot &sl) {
   const auto now = Clock::now();
-  for (auto &r : sl.reqs) {
-    r->state = 2;
-    r->t_done = now;
+  for (auto &p : sl.pieces) {
+    p.r->slices_inflight--;
+    if (!p.last) continue;
+    p.r->state = 2;
+    p.r->t_done = now;
     ++S->n_done;
   }
   for (uint64_t h : sl.touched) {
@@ -285,15 +325,19 @@ void complete(ftx_server_s *S, Slot &sl) {
       maybe_free(S, h);
     }
   }
-  sl.reqs.clear();
+  sl.pieces.clear();
   sl.touched.clear();
+  if (sl.scr_bytes) S->arena.release(sl.scr_off, sl.scr_bytes);
+  sl.scr_bytes = 0;
   sl.busy = false;
+  sl.slice = false;
   sl.done_seq = sl.seq;
 }
 
 void fail_request(ftx_server_s *S, const std::shared_ptr<Request> &r, ft_error e) {
   r->err = e;
   if (r->kind == K_ENCRYPT || r->kind == K_SEGSUM) S->handles[r->dst].failed = true;
+  if (r->state == 0) r->t_launch = Clock::now();
   r->state = 2;
   r->t_done = Clock::now();
   ++S->n_done;
@@ -301,6 +345,16 @@ void fail_request(ftx_server_s *S, const std::shared_ptr<Request> &r, ft_error e
     auto it = S->handles.find(h);
     if (h && it != S->handles.end()) { it->second.queued--; maybe_free(S, h); }
   }
+}
+
+// a batch that could not launch: a request with slices still to come is
+// still queued and leaves the queue with it
+void fail_piece(ftx_server_s *S, const Piece &p, ft_error e) {
+  if (!p.last) {
+    auto it = std::find(S->queue.begin(), S->queue.end(), p.r);
+    if (it != S->queue.end()) S->queue.erase(it);
+  }
+  fail_request(S, p.r, e);
 }
 
 // vector `h` still being written by an in-flight batch on another stream?
@@ -327,16 +381,52 @@ int request_ready(ftx_server_s *S, const Request &r) {
   return 1;
 }
 
-// launch one batch of same-kind requests on slot idx (lock held)
-void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &batch) {
+// the aggregation jobs of a batch of segmented sums (offsets at dins[j] on
+// the device; dins null: for sizing the scratch only, with the host offsets
+// standing in, since the plan counts only the segments of jobs with offsets)
+std::vector<AggJob> segsum_jobs(ftx_server_s *S, const std::vector<Piece> &batch, char *const *dins) {
+  std::vector<AggJob> jobs;
+  for (size_t j = 0; j < batch.size(); ++j) {
+    const Request &r = *batch[j].r;
+    const Handle &Sx = S->handles[r.src];
+    const Handle &D = S->handles[r.dst];
+    const size_t n_el = (size_t)reinterpret_cast<const int64_t *>(in_ptr(r, 0))[r.nseg];
+    for (int h = 0; h < 2; ++h) {
+      AggJob a{};
+      a.kind = AGG_SEGSUM;
+      a.out = h ? D.c2 : D.c1;
+      a.src = h ? Sx.c2 : Sx.c1;
+      a.off = dins ? reinterpret_cast<const int64_t *>(dins[j]) : reinterpret_cast<const int64_t *>(in_ptr(r, 0));
+      a.nseg = r.nseg;
+      a.n = n_el;
+      jobs.push_back(a);
+    }
+  }
+  return jobs;
+}
+
+// launch one batch of same-kind requests (or one slice of a long request) on
+// slot idx (lock held)
+void launch(ftx_server_s *S, int idx, std::vector<Piece> &batch) {
   Slot &sl = S->slots[idx];
-  const Kind kind = batch[0]->kind;
+  const Kind kind = batch[0].r->kind;
   // staging sizes
   size_t in_total = 0;
-  for (auto &r : batch) in_total += up256(in_bytes(*r));
+  for (auto &p : batch) in_total += up256(piece_in_bytes(p));
   const size_t desc = (batch.size() * sizeof(BItem) + 255) / 256 * 256;
-  if (!ensure_staging(sl, in_total + desc, in_total + desc)) {
-    for (auto &r : batch) fail_request(S, r, FT_ERR_OUT_OF_MEMORY);
+  // segmented sums: scratch from the arena while the batch runs (no device
+  // allocation, which could wait for the work in flight), else from the
+  // slot's staging after the descriptors
+  size_t scr_pts = 0, scr_off = 0;
+  bool scr_arena = false;
+  if (kind == K_SEGSUM) {
+    const std::vector<AggJob> sizing = segsum_jobs(S, batch, nullptr);
+    scr_pts = agg_batch_scratch_points(sizing.data(), (int)sizing.size());
+    scr_arena = S->arena.alloc(scr_pts * 128, &scr_off);
+  }
+  if (!ensure_staging(sl, in_total + desc + (scr_arena ? 0 : scr_pts * 128), in_total + desc)) {
+    if (scr_arena) S->arena.release(scr_off, scr_pts * 128);
+    for (auto &p : batch) fail_piece(S, p, FT_ERR_OUT_OF_MEMORY);
     return;
   }
   // host staging: inputs then descriptors
@@ -346,51 +436,54 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
   uint64_t start = 0;
   int nd = 1;
   for (size_t j = 0; j < batch.size(); ++j) {
-    Request &r = *batch[j];
+    const Piece &p = batch[j];
+    Request &r = *p.r;
     Handle &D = S->handles[r.dst];
     BItem &it = items[j];
     std::memset(&it, 0, sizeof it);
-    it.a1 = D.c1;
-    it.a2 = D.c2;
+    it.a1 = shifted(D.c1, p.s0);
+    it.a2 = shifted(D.c2, p.s0);
     if (kind == K_ADD) {
       Handle &Sx = S->handles[r.src];
-      it.b1 = Sx.c1;
-      it.b2 = Sx.c2;
+      it.b1 = shifted(Sx.c1, p.s0);
+      it.b2 = shifted(Sx.c2, p.s0);
     }
     char *din = sl.dev + off;
     dins[j] = din;
     if (kind == K_ENCRYPT) {
       it.m = reinterpret_cast<const uint64_t *>(din);
-      it.r = reinterpret_cast<const uint32_t *>(din + part_off(r, 1));
+      it.r = reinterpret_cast<const uint32_t *>(din + piece_off(p, 1));
       // radix-16 message digits of the public bound (0 or 64: any uint64)
       const int d = (r.msg_bits <= 0 || r.msg_bits >= 64) ? 17 : (r.msg_bits + 3) / 4 + 1;
       nd = std::max(nd, d);
     } else if (kind == K_MUL) {
       it.k = reinterpret_cast<const uint32_t *>(din);
     }
-    off += up256(in_bytes(r));
+    off += up256(piece_in_bytes(p));
     it.start = start;
-    it.n = r.n;
-    start += r.n;
+    it.n = p.len;
+    start += p.len;
   }
   // inputs: one DMA per request (borrowed, ideally pinned, client memory) or
   // a copy through the slot's pinned staging and one DMA for the batch
   std::memcpy(sl.host + off, items.data(), items.size() * sizeof(BItem));
   const BItem *ditems = reinterpret_cast<const BItem *>(sl.dev + off);
   size_t o2 = 0;
-  for (auto &rp : batch) {
-    const Request &r = *rp;
+  for (auto &p : batch) {
     for (int q = 0; q < 2; ++q) {
-      if (!r.bn[q]) continue;
+      const size_t nb = piece_bytes(p, q);
+      if (!nb) continue;
       if (S->borrow)
-        (void)hipMemcpyAsync(sl.dev + o2 + part_off(r, q), in_ptr(r, q), r.bn[q], hipMemcpyHostToDevice, sl.st);
+        (void)hipMemcpyAsync(sl.dev + o2 + piece_off(p, q), piece_ptr(p, q), nb, hipMemcpyHostToDevice, sl.st);
       else
-        std::memcpy(sl.host + o2 + part_off(r, q), in_ptr(r, q), r.bn[q]);
+        std::memcpy(sl.host + o2 + piece_off(p, q), piece_ptr(p, q), nb);
     }
-    o2 += up256(in_bytes(r));
+    o2 += up256(piece_in_bytes(p));
   }
-  // ordering against other streams
-  for (auto &r : batch) {
+  // ordering against other streams (and a slice after the previous slice of
+  // its request: that slice is the last batch that touched its vectors)
+  for (auto &p : batch) {
+    const Request *r = p.r.get();
     for (uint64_t h : {r->dst, r->src}) {
       if (!h) continue;
       auto it = S->handles.find(h);
@@ -431,105 +524,108 @@ void launch(ftx_server_s *S, int idx, std::vector<std::shared_ptr<Request>> &bat
     // every request of the batch is two jobs (C1 and C2 halves) of one
     // persistent aggregation launch; the batch's requests touch distinct
     // vectors, so no job writes what another reads
-    std::vector<AggJob> jobs;
-    for (size_t j = 0; j < batch.size(); ++j) {
-      Request &r = *batch[j];
-      Handle &Sx = S->handles[r.src];
-      Handle &D = S->handles[r.dst];
-      const size_t n_el = (size_t)reinterpret_cast<const int64_t *>(in_ptr(r, 0))[r.nseg];
-      const int64_t *doff = reinterpret_cast<const int64_t *>(dins[j]);
-      for (int h = 0; h < 2; ++h) {
-        AggJob a{};
-        a.kind = AGG_SEGSUM;
-        a.out = h ? D.c2 : D.c1;
-        a.src = h ? Sx.c2 : Sx.c1;
-        a.off = doff;
-        a.nseg = r.nseg;
-        a.n = n_el;
-        jobs.push_back(a);
-      }
-    }
-    const size_t pts = agg_batch_scratch_points(jobs.data(), (int)jobs.size());
-    // scratch after the inputs; the slot is idle so it may be reallocated
-    char *scr = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void **>(&scr), pts * 128, sl.st) != hipSuccess) {
-      e = hipErrorOutOfMemory;
-    } else {
-      e = agg_batch_launch(jobs.data(), (int)jobs.size(), PtView{reinterpret_cast<uint32_t *>(scr), pts}, sl.st);
-      (void)hipFreeAsync(scr, sl.st);
-    }
+    std::vector<AggJob> jobs = segsum_jobs(S, batch, dins.data());
+    char *scr = scr_arena ? S->arena.base + scr_off : sl.dev + off + desc;
+    e = agg_batch_launch(jobs.data(), (int)jobs.size(), PtView{reinterpret_cast<uint32_t *>(scr), scr_pts}, sl.st);
   }
   if (e == hipSuccess) e = hipEventRecord(sl.ev, sl.st);
   if (e != hipSuccess) {
+    if (scr_arena) S->arena.release(scr_off, scr_pts * 128);
     S->fatal = FT_ERR_INTERNAL_LIBRARY_ERROR;
-    for (auto &r : batch) fail_request(S, r, FT_ERR_INTERNAL_LIBRARY_ERROR);
+    for (auto &p : batch) fail_piece(S, p, FT_ERR_INTERNAL_LIBRARY_ERROR);
     return;
   }
   sl.busy = true;
+  sl.slice = batch[0].slice;
+  sl.scr_off = scr_off;
+  sl.scr_bytes = scr_arena ? scr_pts * 128 : 0;
   sl.seq++;
   ++S->n_batches;
-  for (auto &r : batch) {
-    r->state = 1;
-    for (uint64_t h : {r->dst, r->src}) {
+  const auto now = Clock::now();
+  for (auto &p : batch) {
+    Request &r = *p.r;
+    if (r.state == 0) r.t_launch = now;
+    r.state = 1;
+    r.slices_inflight++;
+    for (uint64_t h : {r.dst, r.src}) {
       if (!h) continue;
       Handle &H = S->handles[h];
-      H.queued--;
+      if (p.last) H.queued--;        // a request with slices to come stays queued
       H.inflight++;
       H.last_slot = idx;
       H.last_seq = sl.seq;
       sl.touched.push_back(h);
     }
   }
-  sl.reqs = batch;
+  sl.pieces = batch;
 }
+
+// completion polling interval of the scheduler while batches are in flight
+constexpr std::chrono::microseconds POLL{20};
 
 void scheduler(ftx_server_s *S) {
   std::unique_lock<std::mutex> lk(S->mu);
+  // the waits below are short: let them end on time (1 us timer slack)
+  (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
+  uint64_t notified = ~0ull;
   while (!S->stop) {
     // completions
-    bool any_busy = false;
+    bool any_busy = false, completed = false;
     for (auto &sl : S->slots) {
       if (!sl.busy) continue;
-      if (hipEventQuery(sl.ev) == hipSuccess) complete(S, sl);
-      else any_busy = true;
-    }
-    S->cv_done.notify_all();
-    int idle = -1;
-    for (size_t i = 0; i < S->slots.size(); ++i)
-      if (!S->slots[i].busy) { idle = (int)i; break; }
-    if (S->queue.empty() || idle < 0) {
-      if (any_busy || idle < 0) {
-        lk.unlock();
-        std::this_thread::yield();
-        lk.lock();
+      if (hipEventQuery(sl.ev) == hipSuccess) {
+        complete(S, sl);
+        completed = true;
       } else {
-        S->cv_work.wait_for(lk, std::chrono::microseconds(200));
+        any_busy = true;
       }
+    }
+    if (completed || S->n_done != notified) {
+      notified = S->n_done;
+      S->cv_done.notify_all();
+    }
+    int idle = -1, slices_busy = 0;
+    for (size_t i = 0; i < S->slots.size(); ++i) {
+      if (!S->slots[i].busy && idle < 0) idle = (int)i;
+      slices_busy += S->slots[i].busy && S->slots[i].slice;
+    }
+    // slices of long requests take at most half of the streams (one of one),
+    // so that a long request at the head of the queue does not take every
+    // stream that frees up while short requests wait behind it
+    const int slice_slots = std::max(1, (int)S->slots.size() / 2);
+    // nothing to launch: wait for a submission (which notifies) or, while
+    // batches are in flight, poll their events again after POLL; the lock is
+    // free meanwhile, so clients submitting are not held off by the polling
+    if (S->queue.empty() || idle < 0) {
+      S->cv_work.wait_for(lk, any_busy || idle < 0 ? POLL : std::chrono::microseconds(200));
       continue;
     }
     // form a batch: the oldest ready request and later ready ones of its kind;
     // a request touching a vector of a skipped request, or of one already in
-    // the batch, stays queued (per-vector order, no intra-batch aliasing)
-    std::vector<std::shared_ptr<Request>> batch;
+    // the batch, stays queued (per-vector order, no intra-batch aliasing).  A
+    // long encryption or multiply contributes its next slice, alone.
+    std::vector<Piece> batch;
     std::unordered_set<uint64_t> blocked, in_batch;
     size_t elems = 0;
     Kind kind = K_ENCRYPT;
     for (auto it = S->queue.begin(); it != S->queue.end();) {
       Request &r = **it;
+      const bool sliced = (r.kind == K_ENCRYPT || r.kind == K_MUL) && r.n > S->max_batch;
       const bool touches_blocked = (r.dst && (blocked.count(r.dst) || in_batch.count(r.dst))) ||
                                    (r.src && (blocked.count(r.src) || in_batch.count(r.src)));
-      const int ready = touches_blocked ? 0 : request_ready(S, r);
+      int ready = touches_blocked ? 0 : request_ready(S, r);
       if (ready < 0) {
         auto rr = *it;
         it = S->queue.erase(it);
         fail_request(S, rr, FT_ERR_OUT_OF_BOUNDS);
         continue;
       }
+      if (sliced && (r.slices_inflight >= SLICES_INFLIGHT || slices_busy >= slice_slots)) ready = 0;
       bool take = false;
       if (ready > 0) {
         if (batch.empty()) {
           take = true;
-        } else if (r.kind == kind && elems + r.n <= S->max_batch &&
+        } else if (!sliced && r.kind == kind && elems + r.n <= S->max_batch &&
                    (r.kind != K_SEGSUM || batch.size() < SEGSUM_BATCH_MAX)) {
           take = true;
         }
@@ -557,12 +653,19 @@ void scheduler(ftx_server_s *S) {
       }
       if (take) {
         if (batch.empty()) kind = r.kind;
-        elems += r.n;
+        Piece p{*it, 0, r.n, true, sliced};
+        if (sliced) {
+          p.s0 = r.next;
+          p.len = std::min(S->max_batch, r.n - r.next);
+          p.last = p.s0 + p.len == r.n;
+          r.next += p.len;
+        }
+        elems += p.len;
         if (r.dst) in_batch.insert(r.dst);
         if (r.src) in_batch.insert(r.src);
-        batch.push_back(*it);
-        it = S->queue.erase(it);
-        if (kind == K_SEGSUM && batch.size() == SEGSUM_BATCH_MAX) break;
+        batch.push_back(p);
+        if (p.last) it = S->queue.erase(it);
+        if (sliced || (kind == K_SEGSUM && batch.size() == SEGSUM_BATCH_MAX)) break;
       } else {
         if (r.dst) blocked.insert(r.dst);
         if (r.src) blocked.insert(r.src);
@@ -570,9 +673,7 @@ void scheduler(ftx_server_s *S) {
       }
     }
     if (batch.empty()) {
-      lk.unlock();
-      std::this_thread::yield();
-      lk.lock();
+      S->cv_work.wait_for(lk, POLL);
       continue;
     }
     launch(S, idle, batch);
@@ -605,7 +706,7 @@ ft_error ftx_server_create(ftx_server *out, ftx_key key, size_t hbm_budget_bytes
   S->key = key;
   S->borrow = (flags & FTX_SERVER_BORROW_INPUTS) != 0;
   S->btab = base_table();
-  S->max_batch = max_batch ? max_batch : ((size_t)1 << 22);
+  S->max_batch = max_batch ? max_batch : FTX_SERVER_DEFAULT_BATCH;
   if (hipMalloc(&S->arena_mem, hbm_budget_bytes) != hipSuccess) {
     (void)hipGetLastError();
     delete S;
@@ -618,6 +719,13 @@ ft_error ftx_server_create(ftx_server *out, ftx_key key, size_t hbm_budget_bytes
         hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
       ftx_server_destroy(S);
       return FT_ERR_INTERNAL_LIBRARY_ERROR;
+    }
+    // staging of the default size now, not inside the first batches' launches
+    // (a pinned allocation takes milliseconds and the scheduler would hold
+    // every queued request meanwhile)
+    if (!ensure_staging(sl, 1, 1)) {
+      ftx_server_destroy(S);
+      return FT_ERR_OUT_OF_MEMORY;
     }
   }
   S->th = std::thread(scheduler, S);
@@ -740,15 +848,20 @@ ft_error ftx_server_segsum(ftx_server S, uint64_t h, const int64_t *offsets, siz
   return submit(S, q, req);
 }
 
-ft_error ftx_server_wait(ftx_server S, uint64_t req, double *latency_us) {
+ft_error ftx_server_wait_ex(ftx_server S, uint64_t req, double *latency_us, double *queued_us) {
   std::unique_lock<std::mutex> lk(S->mu);
   auto it = S->reqs.find(req);
   if (it == S->reqs.end()) return FT_ERR_OUT_OF_BOUNDS;
   auto r = it->second;
   S->cv_done.wait(lk, [&] { return r->state == 2; });
   if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count();
+  if (queued_us) *queued_us = std::chrono::duration<double, std::micro>(r->t_launch - r->t_submit).count();
   S->reqs.erase(req);
   return r->err;
+}
+
+ft_error ftx_server_wait(ftx_server S, uint64_t req, double *latency_us) {
+  return ftx_server_wait_ex(S, req, latency_us, nullptr);
 }
 
 ft_error ftx_server_release(ftx_server S, uint64_t h) {

@@ -24,10 +24,13 @@ from . import FT_ERR_OUT_OF_BOUNDS, DeviceWord, FtError, GeArray, check, init, l
 from .workloads import lognormal_segments
 
 ENC, ADD, MUL, SEG, FREE = range(5)
+# the library's default batch cap (FTX_SERVER_DEFAULT_BATCH): longer encryptions
+# and multiplies run as slices of this many ciphertexts
+DEFAULT_BATCH = 1 << 18
 
 
 class Server:
-    def __init__(self, key, budget_bytes=64 << 30, n_streams=4, max_batch=1 << 22, borrow=False):
+    def __init__(self, key, budget_bytes=64 << 30, n_streams=4, max_batch=0, borrow=False):
         init()
         L = lib()
         for name, res, args in [
@@ -38,6 +41,7 @@ class Server:
             ("ftx_server_mul_u32", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
             ("ftx_server_segsum", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]),
             ("ftx_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+            ("ftx_server_wait_ex", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
             ("ftx_server_release", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
             ("ftx_server_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
             ("ftx_server_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -87,6 +91,12 @@ class Server:
         lat = ctypes.c_double()
         check(self.L.ftx_server_wait(self.h, req, ctypes.byref(lat)), "ftx_server_wait")
         return lat.value
+
+    def wait_ex(self, req):
+        """(latency, time queued before the first launch), microseconds"""
+        lat, q = ctypes.c_double(), ctypes.c_double()
+        check(self.L.ftx_server_wait_ex(self.h, req, ctypes.byref(lat), ctypes.byref(q)), "ftx_server_wait_ex")
+        return lat.value, q.value
 
     def release(self, h):
         check(self.L.ftx_server_release(self.h, h), "ftx_server_release")
@@ -169,14 +179,18 @@ def trace_elements(ops):
 
 # ------------------------------------------------------------------ served
 
-def run_served(server, ops, arrivals=None):
+def run_served(server, ops, arrivals=None, info=None):
     """Submit the trace (at arrival times in seconds, or all at once) and wait
     for every request.  Returns (wall seconds, per-request latencies in us,
-    trace vector -> server handle)."""
+    trace vector -> server handle).  `info` (a dict) receives the largest delay
+    of a submission behind its arrival time (ms), the time of the last
+    submission (s) and each request's time queued before its first launch
+    (us)."""
     hmap = {}
     reqs = []
     t0 = time.perf_counter()
     pending_free = []
+    lag = 0.0
     for idx, op in enumerate(ops):
         if arrivals is not None and op[0] != FREE:
             target = t0 + arrivals[idx]
@@ -201,14 +215,23 @@ def run_served(server, ops, arrivals=None):
             server.release(hmap[op[1]])
             continue
         reqs.append(req)
-    lats = [server.wait(q) for q in reqs]
+        if arrivals is not None:
+            lag = max(lag, time.perf_counter() - t0 - arrivals[idx])
+    t_sub = time.perf_counter() - t0
+    done = [server.wait_ex(q) for q in reqs]
     wall = time.perf_counter() - t0
+    lats = [d[0] for d in done]
+    if info is not None:
+        info.update(submit_lag_ms=lag * 1e3, last_submit_s=t_sub, queued_us=np.array([d[1] for d in done]))
     return wall, np.array(lats), hmap
 
 
 def poisson_arrivals(ops, rate, seed):
+    """Arrival times (s) of a Poisson process of `rate` requests per second;
+    a release takes no time of its own (it follows the request before it)."""
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, size=len(ops))
+    is_req = np.array([op[0] != FREE for op in ops], dtype=np.float64)
+    gaps = rng.exponential(1.0 / rate, size=len(ops)) * is_req
     return np.cumsum(gaps)
 
 

@@ -52,6 +52,86 @@ def test_mixed_stream_equals_alone(key, streams, max_batch, budget, max_live):
     srv.close()
 
 
+def unmodified_encryptions(ops, freed):
+    modified = {o[1] for o in ops if o[0] in (SV.ADD, SV.MUL)}
+    return [op for op in ops if op[0] == SV.ENC and op[3] not in freed and op[3] not in modified]
+
+
+def slices(ops, cap):
+    """batches the sliced requests of a trace take at a batch cap of `cap`"""
+    tot = 0
+    for op in ops:
+        n = len(op[1]) if op[0] == SV.ENC else len(op[2]) if op[0] == SV.MUL else 0
+        if n > cap:
+            tot += -(-n // cap)
+    return tot
+
+
+@pytest.mark.parametrize("streams", [1, 3])
+def test_sliced_requests_equal_alone(key, streams):
+    """A batch cap of 1,000 ciphertexts: encryptions and multiplies of 1,024 to
+    8,192 ciphertexts run as slices of 1,000 (the last one short), one at a
+    time on one stream or two in flight on three, interleaved with the other
+    requests and followed by adds, multiplies and segmented sums of the same
+    vectors.  Every result equals the one-shot kernels', and sliced
+    encryptions equal the oracle's."""
+    k, H = key
+    ops, size_of = SV.make_trace(100, seed=60 + streams, mu_log2=11, sigma=1.0, lo_log2=8, hi_log2=13,
+                                 max_live=40)
+    freed = {op[1] for op in ops if op[0] == SV.FREE}
+    n_slices = slices(ops, 1000)
+    assert n_slices > 150
+    srv = SV.Server(k, budget_bytes=1 << 30, n_streams=streams, max_batch=1000)
+    wall, lats, hmap = SV.run_served(srv, ops)
+    assert len(lats) == sum(1 for op in ops if op[0] != SV.FREE)
+    st = srv.stats()
+    assert st["completed"] == len(lats)
+    assert st["batches"] >= n_slices
+    _, vec = SV.run_oneshot(k, ops, size_of)
+    assert compare_all(srv, hmap, vec, size_of, freed) == len(hmap) - len(freed)
+    enc = [op for op in unmodified_encryptions(ops, freed) if len(op[1]) > 1000]
+    assert len(enc) >= 2
+    for op in enc[:2]:
+        a, b = srv.read(hmap[op[3]], len(op[1]))
+        o1, o2 = O.encrypt(H, op[1], op[2])
+        assert np.array_equal(a, o1.to_bytes()) and np.array_equal(b, o2.to_bytes())
+    srv.close()
+
+
+def test_config4_mix_full_size(key):
+    """The serving workload at its stated sizes: 200 requests of the 40 % encrypt
+    / 30 % add / 15 % x u32 / 15 % segmented-sum mix, lognormal sizes in
+    [2^8, 2^22] ciphertexts (this trace holds two 2^22-ciphertext encryptions
+    and a 2^22 multiply, so slicing runs at full length, and a 2^20 add), a
+    64 GiB budget, four streams, borrowed pinned inputs, the
+    default batch cap.  Every vector left at the end equals the one-shot
+    kernels' result byte for byte, and three served encryptions (one of them
+    sliced) equal the oracle's."""
+    k, H = key
+    ops, size_of = SV.make_trace(200, seed=5, max_live=64)
+    pool, ops = SV.pin_inputs(ops)
+    freed = {op[1] for op in ops if op[0] == SV.FREE}
+    assert max(size_of.values()) == 1 << 22
+    assert slices(ops, 1 << 18) >= 48
+    srv = SV.Server(k, budget_bytes=64 << 30, n_streams=4, borrow=True)
+    wall, lats, hmap = SV.run_served(srv, ops)
+    assert len(lats) == sum(1 for op in ops if op[0] != SV.FREE)
+    _, vec = SV.run_oneshot(k, ops, size_of)
+    live = len(hmap) - len(freed)
+    assert live > 0
+    assert compare_all(srv, hmap, vec, size_of, freed) == live
+    del vec
+    enc = sorted(unmodified_encryptions(ops, freed), key=lambda op: len(op[1]))
+    picks = enc[:2] + [op for op in enc if len(op[1]) > 1 << 18][:1]
+    assert len(picks) == 3 and len(picks[2][1]) > 1 << 18
+    for op in picks:
+        a, b = srv.read(hmap[op[3]], len(op[1]))
+        o1, o2 = O.encrypt(H, op[1], op[2])
+        assert np.array_equal(a, o1.to_bytes()) and np.array_equal(b, o2.to_bytes()), len(op[1])
+    srv.close()
+    del pool
+
+
 def test_releases_and_oracle(key):
     k, H = key
     ops, size_of = SV.make_trace(150, seed=77, mu_log2=9, sigma=1.0, lo_log2=8, hi_log2=11, max_live=12)
@@ -102,7 +182,9 @@ def test_segsum_requests_batched(key):
     ftx_segmented_sum of the same ciphertexts."""
     k, H = key
     rng = np.random.default_rng(2024)
-    srv = SV.Server(k, budget_bytes=1 << 30, n_streams=2)
+    # a batch cap above 2^20, so that the two large encryptions below run
+    # whole (not sliced) and hold both streams
+    srv = SV.Server(k, budget_bytes=1 << 30, n_streams=2, max_batch=1 << 21)
     srcs = []
     for i in range(24):
         n = int(rng.integers(300, 6000))
