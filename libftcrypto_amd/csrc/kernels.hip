@@ -348,12 +348,56 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long v) { *p = v
 
 // ------------------------------------------------------ reduction
 
-// Each block sums a grid-strided share of its points, then a 256-lane tree
-// in LDS; part[blockIdx.x] receives the block sum.
+// q = the point p of lane (lane + d) of the wave; lanes past the wave's end
+// get their own p back (their sums are never used)
+__device__ __forceinline__ void shfl_down_p3(ge_p3 &q, const ge_p3 &p, int d) {
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    q.X.v[l] = __shfl_down(p.X.v[l], d);
+    q.Y.v[l] = __shfl_down(p.Y.v[l], d);
+    q.Z.v[l] = __shfl_down(p.Z.v[l], d);
+    q.T.v[l] = __shfl_down(p.T.v[l], d);
+  }
+}
+
+// lane 0 of the wave receives the sum of the points of lanes 0 .. 2 top - 1:
+// lane i adds lane i + d for d = top .. 1 (the unified addition is exact for
+// every pair, so the lanes whose partner is past the end compute harmless
+// extra sums)
+__device__ __forceinline__ void wave_sum_p3(ge_p3 &acc, int top) {
+#pragma unroll 1
+  for (int d = top; d > 0; d >>= 1) {
+    ge_p3 q;
+    shfl_down_p3(q, acc, d);
+    ge_p3_add(acc, acc, q);
+  }
+}
+
+__device__ __forceinline__ void p3_to_lds(uint32_t *d, const ge_p3 &p) {
+#pragma unroll
+  for (int l = 0; l < 9; ++l) { d[l] = p.X.v[l]; d[9 + l] = p.Y.v[l]; d[18 + l] = p.Z.v[l]; d[27 + l] = p.T.v[l]; }
+}
+
+__device__ __forceinline__ void p3_from_lds(ge_p3 &p, const uint32_t *s) {
+#pragma unroll
+  for (int l = 0; l < 9; ++l) { p.X.v[l] = s[l]; p.Y.v[l] = s[9 + l]; p.Z.v[l] = s[18 + l]; p.T.v[l] = s[27 + l]; }
+}
+
+// Each block sums a grid-strided share of its points; then waves 2-3 hand
+// their lanes' sums to waves 0-1 and wave 1 to wave 0 through LDS (raw
+// limbs of class N, no packing), and wave 0 sums its lanes by shuffles (six
+// levels, no barrier): nine wave-wide additions per block, as few as a tree
+// over 256 lanes needs.  When the grid has one point per lane or fewer, the
+// levels that would only add identities are skipped (the decisions depend
+// on n and the block only, so every thread of a block takes the same
+// branch): a block of one point stores it, one of 64 runs six levels.
+// part[blockIdx.x] receives the block sum.
 __global__ void __launch_bounds__(TPB) k_reduce_blocks(PtView part, PtView v, size_t n) {
-  __shared__ __align__(16) uint32_t sh[TPB / 2 * 32];     // 128 points x 128 B
+  static_assert(TPB == 256, "four waves per block");
+  __shared__ uint32_t sh[128 * 36];
   const size_t lanes = (size_t)gridDim.x * TPB;
-  size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
+  const size_t b0 = blockIdx.x * (size_t)TPB;
+  size_t i = b0 + threadIdx.x;
   ge_p3 acc, q;
   PW nw;
   ge_p3_0(acc);
@@ -372,30 +416,25 @@ __global__ void __launch_bounds__(TPB) k_reduce_blocks(PtView part, PtView v, si
     pin_pw(nw);
     un_pw(q, nw);
   }
-  // tree: lanes [s, 2s) hand their sums to lanes [0, s)
-  for (int s = TPB / 2; s > 0; s >>= 1) {
-    const int t = threadIdx.x;
-    if (t >= s && t < 2 * s) {
-      uint32_t w[8];
-      uint32_t *dst = sh + (t - s) * 32;
-      fe_store_words(w, acc.X); for (int q = 0; q < 8; ++q) dst[q] = w[q];
-      fe_store_words(w, acc.Y); for (int q = 0; q < 8; ++q) dst[8 + q] = w[q];
-      fe_store_words(w, acc.Z); for (int q = 0; q < 8; ++q) dst[16 + q] = w[q];
-      fe_store_words(w, acc.T); for (int q = 0; q < 8; ++q) dst[24 + q] = w[q];
-    }
+  // lanes of this block holding a point (all when the grid strides)
+  const size_t act = n >= lanes ? TPB : (n > b0 ? (n - b0 < TPB ? n - b0 : TPB) : 0);
+  const int t = threadIdx.x;
+  if (act > 128) {
+    if (t >= 128) p3_to_lds(sh + (t - 128) * 36, acc);
     __syncthreads();
-    if (t < s) {
-      ge_p3 q;
-      const uint32_t *src = sh + t * 32;
-      fe_from_words(q.X, src);
-      fe_from_words(q.Y, src + 8);
-      fe_from_words(q.Z, src + 16);
-      fe_from_words(q.T, src + 24);
-      ge_p3_add(acc, acc, q);
-    }
+    if (t < 128) { p3_from_lds(q, sh + t * 36); ge_p3_add(acc, acc, q); }
     __syncthreads();
   }
-  if (threadIdx.x == 0) store_p3(part, blockIdx.x, acc);
+  if (act > 64) {
+    if (t >= 64 && t < 128) p3_to_lds(sh + (t - 64) * 36, acc);
+    __syncthreads();
+    if (t < 64) { p3_from_lds(q, sh + t * 36); ge_p3_add(acc, acc, q); }
+  }
+  if (t >= 64) return;
+  int top = 32;
+  while (top > 1 && (size_t)top >= act) top >>= 1;      // 2 top >= act points
+  if (act > 1) wave_sum_p3(acc, top);
+  if (t == 0) store_p3(part, blockIdx.x, acc);
 }
 
 // ------------------------------------------------------ host launchers
