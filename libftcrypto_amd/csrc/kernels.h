@@ -69,6 +69,8 @@ struct AggTune {
 constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
 constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed run-tree radix log2 range
 constexpr unsigned REDUCE_BLOCKS_MAX = 4096;
+// reduces of at most this many points run their first pass on quad lanes
+constexpr size_t REDUCE_QUAD_MAX = (size_t)1 << 18;
 int tune_class(size_t n);
 AggTune tune_for(size_t n);
 void tune_init_from_env();
