@@ -241,7 +241,7 @@ __device__ __forceinline__ void build_cached4(ge_cached tab[4], const ge_p3 &p) 
 // The same four multiples scaled to one shared Z (lambda_j = prod of the
 // other entries' Z): each entry then needs three field elements, the select
 // merges 27 words per entry instead of 36 and the table takes 117 registers
-// instead of 144, which keeps the 256-bit window loop free of spills.  It
+// instead of 144, which shortens the 256-bit window loop's spills.  It
 // costs 19 multiplies per scalar mult: 1.6 % faster over 85 windows, slower
 // over the 11 windows of a 32-bit multiplier (which keeps the plain table).
 struct ge_cached3 { fe YpX, YmX, T2d; };
@@ -281,15 +281,34 @@ __device__ __forceinline__ void build_cached4_coz(ge_cached3 tab[4], fe &zc, con
   scaled_cached3(tab[3], m3, l);
 }
 
-// |b| * P from the shared-Z table; b = 0 gives the identity (zc, zc, zc, 0)
-__device__ __forceinline__ void cached_select4_coz(ge_cached &t, const ge_cached3 tab[4], const fe &zc, int b) {
+// Entries of the shared-Z table parked in LDS: 27 words of an entry in
+// seven 16-byte chunks, chunk q at park[q * PARK_STRIDE] (one slot per lane
+// and entry, the address independent of any digit)
+constexpr int PARK_STRIDE = 256;
+constexpr int PARK_CHUNKS = 7;
+
+__device__ __forceinline__ void park_cached3(uint4 *park, const ge_cached3 &e) {
+  uint32_t w[28];
+#pragma unroll
+  for (int l = 0; l < 9; ++l) { w[l] = e.YpX.v[l]; w[9 + l] = e.YmX.v[l]; w[18 + l] = e.T2d.v[l]; }
+  w[27] = 0;
+#pragma unroll
+  for (int q = 0; q < PARK_CHUNKS; ++q) park[q * PARK_STRIDE] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
+// |b| * P from the shared-Z table with entries 1P..NR P in registers and
+// the others parked in LDS (park + (j - NR) * PARK_CHUNKS * PARK_STRIDE for
+// entry j, all read every window, whatever b); b = 0 gives the identity
+template <int NR>
+__device__ __forceinline__ void cached_select4_coz_park(ge_cached &t, const ge_cached3 tab[NR], const uint4 *park,
+                                                        const fe &zc, int b) {
   const uint32_t neg = (uint32_t)b >> 31;
   const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
   t.YpX = zc;
   t.YmX = zc;
   fe_0(t.T2d);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < NR; ++j) {
     const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
 #pragma unroll
     for (int l = 0; l < 9; ++l) {
@@ -298,8 +317,32 @@ __device__ __forceinline__ void cached_select4_coz(ge_cached &t, const ge_cached
       t.T2d.v[l] = m ? tab[j].T2d.v[l] : t.T2d.v[l];
     }
   }
+#pragma unroll
+  for (int j = NR; j < 4; ++j) {
+    const uint4 *e = park + (j - NR) * PARK_CHUNKS * PARK_STRIDE;
+    uint32_t w[28];
+#pragma unroll
+    for (int q = 0; q < PARK_CHUNKS; ++q) {
+      const uint4 x = e[q * PARK_STRIDE];
+      w[4 * q] = x.x; w[4 * q + 1] = x.y; w[4 * q + 2] = x.z; w[4 * q + 3] = x.w;
+    }
+    const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      t.YpX.v[l] = m ? w[l] : t.YpX.v[l];
+      t.YmX.v[l] = m ? w[9 + l] : t.YmX.v[l];
+      t.T2d.v[l] = m ? w[18 + l] : t.T2d.v[l];
+    }
+  }
   t.Z = zc;
   ge_cached_cneg(t, neg);
+}
+
+// entries NR..3 of the lane's table in its LDS slots, as read above
+template <int NR>
+__device__ __forceinline__ void park_entries(uint4 *park, const ge_cached3 tab[4]) {
+#pragma unroll
+  for (int j = NR; j < 4; ++j) park_cached3(park + (j - NR) * PARK_CHUNKS * PARK_STRIDE, tab[j]);
 }
 
 __device__ __forceinline__ void cached_select4(ge_cached &t, const ge_cached tab[4], int b) {
@@ -413,8 +456,14 @@ __device__ __forceinline__ int pop_hi4(uint32_t s[11], int p) {
 }
 
 // h = k p for a 256-bit scalar in radix 8 (85 windows of an add and three
-// doublings, then the last add), with the reference's result for k >= 2^255
-__device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const uint32_t k[8]) {
+// doublings, then the last add), with the reference's result for k >= 2^255;
+// the table's entries VB8_REG + 1 .. 4 (times P) live in the lane's LDS
+// slots from `park` (see park_cached3), which cuts the 256-register window
+// loop's spills from 92 to 12 bytes per lane (and its time by about 1 %)
+constexpr int VB8_REG = 2;
+constexpr int VB8_PARK_WORDS = (4 - VB8_REG) * PARK_CHUNKS * 4;     // per lane
+
+__device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const uint32_t k[8], uint4 *park) {
   uint32_t m[8], e[11];
   const uint32_t neg = scalar_effective(m, k);
   recode8(e, m);
@@ -429,6 +478,7 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
   ge_cached3 tab[4];
   fe zc;
   build_cached4_coz(tab, zc, p);
+  park_entries<VB8_REG>(park, tab);
   ge_p3_0(h);
   ge_cached c;
   ge_p1p1 r;
@@ -437,7 +487,7 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
   asm volatile("" : "+s"(plo));
 #pragma unroll 1
   for (int q = 87; q > plo; --q) {
-    cached_select4_coz(c, tab, zc, pop_hi4(e, q));
+    cached_select4_coz_park<VB8_REG>(c, tab, park, zc, pop_hi4(e, q));
     ge_add(r, h, c);
     ge_p1p1_to_p2(s, r);
     ge_p2_dbl(r, s); ge_p1p1_to_p2(s, r);
@@ -446,7 +496,7 @@ __device__ __forceinline__ void varbase_mul8(ge_p3 &h, const ge_p3 &p_in, const 
     ge_p1p1_to_p3(h, r);
   }
   pin_p3(h);
-  cached_select4_coz(c, tab, zc, pop_hi4(e, plo));
+  cached_select4_coz_park<VB8_REG>(c, tab, park, zc, pop_hi4(e, plo));
   ge_add(r, h, c);
   ge_p1p1_to_p3(h, r);
 }

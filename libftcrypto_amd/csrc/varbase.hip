@@ -27,17 +27,20 @@ __global__ void __launch_bounds__(TPB) k_varbase(PtView pts, const uint32_t *sc,
   store_p3(pts, i, h);
 }
 
-// the same in radix 8: a 4-entry table, two waves per SIMD (a few spilled
-// words), the reference's result for every scalar (varbase_mul8)
+// the same in radix 8: a 4-entry table (two entries in per-lane LDS slots,
+// 56 KiB per block), two waves per SIMD, the reference's result for every
+// scalar (varbase_mul8)
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_varbase_r8(PtView pts, const uint32_t *sc, size_t sc_stride, size_t n) {
+  static_assert(TPB == PARK_STRIDE, "one LDS slot per lane");
+  __shared__ uint4 park[VB8_PARK_WORDS / 4 * PARK_STRIDE];
   const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
   if (i >= n) return;
   uint32_t k[8];
   ld_words(k, sc + i * sc_stride);
   ge_p3 p, h;
   load_p3(p, pts, i);
-  varbase_mul8(h, p, k);
+  varbase_mul8(h, p, k, park + threadIdx.x);
   store_p3(pts, i, h);
 }
 
