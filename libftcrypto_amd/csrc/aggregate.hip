@@ -29,6 +29,7 @@
 #include <mutex>
 
 #include "kernels.h"
+#include "quad.h"
 
 namespace ft {
 
@@ -124,73 +125,20 @@ static hipError_t launch_chunk_scan(PtView v, size_t n, PtView pre, bool exclusi
 
 // ---------------------------------------------- block scan (short levels)
 //
-// A level of at most BSCAN_MAX points is scanned by blocks of 256 lanes, one
-// point per lane: six shuffle-and-add steps scan each wave, the four wave
-// totals meet in LDS, and block totals are scanned the same way (exclusive)
-// and added back.  That is about ten adds per point, against two for a chunk
-// level, but a chain about ten adds deep instead of two chains of 2^ch: the
-// short top levels of a scan are latency-bound, the long bottom ones not.
+// A level of at most BSCAN_MAX points is scanned by blocks of 256 points
+// (k_block_scan_q below): shuffle-and-add steps scan each wave, the wave
+// totals are scanned the same way, and block totals are scanned the same
+// way (exclusive) and added back.  That is about ten adds per point, against
+// two for a chunk level, but a chain about ten adds deep instead of two
+// chains of 2^ch: the short top levels of a scan are latency-bound, the
+// long bottom ones not.
 constexpr size_t BSCAN_MAX = (size_t)1 << 16;
 
-__device__ __forceinline__ void shfl_up_fe(fe &r, const fe &a, int d) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k) r.v[k] = (uint32_t)__shfl_up((int)a.v[k], (unsigned)d, 64);
-}
-
 __device__ __forceinline__ void shfl_up_p3(ge_p3 &r, const ge_p3 &a, int d) {
-  shfl_up_fe(r.X, a.X, d);
-  shfl_up_fe(r.Y, a.Y, d);
-  shfl_up_fe(r.Z, a.Z, d);
-  shfl_up_fe(r.T, a.T, d);
-}
-
-// in-block inclusive (exclusive) scan of v[0, m) in place; block totals to
-// tot when tot.w is set
-__global__ void __launch_bounds__(ATPB) k_block_scan(PtView v, size_t m, PtView tot, int exclusive) {
-  __shared__ ge_p3 wt[ATPB / 64];
-  const size_t i = blockIdx.x * (size_t)ATPB + threadIdx.x;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  ge_p3 x;
-  if (i < m) load_p3(x, v, i);
-  else ge_p3_0(x);
-#pragma unroll 1
-  for (int d = 1; d < 64; d <<= 1) {
-    ge_p3 y;
-    shfl_up_p3(y, x, d);
-    if (lane >= d) ge_p3_add(x, x, y);
-  }
-  if (lane == 63) wt[w] = x;
-  __syncthreads();
-  ge_p3 pre;                                     // sum of the earlier waves (w >= 1)
-  if (w > 0) {
-    pre = wt[0];
-#pragma unroll 1
-    for (int q = 1; q < w; ++q) ge_p3_add(pre, pre, wt[q]);
-    ge_p3_add(x, pre, x);
-  }
-  if (tot.w && threadIdx.x == ATPB - 1) store_p3(tot, blockIdx.x, x);
-  if (exclusive) {
-    ge_p3 y;
-    shfl_up_p3(y, x, 1);
-    if (lane == 0) {
-      if (w > 0) y = pre;
-      else ge_p3_0(y);
-    }
-    x = y;
-  }
-  if (i < m) store_p3(v, i, x);
-}
-
-// v[i] = pre[i / 256] + v[i] for the blocks after the first
-__global__ void __launch_bounds__(ATPB) k_block_add(PtView v, size_t m, PtView pre) {
-  const size_t b = blockIdx.x + 1;
-  const size_t i = b * ATPB + threadIdx.x;
-  if (i >= m) return;
-  ge_p3 x, p;
-  load_p3(x, v, i);
-  load_p3(p, pre, b);
-  ge_p3_add(x, p, x);
-  store_p3(v, i, x);
+  fe_shfl_up(r.X, a.X, d);
+  fe_shfl_up(r.Y, a.Y, d);
+  fe_shfl_up(r.Z, a.Z, d);
+  fe_shfl_up(r.T, a.T, d);
 }
 
 // points of scratch the block scan of m points needs (its block totals)
@@ -200,18 +148,151 @@ static size_t block_scan_points(size_t m) {
   return tot;
 }
 
-static hipError_t block_scan(PtView v, size_t m, bool exclusive, PtView work, hipStream_t s) {
+// ------------------------------------------- quad-lane scan (short levels)
+//
+// Levels with few chunks are latency-bound: each lane's chain of 2^ch adds
+// (and stores) is the critical path and the GPU holds far more waves than
+// the level has.  There a chunk walk goes to the four lanes of a quad (one
+// coordinate each, quad.h), whose point add takes three dependent
+// multiplies instead of nine, and the block scan of the top level scans 256
+// points per block of 256 quads.  Results are the same points as the
+// one-lane kernels'.  Four lanes per point also issue about 1.7 times the
+// instructions of one lane per point, so the quad form pays only where a
+// level is short: at 2^16 chunks of 16 the walk that stores every output is
+// 26 % faster on quads (96 against 130 us), the one that only sums them is
+// not (83 against 82 us) and stays on one lane per chunk; the block scan of
+// 2^16 points takes 36 + 33 us against 49 + 44.
+constexpr size_t QUAD_CHUNKS_MAX = (size_t)1 << 16;
+constexpr int QBTPB = 1024;                 // block-scan blocks: 256 quads
+
+__device__ __forceinline__ const uint32_t *coord(PtView v, int c, size_t i) { return v.w + (c * v.cap + i) * 8; }
+__device__ __forceinline__ uint32_t *coord_w(PtView v, int c, size_t i) { return v.w + (c * v.cap + i) * 8; }
+
+// walk chunk t on quad t from its exclusive prefix (identity when pre.w is
+// null), storing each output
+template <int CH>
+__global__ void __launch_bounds__(ATPB) k_chunk_scan_q(PtView v, size_t n, PtView pre, int exclusive) {
+  const int c = threadIdx.x & 3;
+  const size_t t = (blockIdx.x * (size_t)ATPB + threadIdx.x) >> 2;
+  const size_t lo = t << CH;
+  if (lo >= n) return;
+  const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
+  fe run, x;
+  if (pre.w) load_fe(run, coord(pre, c, t));
+  else quad_identity(run, c);
+  load_fe(x, coord(v, c, lo));
+#pragma unroll 1
+  for (size_t g = lo; g < hi; ++g) {
+    fe nx;
+    load_fe(nx, coord(v, c, g + 1 < hi ? g + 1 : g));
+    if (exclusive) store_fe(coord_w(v, c, g), run);
+    quad_add(run, x, c);
+    if (!exclusive) store_fe(coord_w(v, c, g), run);
+    x = nx;
+  }
+}
+
+static hipError_t launch_chunk_scan_q(PtView v, size_t n, PtView pre, bool exclusive, int ch, hipStream_t s) {
+  const size_t T = tiles(n, ch);
+  if (!T) return hipSuccess;
+  const dim3 grid((unsigned)((4 * T + ATPB - 1) / ATPB));
+  switch (ch) {
+    case 4: hipLaunchKernelGGL(k_chunk_scan_q<4>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 5: hipLaunchKernelGGL(k_chunk_scan_q<5>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 6: hipLaunchKernelGGL(k_chunk_scan_q<6>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// in-block inclusive (exclusive) scan of v[0, m) in place, 256 points per
+// block on its 256 quads: each wave scans its 16 quads (shuffles up by 4, 8,
+// 16, 32 lanes), wave 0 scans the 16 wave totals the same way, and each wave
+// adds the total of the waves before it: nine dependent adds per point.
+// Block totals to tot when tot.w is set.
+__global__ void __launch_bounds__(QBTPB) k_block_scan_q(PtView v, size_t m, PtView tot, int exclusive) {
+  constexpr int NW = QBTPB / 64;
+  __shared__ uint32_t wt[NW * 4 * 9];
+  const int c = threadIdx.x & 3, q = threadIdx.x >> 2;
+  const int wq = q & 15, w = q >> 4;
+  const size_t i = blockIdx.x * (size_t)(QBTPB / 4) + q;
+  fe x;
+  if (i < m) load_fe(x, coord(v, c, i));
+  else quad_identity(x, c);
+#pragma unroll 1
+  for (int d = 1; d < 16; d <<= 1) {
+    fe y, z;
+    fe_shfl_up(y, x, 4 * d);
+    z = x;
+    quad_add(z, y, c);
+#pragma unroll
+    for (int l = 0; l < 9; ++l) x.v[l] = wq >= d ? z.v[l] : x.v[l];
+  }
+  if (wq == 15) {
+#pragma unroll
+    for (int l = 0; l < 9; ++l) wt[(w * 4 + c) * 9 + l] = x.v[l];
+  }
+  __syncthreads();
+  if (w == 0) {
+    fe t;
+#pragma unroll
+    for (int l = 0; l < 9; ++l) t.v[l] = wt[(wq * 4 + c) * 9 + l];
+#pragma unroll 1
+    for (int d = 1; d < 16; d <<= 1) {
+      fe y, z;
+      fe_shfl_up(y, t, 4 * d);
+      z = t;
+      quad_add(z, y, c);
+#pragma unroll
+      for (int l = 0; l < 9; ++l) t.v[l] = wq >= d ? z.v[l] : t.v[l];
+    }
+#pragma unroll
+    for (int l = 0; l < 9; ++l) wt[(wq * 4 + c) * 9 + l] = t.v[l];    // waves 0 .. wq
+  }
+  __syncthreads();
+  fe pre;                                        // total of the earlier waves
+  if (w > 0) {
+#pragma unroll
+    for (int l = 0; l < 9; ++l) pre.v[l] = wt[((w - 1) * 4 + c) * 9 + l];
+    quad_add(x, pre, c);
+  } else {
+    quad_identity(pre, c);
+  }
+  if (tot.w && q == QBTPB / 4 - 1) store_fe(coord_w(tot, c, blockIdx.x), x);
+  if (exclusive) {
+    fe y;
+    fe_shfl_up(y, x, 4);
+#pragma unroll
+    for (int l = 0; l < 9; ++l) x.v[l] = wq == 0 ? pre.v[l] : y.v[l];
+  }
+  if (i < m) store_fe(coord_w(v, c, i), x);
+}
+
+// v[i] = pre[b] + v[i] for the points of blocks b >= 1 (256 per block)
+__global__ void __launch_bounds__(QBTPB) k_block_add_q(PtView v, size_t m, PtView pre) {
+  const int c = threadIdx.x & 3;
+  const size_t b = blockIdx.x + 1;
+  const size_t i = b * (QBTPB / 4) + (threadIdx.x >> 2);
+  if (i >= m) return;
+  fe x, p;
+  load_fe(x, coord(v, c, i));
+  load_fe(p, coord(pre, c, b));
+  quad_add(x, p, c);
+  store_fe(coord_w(v, c, i), x);
+}
+
+static hipError_t block_scan_q(PtView v, size_t m, bool exclusive, PtView work, hipStream_t s) {
   const size_t nb = tiles(m, 8);
   if (nb <= 1) {
-    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(ATPB), 0, s, v, m, PtView{nullptr, 0}, (int)exclusive);
+    hipLaunchKernelGGL(k_block_scan_q, dim3(1), dim3(QBTPB), 0, s, v, m, PtView{nullptr, 0}, (int)exclusive);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nb), dim3(ATPB), 0, s, v, m, work, (int)exclusive);
+  hipLaunchKernelGGL(k_block_scan_q, dim3((unsigned)nb), dim3(QBTPB), 0, s, v, m, work, (int)exclusive);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = block_scan(work, nb, true, PtView{work.w + nb * 8, work.cap}, s);
+  e = block_scan_q(work, nb, true, PtView{work.w + nb * 8, work.cap}, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_block_add, dim3((unsigned)(nb - 1)), dim3(ATPB), 0, s, v, m, work);
+  hipLaunchKernelGGL(k_block_add_q, dim3((unsigned)(nb - 1)), dim3(QBTPB), 0, s, v, m, work);
   return hipGetLastError();
 }
 
@@ -259,9 +340,12 @@ hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, in
     }
     ++L;
   }
-  hipError_t e = block_scan(lv[L], ln[L], L > 0 ? true : exclusive, PtView{scratch.w + used * 8, scratch.cap}, s);
-  for (int l = L - 1; l >= 0 && e == hipSuccess; --l)
-    e = launch_chunk_scan(lv[l], ln[l], lv[l + 1], l > 0 ? true : exclusive, ch, s);
+  hipError_t e = block_scan_q(lv[L], ln[L], L > 0 ? true : exclusive, PtView{scratch.w + used * 8, scratch.cap}, s);
+  for (int l = L - 1; l >= 0 && e == hipSuccess; --l) {
+    const bool ex = l > 0 ? true : exclusive;
+    e = ln[l + 1] <= QUAD_CHUNKS_MAX ? launch_chunk_scan_q(lv[l], ln[l], lv[l + 1], ex, ch, s)
+                                     : launch_chunk_scan(lv[l], ln[l], lv[l + 1], ex, ch, s);
+  }
   return e;
 }
 
