@@ -264,7 +264,7 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
  * the first reduce pass; u32_window
  * (ciphertext x u32) and varbase_window (ft_mul, ft_scale) are 4 (radix-16
  * digits, 8-entry table) or 3 (radix 8, 4 entries); lanes per ciphertext of
- * an encryption (ftx_tune_get_ct_lanes) is 1 or 2.  Results are the same
+ * an encryption (ftx_tune_get_ct_lanes) is 1, 2 or 4.  Results are the same
  * under every schedule.
  * ftx_autotune times every candidate for classes 0..max_class on the device
  * (class 3 at n = 2^24, about 3.8 GB while it runs), keeps the fastest and

@@ -215,6 +215,25 @@ k_encrypt_split_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t 
   encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
 }
 
+// four lanes per ciphertext (encrypt_quarter) in 512-lane blocks at four
+// waves per SIMD: for batches that two lanes per ciphertext leave at two
+// waves per SIMD or fewer
+template <bool ACC>
+__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_encrypt_quarter(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
+                  const uint32_t *r, size_t n, const unsigned long long *gate) {
+  if (gated(gate)) return;
+  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
+  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
+  lds_load_table(tb, btab);
+  lds_load_table(th, htab);
+  __syncthreads();
+  const size_t t = blockIdx.x * (size_t)ETPB + threadIdx.x;
+  const size_t i = t >> 2;
+  if (i >= n) return;                     // whole groups of four (ETPB is a multiple of 4)
+  encrypt_quarter<ACC>(c1, c2, i, (uint32_t)(t & 3), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
+}
+
 // Several ciphertext arrays encrypted in one launch, two lanes per
 // ciphertext, the jobs' ciphertexts laid end to end from multiples of 32 (a
 // wave's 32 ciphertexts), so that the job of a wave is one: lane pair t finds
@@ -562,7 +581,8 @@ hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_b
   FT_LAUNCH_CHECK();
 }
 
-// lanes: 1 (one lane per ciphertext) or 2 (C1 and C2 on separate lanes)
+// lanes: 1 (one lane per ciphertext), 2 (C1 and C2 on separate lanes) or
+// 4 (each comb's odd and even digits on separate lanes as well)
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
                             hipStream_t s, const unsigned long long *gate) {
@@ -591,6 +611,13 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
       hipLaunchKernelGGL(k_encrypt_split<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
     else
       hipLaunchKernelGGL(k_encrypt_split<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
+  } else if (lanes == 4) {
+    static_assert(ETPB % 4 == 0, "whole groups of four lanes per block");
+    const dim3 grid(grid_for(4 * n, ETPB));
+    if (accumulate)
+      hipLaunchKernelGGL(k_encrypt_quarter<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
+    else
+      hipLaunchKernelGGL(k_encrypt_quarter<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else {
     return hipErrorInvalidValue;
   }

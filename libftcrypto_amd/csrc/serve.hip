@@ -496,7 +496,9 @@ void launch(ftx_server_s *S, int idx, std::vector<Piece> &batch) {
   const unsigned grid = (unsigned)((start + TPB - 1) / TPB);
   if (e == hipSuccess && start > 0) {
     if (kind == K_ENCRYPT) {
-      const bool split = tune_for(start).encl == 2;
+      // two lanes per ciphertext for a two- or four-lane schedule (the
+      // batched kernels have no four-lane form)
+      const bool split = tune_for(start).encl >= 2;
       if (!split && start >= ENC_WIDE_MIN)
         hipLaunchKernelGGL(kb_encrypt_wide, dim3((unsigned)((start + ETPB - 1) / ETPB)), dim3(ETPB), 0, sl.st,
                            ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);

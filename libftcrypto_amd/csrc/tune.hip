@@ -15,9 +15,11 @@
 //   scanlb  scan form: 0 chunk levels (a walk summing chunks, upper levels,
 //           a second walk from each chunk's prefix), 1 single pass with a
 //           decoupled look-back over 8,192-point tiles
-//   encl    lanes per ciphertext of an encryption or rerandomisation: 1, or
+//   encl    lanes per ciphertext of an encryption or rerandomisation: 1,
 //           2 (C1 on one lane, C2 on the next; twice the waves of a small
-//           batch, each lane with one comb pass).  The nine limbs of a field
+//           batch, each lane with one comb pass) or 4 (each comb's odd and
+//           even digits on two lanes, joined by four doublings and an add:
+//           twice the waves again, 5 % more work).  The nine limbs of a field
 //           element always stay in one lane: splitting them would exchange
 //           partial column sums between lanes inside every multiply.
 // Every schedule gives the same group element (point addition is
@@ -77,7 +79,7 @@ bool valid(const AggTune &t) {
   auto ch_ok = [](int c) { return c >= CH_MIN && c <= CH_MAX; };
   return ch_ok(t.seg1) && t.segu >= TREE_LR_MIN && t.segu <= TREE_LR_MAX && ch_ok(t.scan) && t.reduce >= 1 &&
          (unsigned)t.reduce <= REDUCE_BLOCKS_MAX && (t.u32w == 3 || t.u32w == 4) &&
-         (t.vbw == 3 || t.vbw == 4) && (t.encl == 1 || t.encl == 2) && (t.scanlb == 0 || t.scanlb == 1);
+         (t.vbw == 3 || t.vbw == 4) && (t.encl == 1 || t.encl == 2 || t.encl == 4) && (t.scanlb == 0 || t.scanlb == 1);
 }
 
 void tune_store(int c, const AggTune &t) {
@@ -200,7 +202,7 @@ ft_error ftx_tune_get_ct_lanes(int size_class, int *lanes) {
 
 ft_error ftx_tune_set_ct_lanes(int size_class, int lanes) {
   if (size_class < 0 || size_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
-  if (lanes != 1 && lanes != 2) return FT_ERR_BAD_SIZE;
+  if (lanes != 1 && lanes != 2 && lanes != 4) return FT_ERR_BAD_SIZE;
   if (!T_ready) tune_defaults();
   T_encl[size_class] = lanes;
   return FT_ERR_NO_ERROR;
@@ -392,10 +394,11 @@ ft_error ftx_autotune(int max_class, const char *path) {
       if (t < bt) { bt = t; best.vbw = w; }
     }
     // encryption with one lane per ciphertext against two (C1 and C2 on
-    // separate lanes); B's comb table for both keys and arbitrary r (any
-    // scalars time the same: the combs are constant-time)
+    // separate lanes) and four (each comb split in two); B's comb table for
+    // both keys and arbitrary r (any scalars time the same: the combs are
+    // constant-time)
     bt = INFINITY;
-    for (int l = 1; l <= 2; ++l) {
+    for (int l = 1; l <= 4; l *= 2) {
       const size_t ne = std::min(n, ENC_TUNE_MAX);
       ft_error e = time_ms(&t, ev, st.s, c < 2 ? 3 : 1, [&] {
         return launch_encrypt_l(E1, E2, base_table(), base_table(), (const uint64_t *)sc.p, 1,

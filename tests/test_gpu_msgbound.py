@@ -65,7 +65,7 @@ def check_sample(c1, c2, H, m, r, seed):
     assert np.array_equal(e2, o2.to_bytes())
 
 
-@pytest.fixture(params=[1, 2])
+@pytest.fixture(params=[1, 2, 4])
 def lanes(request):
     """lanes per ciphertext of every size class, restored afterwards"""
     L = ft.lib()

@@ -246,9 +246,10 @@ def test_autotune_keeps_results(tmp_path):
 
 @pytest.mark.parametrize("n", [1000, 300000])
 def test_every_ct_lanes(n):
-    """Encryption and rerandomisation with one lane per ciphertext and with
-    two (C1 and C2 on separate lanes) against the oracle; at 300,000 the
-    one-lane form is the 512-lane-block kernel"""
+    """Encryption and rerandomisation with one lane per ciphertext, with two
+    (C1 and C2 on separate lanes) and with four (each comb's odd and even
+    digits apart) against the oracle; at 300,000 the one-lane form is the
+    512-lane-block kernel"""
     c = ft.size_class(n)
     before = ft.ct_lanes_table()[c]
     x = ft.random_scalars(1, 4800, 0)
@@ -262,7 +263,7 @@ def test_every_ct_lanes(n):
     q1, q2 = o1.copy(), o2.copy()
     O.rerandomize(q1, q2, H, rho)
     try:
-        for lanes in (1, 2):
+        for lanes in (1, 2, 4):
             ft.tune_set_ct_lanes(c, lanes)
             g1, g2 = ft.encrypt(key, m, r, msg_bits=8)
             same(g1, o1)
