@@ -61,6 +61,14 @@ FT_HD void fe_neg(fe &r, const fe &a) {
   for (int i = 1; i < 9; ++i) r.v[i] = KI - a.v[i];
 }
 
+// r = neg ? K - b : b for a 0 / all-ones mask (x ^ ~0 = -1 - x, so adding
+// K + 1 gives K - x modulo 2^32); b of class N
+FT_HD void fe_cneg_mask(fe &r, const fe &b, uint32_t mask) {
+  r.v[0] = (b.v[0] ^ mask) + (mask & (K0 + 1u));
+#pragma unroll
+  for (int l = 1; l < 9; ++l) r.v[l] = (b.v[l] ^ mask) + (mask & (KI + 1u));
+}
+
 // carry to class N; limbs up to 2^32 - 16 (callers pass at most 2^31.4)
 FT_HD void fe_carry(fe &r) {
 #pragma unroll

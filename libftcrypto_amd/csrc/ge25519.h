@@ -139,19 +139,27 @@ FT_HD void ge_dbl_add_public(ge_p2 &s, int d, const ge_cached &c1, const ge_cach
   ge_p1p1_to_p2(s, r);
 }
 
-// -q for a cached point: swap Y+X and Y-X, negate 2dT (T2d of class N)
+// -q for a cached point when neg = 1: swap Y+X and Y-X (two selects per
+// limb), negate 2dT (T2d of class N) as (x ^ mask) + (mask & (K + 1)), one
+// instruction per limb; the values are those of fe_cswap and fe_neg
 FT_HD void ge_cached_cneg(ge_cached &q, uint32_t neg) {
-  fe n;
-  fe_cswap(q.YpX, q.YmX, neg);
-  fe_neg(n, q.T2d);
-  fe_cmov(q.T2d, n, neg);
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const uint32_t a = q.YpX.v[l], b = q.YmX.v[l];
+    q.YpX.v[l] = neg ? b : a;
+    q.YmX.v[l] = neg ? a : b;
+  }
+  fe_cneg_mask(q.T2d, q.T2d, 0u - neg);
 }
 
 FT_HD void ge_precomp_cneg(ge_precomp &q, uint32_t neg) {
-  fe n;
-  fe_cswap(q.ypx, q.ymx, neg);
-  fe_neg(n, q.xy2d);
-  fe_cmov(q.xy2d, n, neg);
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const uint32_t a = q.ypx.v[l], b = q.ymx.v[l];
+    q.ypx.v[l] = neg ? b : a;
+    q.ymx.v[l] = neg ? a : b;
+  }
+  fe_cneg_mask(q.xy2d, q.xy2d, 0u - neg);
 }
 
 // full p3 + p3 -> p3 (9M)

@@ -29,14 +29,6 @@ constexpr int QP_SWAP = 1 | (0 << 2) | (3 << 4) | (2 << 6);   // [1, 0, 3, 2]
 constexpr int QP_OP1 = 0 | (2 << 2) | (2 << 4) | (0 << 6);    // [0, 2, 2, 0]: E G G E
 constexpr int QP_OP2 = 3 | (1 << 2) | (3 << 4) | (1 << 6);    // [3, 1, 3, 1]: F H F H
 
-// r = neg ? K - b : b for a 0 / all-ones mask (x ^ ~0 = -1 - x, so adding
-// K + 1 gives K - x modulo 2^32); b of class N
-__device__ __forceinline__ void fe_cneg_mask(fe &r, const fe &b, uint32_t mask) {
-  r.v[0] = (b.v[0] ^ mask) + (mask & (K0 + 1u));
-#pragma unroll
-  for (int l = 1; l < 9; ++l) r.v[l] = (b.v[l] ^ mask) + (mask & (KI + 1u));
-}
-
 // the first factor of step 1 in lane c from coordinate m of the lane and n
 // of its pair lane (c ^ 1): Y - X (c = 0: n = Y, m = X), Y + X (c = 1), T
 // (c = 2: n = T) or Z (c = 3: n = Z)
