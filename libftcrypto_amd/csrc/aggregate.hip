@@ -71,30 +71,87 @@ __device__ __forceinline__ void scan_step(ge_p3 &run, PW &q, int exclusive) {
   if (!exclusive) pk_pw(q, run);
 }
 
+// Lane-transposed store of one coordinate of a group: every lane of the wave
+// holds one whole 128-byte line (its four points' 32-byte pieces); the lines
+// go through LDS so that each store instruction writes eight whole lines
+// (eight lanes per line) instead of a 16-byte piece of 64 different lines.
+// Slot (line L, piece p) sits at L * 8 + (p ^ ((L >> 1) & 7)): conflict-free
+// for the line writes and the piece reads.
+template <int CH>
+__device__ __forceinline__ void st_line_tr(uint4 *tr, uint8_t *dst, int lane, const uint32_t *a, const uint32_t *b,
+                                           const uint32_t *c, const uint32_t *d) {
+  const int sw = (lane >> 1) & 7;
+  uint4 *row = tr + lane * 8;
+  row[0 ^ sw] = make_uint4(a[0], a[1], a[2], a[3]);
+  row[1 ^ sw] = make_uint4(a[4], a[5], a[6], a[7]);
+  row[2 ^ sw] = make_uint4(b[0], b[1], b[2], b[3]);
+  row[3 ^ sw] = make_uint4(b[4], b[5], b[6], b[7]);
+  row[4 ^ sw] = make_uint4(c[0], c[1], c[2], c[3]);
+  row[5 ^ sw] = make_uint4(c[4], c[5], c[6], c[7]);
+  row[6 ^ sw] = make_uint4(d[0], d[1], d[2], d[3]);
+  row[7 ^ sw] = make_uint4(d[4], d[5], d[6], d[7]);
+  __builtin_amdgcn_wave_barrier();
+  const int pc = lane & 7;
+  const size_t off = ((size_t)(lane >> 3) << (CH + 5)) + (size_t)pc * 16;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int L = 8 * k + (lane >> 3);
+    const uint4 x = tr[L * 8 + (pc ^ ((L >> 1) & 7))];
+    *reinterpret_cast<uint4 *>(dst + off + ((size_t)(8 * k) << (CH + 5))) = x;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int CH, bool TR>
+__device__ __forceinline__ void chunk_walk(PtView v, size_t lo, size_t hi, ge_p3 &run, int exclusive, uint4 *tr,
+                                           size_t wave0, int lane) {
+#pragma unroll 1
+  for (size_t gi = lo; gi < hi; gi += 4) {
+    // an opaque copy of the element index: the plane addresses are then
+    // formed each group from it and the uniform plane bases
+    size_t g = gi;
+    asm volatile("" : "+v"(g));
+    PW q0, q1, q2, q3;
+    FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    scan_step(run, q0, exclusive);
+    if (TR || g + 1 < hi) scan_step(run, q1, exclusive);
+    if (TR || g + 2 < hi) scan_step(run, q2, exclusive);
+    if (TR || g + 3 < hi) scan_step(run, q3, exclusive);
+    if (TR) {
+      uint8_t *base = reinterpret_cast<uint8_t *>(v.w) + ((wave0 << CH) + (g - lo)) * 32;
+      const size_t plane = v.cap * 32;
+      st_line_tr<CH>(tr, base, lane, q0.x, q1.x, q2.x, q3.x);
+      st_line_tr<CH>(tr, base + plane, lane, q0.y, q1.y, q2.y, q3.y);
+      st_line_tr<CH>(tr, base + 2 * plane, lane, q0.z, q1.z, q2.z, q3.z);
+      st_line_tr<CH>(tr, base + 3 * plane, lane, q0.t, q1.t, q2.t, q3.t);
+    } else {
+      st_pw4(v, g, (int)(hi - g < 4 ? hi - g : 4), q0, q1, q2, q3);
+    }
+  }
+}
+
 // walk chunk t from its exclusive prefix (identity when pre.w is null); the
-// four outputs of a group are written together, each coordinate's line by
-// consecutive stores (7 % faster than point by point).
-// Two waves per SIMD (a few spilled words) beat one wave without spills.
+// four outputs of a group are written together.  A wave whose 64 chunks are
+// all whole writes them through LDS, a whole 128-byte line per eight lanes
+// and store; the array's last wave stores each coordinate's line by
+// consecutive stores of its lane.
 template <int CH>
 __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
+  __shared__ uint4 trs[ATPB / 64][64 * 8];
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const size_t wave0 = t - (size_t)lane;
   const size_t lo = t << CH;
   if (lo >= n) return;
   const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
   ge_p3 run;
   if (pre.w) load_p3(run, pre, t);
   else ge_p3_0(run);
-#pragma unroll 1
-  for (size_t g = lo; g < hi; g += 4) {
-    PW q0, q1, q2, q3;
-    FT_LOAD4(q0, q1, q2, q3, v, g, hi);
-    scan_step(run, q0, exclusive);
-    if (g + 1 < hi) scan_step(run, q1, exclusive);
-    if (g + 2 < hi) scan_step(run, q2, exclusive);
-    if (g + 3 < hi) scan_step(run, q3, exclusive);
-    st_pw4(v, g, (int)(hi - g < 4 ? hi - g : 4), q0, q1, q2, q3);
-  }
+  if (((wave0 + 64) << CH) <= n)   // wave-uniform: all 64 chunks whole
+    chunk_walk<CH, true>(v, lo, hi, run, exclusive, trs[threadIdx.x >> 6], wave0, lane);
+  else
+    chunk_walk<CH, false>(v, lo, hi, run, exclusive, nullptr, wave0, lane);
 }
 
 static hipError_t launch_chunk_sums(PtView sums, PtView v, size_t n, int ch, hipStream_t s) {
@@ -554,7 +611,13 @@ k_scan_lookback(PtView v, size_t n, int exclusive, PtView agg, PtView inc, unsig
     ge_p3_add(run, run, t);
   }
 #pragma unroll 1
-  for (size_t g = lo; g < hi; g += 4) {
+  for (size_t gi = lo; gi < hi; gi += 4) {
+    // an opaque copy of the element index: the plane addresses are then
+    // formed each group from it and the uniform plane bases, instead of
+    // four per-lane plane pointers kept across the loop (which spill, and
+    // whose reloads wait on the previous group's stores)
+    size_t g = gi;
+    asm volatile("" : "+v"(g));
     PW q0, q1, q2, q3;
     FT_LOAD4(q0, q1, q2, q3, v, g, hi);
     scan_step(run, q0, exclusive);
