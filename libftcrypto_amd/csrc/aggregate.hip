@@ -39,26 +39,92 @@ static size_t tiles(size_t n, int ch) { return (n + ((size_t)1 << ch) - 1) >> ch
 
 // ------------------------------------------------------------------ scan
 
-// sums[t] = sum of elements [2^CH t, 2^CH (t + 1)), read a cache line at a time
+// Lane-transposed load of one coordinate of a group for a wave of whole
+// chunks: eight coalesced loads (eight lanes per 128-byte line, eight whole
+// lines per load) land in r[0..7]; through LDS every lane then gets its own
+// line (its four points' 32-byte pieces) back in r.
 template <int CH>
-__global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size_t n) {
-  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
-  const size_t lo = t << CH;
-  if (lo >= n) return;
-  const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
-  ge_p3 acc, x;
-  ge_p3_0(acc);
+__device__ __forceinline__ void ld_line_issue(uint4 r[8], const uint8_t *src, int lane) {
+  const size_t off = ((size_t)(lane >> 3) << (CH + 5)) + (size_t)(lane & 7) * 16;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = *reinterpret_cast<const uint4 *>(src + off + ((size_t)(8 * k) << (CH + 5)));
+}
+__device__ __forceinline__ void ld_line_tr(uint4 r[8], uint4 *tr, int lane) {
+  const int pc = lane & 7;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int L = 8 * k + (lane >> 3);
+    tr[L * 8 + (pc ^ ((L >> 1) & 7))] = r[k];
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int sw = (lane >> 1) & 7;
+#pragma unroll
+  for (int p = 0; p < 8; ++p) r[p] = tr[lane * 8 + (p ^ sw)];
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ void line_to_pw(PW &a, PW &b, PW &c, PW &d, int co, const uint4 r[8]) {
+  uint32_t *pa = co == 0 ? a.x : co == 1 ? a.y : co == 2 ? a.z : a.t;
+  uint32_t *pb = co == 0 ? b.x : co == 1 ? b.y : co == 2 ? b.z : b.t;
+  uint32_t *pc = co == 0 ? c.x : co == 1 ? c.y : co == 2 ? c.z : c.t;
+  uint32_t *pd = co == 0 ? d.x : co == 1 ? d.y : co == 2 ? d.z : d.t;
+  pa[0] = r[0].x; pa[1] = r[0].y; pa[2] = r[0].z; pa[3] = r[0].w;
+  pa[4] = r[1].x; pa[5] = r[1].y; pa[6] = r[1].z; pa[7] = r[1].w;
+  pb[0] = r[2].x; pb[1] = r[2].y; pb[2] = r[2].z; pb[3] = r[2].w;
+  pb[4] = r[3].x; pb[5] = r[3].y; pb[6] = r[3].z; pb[7] = r[3].w;
+  pc[0] = r[4].x; pc[1] = r[4].y; pc[2] = r[4].z; pc[3] = r[4].w;
+  pc[4] = r[5].x; pc[5] = r[5].y; pc[6] = r[5].z; pc[7] = r[5].w;
+  pd[0] = r[6].x; pd[1] = r[6].y; pd[2] = r[6].z; pd[3] = r[6].w;
+  pd[4] = r[7].x; pd[5] = r[7].y; pd[6] = r[7].z; pd[7] = r[7].w;
+}
+
+template <int CH, bool TR>
+__device__ __forceinline__ void chunk_sum_walk(ge_p3 &acc, PtView v, size_t lo, size_t hi, uint4 *tr, size_t wave0,
+                                               int lane) {
+  ge_p3 x;
 #pragma unroll 1
-  for (size_t g = lo; g < hi; g += 4) {
+  for (size_t gi = lo; gi < hi; gi += 4) {
+    size_t g = gi;
+    asm volatile("" : "+v"(g));
     PW q0, q1, q2, q3;
-    FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    if (TR) {
+      const uint8_t *base = reinterpret_cast<const uint8_t *>(v.w) + ((wave0 << CH) + (g - lo)) * 32;
+      const size_t plane = v.cap * 32;
+      uint4 r[4][8];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ld_line_issue<CH>(r[c], base + c * plane, lane);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ld_line_tr(r[c], tr, lane);
+        line_to_pw(q0, q1, q2, q3, c, r[c]);
+      }
+    } else {
+      FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    }
     un_pw(x, q0);
     if (g == lo) acc = x;
     else ge_p3_add(acc, acc, x);
-    if (g + 1 < hi) { un_pw(x, q1); ge_p3_add(acc, acc, x); }
-    if (g + 2 < hi) { un_pw(x, q2); ge_p3_add(acc, acc, x); }
-    if (g + 3 < hi) { un_pw(x, q3); ge_p3_add(acc, acc, x); }
+    if (TR || g + 1 < hi) { un_pw(x, q1); ge_p3_add(acc, acc, x); }
+    if (TR || g + 2 < hi) { un_pw(x, q2); ge_p3_add(acc, acc, x); }
+    if (TR || g + 3 < hi) { un_pw(x, q3); ge_p3_add(acc, acc, x); }
   }
+}
+
+// sums[t] = sum of elements [2^CH t, 2^CH (t + 1)), read a cache line at a time
+template <int CH>
+__global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size_t n) {
+  __shared__ uint4 trs[ATPB / 64][64 * 8];
+  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const size_t wave0 = t - (size_t)lane;
+  const size_t lo = t << CH;
+  if (lo >= n) return;
+  const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
+  ge_p3 acc;
+  ge_p3_0(acc);
+  if (((wave0 + 64) << CH) <= n)   // wave-uniform: all 64 chunks whole
+    chunk_sum_walk<CH, true>(acc, v, lo, hi, trs[threadIdx.x >> 6], wave0, lane);
+  else
+    chunk_sum_walk<CH, false>(acc, v, lo, hi, nullptr, wave0, lane);
   store_p3(sums, t, acc);
 }
 
@@ -112,18 +178,30 @@ __device__ __forceinline__ void chunk_walk(PtView v, size_t lo, size_t hi, ge_p3
     size_t g = gi;
     asm volatile("" : "+v"(g));
     PW q0, q1, q2, q3;
-    FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    const uint8_t *base = reinterpret_cast<const uint8_t *>(v.w) + ((wave0 << CH) + (g - lo)) * 32;
+    const size_t plane = v.cap * 32;
+    if (TR) {
+      uint4 r[4][8];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ld_line_issue<CH>(r[c], base + c * plane, lane);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ld_line_tr(r[c], tr, lane);
+        line_to_pw(q0, q1, q2, q3, c, r[c]);
+      }
+    } else {
+      FT_LOAD4(q0, q1, q2, q3, v, g, hi);
+    }
     scan_step(run, q0, exclusive);
     if (TR || g + 1 < hi) scan_step(run, q1, exclusive);
     if (TR || g + 2 < hi) scan_step(run, q2, exclusive);
     if (TR || g + 3 < hi) scan_step(run, q3, exclusive);
     if (TR) {
-      uint8_t *base = reinterpret_cast<uint8_t *>(v.w) + ((wave0 << CH) + (g - lo)) * 32;
-      const size_t plane = v.cap * 32;
-      st_line_tr<CH>(tr, base, lane, q0.x, q1.x, q2.x, q3.x);
-      st_line_tr<CH>(tr, base + plane, lane, q0.y, q1.y, q2.y, q3.y);
-      st_line_tr<CH>(tr, base + 2 * plane, lane, q0.z, q1.z, q2.z, q3.z);
-      st_line_tr<CH>(tr, base + 3 * plane, lane, q0.t, q1.t, q2.t, q3.t);
+      uint8_t *out = const_cast<uint8_t *>(base);
+      st_line_tr<CH>(tr, out, lane, q0.x, q1.x, q2.x, q3.x);
+      st_line_tr<CH>(tr, out + plane, lane, q0.y, q1.y, q2.y, q3.y);
+      st_line_tr<CH>(tr, out + 2 * plane, lane, q0.z, q1.z, q2.z, q3.z);
+      st_line_tr<CH>(tr, out + 3 * plane, lane, q0.t, q1.t, q2.t, q3.t);
     } else {
       st_pw4(v, g, (int)(hi - g < 4 ? hi - g : 4), q0, q1, q2, q3);
     }
@@ -355,10 +433,16 @@ static hipError_t block_scan_q(PtView v, size_t m, bool exclusive, PtView work, 
 
 // scratch of a scan of n points with chunks of 2^ch: the chunk sums of every
 // level longer than BSCAN_MAX, then the block totals of the top level
+// Chunk size of level l of a scan whose first level has 2^ch points per
+// lane: the levels above the first are short and latency-bound (a lane's
+// chain of 2^ch dependent adds sets their time), so they take the smallest
+// chunks (2^18 points at 64 per lane: 250 us; at 16 per lane: about 60).
+static int scan_level_ch(int l, int ch) { return l == 0 ? ch : CH_MIN; }
+
 static size_t scan_plan_points(size_t n, int ch) {
   size_t used = 0, m = n;
-  while (m > BSCAN_MAX) {
-    m = tiles(m, ch);
+  for (int l = 0; m > BSCAN_MAX; ++l) {
+    m = tiles(m, scan_level_ch(l, ch));
     used += m;
   }
   return used + block_scan_points(m);
@@ -387,12 +471,12 @@ hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, in
   ln[0] = n;
   size_t used = 0;
   while (ln[L] > BSCAN_MAX) {
-    const size_t m = tiles(ln[L], ch);
+    const size_t m = tiles(ln[L], scan_level_ch(L, ch));
     lv[L + 1] = PtView{scratch.w + used * 8, scratch.cap};
     ln[L + 1] = m;
     used += m;
     if (L > 0 || !level1_done) {
-      hipError_t e = launch_chunk_sums(lv[L + 1], lv[L], ln[L], ch, s);
+      hipError_t e = launch_chunk_sums(lv[L + 1], lv[L], ln[L], scan_level_ch(L, ch), s);
       if (e != hipSuccess) return e;
     }
     ++L;
@@ -400,8 +484,9 @@ hipError_t scan_launch_ch(PtView v, size_t n, bool exclusive, PtView scratch, in
   hipError_t e = block_scan_q(lv[L], ln[L], L > 0 ? true : exclusive, PtView{scratch.w + used * 8, scratch.cap}, s);
   for (int l = L - 1; l >= 0 && e == hipSuccess; --l) {
     const bool ex = l > 0 ? true : exclusive;
-    e = ln[l + 1] <= QUAD_CHUNKS_MAX ? launch_chunk_scan_q(lv[l], ln[l], lv[l + 1], ex, ch, s)
-                                     : launch_chunk_scan(lv[l], ln[l], lv[l + 1], ex, ch, s);
+    const int c = scan_level_ch(l, ch);
+    e = ln[l + 1] <= QUAD_CHUNKS_MAX ? launch_chunk_scan_q(lv[l], ln[l], lv[l + 1], ex, c, s)
+                                     : launch_chunk_scan(lv[l], ln[l], lv[l + 1], ex, c, s);
   }
   return e;
 }
@@ -688,10 +773,12 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
 // MODE 0: elements in order; 1: gathered from the planes of src; 2:
 // gathered from the point-contiguous copy `aos`.  Two waves per SIMD (the
 // gather forms would otherwise take a few registers over 256).
-template <int CH, int MODE, class OffT>
+// TR (MODE 0, a wave of 64 whole chunks starting at chunk wave0): each
+// group's points arrive by lane-transposed loads through the wave's LDS area tr.
+template <int CH, int MODE, class OffT, bool TR = false>
 __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, PtView src, size_t n,
                                          const OffT off, size_t nseg, const int32_t *gather, PtView aos,
-                                         size_t aos_base, size_t t) {
+                                         size_t aos_base, size_t t, uint4 *tr = nullptr, size_t wave0 = 0) {
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
   const int64_t chi = clo + (1 << CH);
@@ -790,7 +877,21 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
     for (int64_t g = clo; g < cend; g += 4) {
       refill();
       PW q0, q1, q2, q3;
-      FT_LOAD4(q0, q1, q2, q3, src, (size_t)g, (size_t)cend);
+      if (TR) {
+        const int lane = (int)(t - wave0);
+        const uint8_t *base = reinterpret_cast<const uint8_t *>(src.w) + ((wave0 << CH) + (size_t)(g - clo)) * 32;
+        const size_t plane = src.cap * 32;
+        uint4 r[4][8];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ld_line_issue<CH>(r[c], base + c * plane, lane);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          ld_line_tr(r[c], tr, lane);
+          line_to_pw(q0, q1, q2, q3, c, r[c]);
+        }
+      } else {
+        FT_LOAD4(q0, q1, q2, q3, src, (size_t)g, (size_t)cend);
+      }
       if (take(g)) { un_pw(x, q0); accum(); }
       if (g + 1 < cend && take(g + 1)) { un_pw(x, q1); accum(); }
       if (g + 2 < cend && take(g + 2)) { un_pw(x, q2); accum(); }
@@ -804,8 +905,18 @@ template <int CH, int MODE>
 __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2))) k_seg_level1(PtView out, PtView rec, int64_t *keys, PtView src,
                                                      size_t n, const int64_t *off, size_t nseg,
                                                      const int32_t *gather, PtView aos, size_t aos_base) {
-  seg_walk<CH, MODE>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base,
-                     blockIdx.x * (size_t)ATPB + threadIdx.x);
+  const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  if constexpr (MODE == 0) {
+    __shared__ uint4 trs[ATPB / 64][64 * 8];
+    const size_t wave0 = t - (threadIdx.x & 63);
+    if (((wave0 + 64) << CH) <= n)   // wave-uniform: all 64 chunks whole
+      seg_walk<CH, 0, OffArr, true>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base, t,
+                                    trs[threadIdx.x >> 6], wave0);
+    else
+      seg_walk<CH, 0>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base, t);
+  } else {
+    seg_walk<CH, MODE>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base, t);
+  }
 }
 
 // One pass of the run tree with radix 2^LR and stride d: lane c (a level-1
@@ -1057,6 +1168,8 @@ __device__ __forceinline__ void reduce_tile(PtView part, size_t slot, PtView v, 
 template <int CH>
 __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ctr) {
+  __shared__ uint4 trs[ATPB / 64][64 * 8];
+  uint4 *tr = trs[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
 #pragma unroll 1
   for (;;) {
@@ -1077,13 +1190,24 @@ k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ct
       const PtView r{rec.w + (size_t)B.job[j].cb * 2 * 8, rec.cap};
       int64_t *k = keys + 2 * (size_t)B.job[j].cb;
       const size_t c = lt * 64 + lane;
-      if (B.job[j].off)
-        seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, OffArr{B.job[j].off}, B.job[j].nseg,
-                        nullptr, PtView{nullptr, 0}, 0, c);
-      else
-        seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n,
-                        OffUni{B.job[j].len, (int64_t)B.job[j].n}, B.job[j].nseg, nullptr, PtView{nullptr, 0},
-                        0, c);
+      const bool whole = ((lt * 64 + 64) << CH) <= B.job[j].n;   // wave-uniform
+      if (B.job[j].off) {
+        const OffArr o{B.job[j].off};
+        if (whole)
+          seg_walk<CH, 0, OffArr, true>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
+                                        PtView{nullptr, 0}, 0, c, tr, lt * 64);
+        else
+          seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
+                          PtView{nullptr, 0}, 0, c);
+      } else {
+        const OffUni o{B.job[j].len, (int64_t)B.job[j].n};
+        if (whole)
+          seg_walk<CH, 0, OffUni, true>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
+                                        PtView{nullptr, 0}, 0, c, tr, lt * 64);
+        else
+          seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
+                          PtView{nullptr, 0}, 0, c);
+      }
     }
   }
 }

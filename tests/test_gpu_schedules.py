@@ -103,6 +103,34 @@ def test_every_scan_schedule(n):
     ft.tune_set(c, *base)
 
 
+def test_scan_upper_chunk_levels_every_chunk_size():
+    """n = 2^22 + 5 points k_i B with small k_i: the first chunk level holds
+    more than 2^16 sums for every first-level chunk size, so a second chunk
+    level (16 per lane) runs below the block scan; inclusive and exclusive
+    outputs at chunk, wave and level edges and at random positions equal the
+    fixed-base multiples of the prefix sums of the k_i."""
+    n = (1 << 22) + 5
+    c = ft.size_class(n)
+    base = ft.tune_table()[c]
+    k = (np.arange(n, dtype=np.uint64) * np.uint64(40503) + np.uint64(7)) % np.uint64(1 << 20)
+    inc = np.cumsum(k)
+    exc = inc - k
+    edges = [0, 1, 15, 16, 63, 64, 1023, 1024, 4095, 4096, 65535, 65536, (1 << 18) - 1, 1 << 18,
+             (1 << 20) + 1, (1 << 22) - 1, 1 << 22, n - 1]
+    pick = np.array(sorted(set(edges) | set(np.random.default_rng(5).integers(0, n, 300).tolist())))
+    want_inc = ft.GeArray.from_small_scalars(inc[pick]).to_bytes_folded().reshape(-1, 32)
+    want_exc = ft.GeArray.from_small_scalars(exc[pick]).to_bytes_folded().reshape(-1, 32)
+    for ch in (4, 5, 6):
+        ft.tune_set(c, base[0], base[1], ch, *base[3:])
+        for exclusive, want in ((False, want_inc), (True, want_exc)):
+            g = ft.GeArray.from_small_scalars(k)
+            g.prescan() if exclusive else g.scan()
+            got = g.get_subset(pick.astype(np.uint64)).to_bytes_folded().reshape(-1, 32)
+            bad = np.nonzero((got != want).any(axis=1))[0]
+            assert len(bad) == 0, (ch, exclusive, pick[bad[:5]].tolist())
+    ft.tune_set(c, *base)
+
+
 @pytest.mark.parametrize("n", [1, 63, 5000, 8192, 8193, 65537, 70000])
 def test_lookback_scan_small(n):
     """the single-pass look-back scan (8,192-point tiles): one tile, a tile
