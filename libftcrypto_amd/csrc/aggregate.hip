@@ -768,6 +768,61 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
   if (!inside) keys[slot] = (int64_t)s;
 }
 
+// Flush queue of a wave walking whole chunks (LDS): a lane whose segment
+// ends writes its unpacked sum (36 limbs) and destination to the next free
+// entry; the wave packs and stores the queued sums together, one entry per
+// lane, when half the queue is used (at a group's start) and at the end of
+// the walk.  Lanes' segment ends fall at different elements, so a wave that
+// packed and stored at each end would run the pack at most crossings of any
+// lane; from the queue it runs once per drain.  A full queue sends further
+// sums straight to memory.  dst: the output index, or the record slot with
+// bit 63 set (key = segment id).
+constexpr int SEGQ_N = 32;
+struct SegQ {
+  uint4 p[9][SEGQ_N];
+  uint64_t dst[SEGQ_N];
+  int64_t key[SEGQ_N];
+};
+
+__device__ __forceinline__ void segq_put(SegQ *q, int pos, const ge_p3 &a, uint64_t dst, int64_t key) {
+  const uint32_t *w[4] = {a.X.v, a.Y.v, a.Z.v, a.T.v};
+  uint32_t f[36];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int l = 0; l < 9; ++l) f[9 * c + l] = w[c][l];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) q->p[j][pos] = make_uint4(f[4 * j], f[4 * j + 1], f[4 * j + 2], f[4 * j + 3]);
+  q->dst[pos] = dst;
+  q->key[pos] = key;
+}
+
+// entries [0, qn) to memory, entry l by lane l
+__device__ __forceinline__ void segq_drain(SegQ *q, unsigned qn, PtView out, PtView rec, int64_t *keys, int lane) {
+  __builtin_amdgcn_wave_barrier();
+  if ((unsigned)lane < qn) {
+    uint32_t f[36];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      const uint4 u = q->p[j][lane];
+      f[4 * j] = u.x; f[4 * j + 1] = u.y; f[4 * j + 2] = u.z; f[4 * j + 3] = u.w;
+    }
+    ge_p3 a;
+    uint32_t *w[4] = {a.X.v, a.Y.v, a.Z.v, a.T.v};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int l = 0; l < 9; ++l) w[c][l] = f[9 * c + l];
+    const uint64_t d = q->dst[lane];
+    const bool r = d >> 63;
+    const size_t i = (size_t)(d & ~(1ull << 63));
+    const PtView v{r ? rec.w : out.w, r ? rec.cap : out.cap};
+    store_p3(v, i, a);
+    if (r) keys[i] = q->key[lane];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Level 1: lane t walks elements [2^CH t, 2^CH (t + 1)) of the input (through
 // the gather indices when given), segment boundaries from the offsets.
 // MODE 0: elements in order; 1: gathered from the planes of src; 2:
@@ -778,7 +833,8 @@ __device__ __forceinline__ void seg_flush(PtView out, PtView rec, int64_t *keys,
 template <int CH, int MODE, class OffT, bool TR = false>
 __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, PtView src, size_t n,
                                          const OffT off, size_t nseg, const int32_t *gather, PtView aos,
-                                         size_t aos_base, size_t t, uint4 *tr = nullptr, size_t wave0 = 0) {
+                                         size_t aos_base, size_t t, uint4 *tr = nullptr, size_t wave0 = 0,
+                                         SegQ *sq = nullptr) {
   const int64_t clo = (int64_t)(t << CH);
   if (clo >= (int64_t)n) return;
   const int64_t chi = clo + (1 << CH);
@@ -786,7 +842,11 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
   if (rec.w) {                                   // slots left unused keep key -1
     keys[2 * t] = -1;
     keys[2 * t + 1] = -1;
+    // a queued record's key may be written by another lane of the wave
+    if (TR) __threadfence_block();
   }
+  const int lane = (int)(t - wave0);
+  unsigned qn = 0;                               // queued entries (wave-uniform)
   size_t s = seg_find1(off, 0, nseg, clo);
   // segment s = [lo, bnd[0]); bnd[k] = end of segment s + k (INT64_MAX past
   // the last segment).  The ends are reloaded at the start of every group of
@@ -811,9 +871,32 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
   ge_p3_0(acc);
   bool touched = false;
   // advance the segment cursor to element i; true when i lies in segment s
+  // queue (TR) or store the sum of segment s, which ends before element i
+  auto flush = [&](bool need) {
+    if (!TR) {
+      if (need) seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
+      return;
+    }
+    const uint64_t m = __ballot(need);
+    if (!m) return;
+    if (need) {
+      const unsigned pos =
+          qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      if (pos < (unsigned)SEGQ_N) {
+        const bool inside = lo >= clo && bnd[0] <= chi;
+        const size_t slot = 2 * t + (lo < clo ? 0 : 1);
+        segq_put(sq, (int)pos, acc, inside ? (uint64_t)s : (slot | (1ull << 63)), (int64_t)s);
+      } else {
+        seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
+      }
+    }
+    qn = min(qn + (unsigned)__popcll(m), (unsigned)SEGQ_N);
+  };
   auto take = [&](int64_t i) -> bool {
-    if (s < nseg && bnd[0] <= i) {
-      if (touched) seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
+    const bool cross = s < nseg && bnd[0] <= i;
+    if (TR) flush(cross && touched);
+    if (cross) {
+      if (!TR) flush(touched);
       touched = false;
       ++s;
       lo = bnd[0];
@@ -875,10 +958,13 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
   } else {
 #pragma unroll 1
     for (int64_t g = clo; g < cend; g += 4) {
+      if (TR && qn >= (unsigned)SEGQ_N / 2) {
+        segq_drain(sq, qn, out, rec, keys, lane);
+        qn = 0;
+      }
       refill();
       PW q0, q1, q2, q3;
       if (TR) {
-        const int lane = (int)(t - wave0);
         const uint8_t *base = reinterpret_cast<const uint8_t *>(src.w) + ((wave0 << CH) + (size_t)(g - clo)) * 32;
         const size_t plane = src.cap * 32;
         uint4 r[4][8];
@@ -898,7 +984,12 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
       if (g + 3 < cend && take(g + 3)) { un_pw(x, q3); accum(); }
     }
   }
-  if (touched) seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
+  if (TR) {
+    flush(touched);
+    if (qn) segq_drain(sq, qn, out, rec, keys, lane);
+  } else if (touched) {
+    seg_flush(out, rec, keys, s, lo, bnd[0], clo, chi, t, acc);
+  }
 }
 
 template <int CH, int MODE>
@@ -908,10 +999,11 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   if constexpr (MODE == 0) {
     __shared__ uint4 trs[ATPB / 64][64 * 8];
+    __shared__ SegQ sqs[ATPB / 64];
     const size_t wave0 = t - (threadIdx.x & 63);
     if (((wave0 + 64) << CH) <= n)   // wave-uniform: all 64 chunks whole
       seg_walk<CH, 0, OffArr, true>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base, t,
-                                    trs[threadIdx.x >> 6], wave0);
+                                    trs[threadIdx.x >> 6], wave0, &sqs[threadIdx.x >> 6]);
     else
       seg_walk<CH, 0>(out, rec, keys, src, n, OffArr{off}, nseg, gather, aos, aos_base, t);
   } else {
@@ -1169,7 +1261,9 @@ template <int CH>
 __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ctr) {
   __shared__ uint4 trs[ATPB / 64][64 * 8];
+  __shared__ SegQ sqs[ATPB / 64];
   uint4 *tr = trs[threadIdx.x >> 6];
+  SegQ *sq = &sqs[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
 #pragma unroll 1
   for (;;) {
@@ -1195,7 +1289,7 @@ k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ct
         const OffArr o{B.job[j].off};
         if (whole)
           seg_walk<CH, 0, OffArr, true>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
-                                        PtView{nullptr, 0}, 0, c, tr, lt * 64);
+                                        PtView{nullptr, 0}, 0, c, tr, lt * 64, sq);
         else
           seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
                           PtView{nullptr, 0}, 0, c);
@@ -1203,7 +1297,7 @@ k_agg_batch(const BBatch B, PtView rec, int64_t *keys, PtView part, unsigned *ct
         const OffUni o{B.job[j].len, (int64_t)B.job[j].n};
         if (whole)
           seg_walk<CH, 0, OffUni, true>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
-                                        PtView{nullptr, 0}, 0, c, tr, lt * 64);
+                                        PtView{nullptr, 0}, 0, c, tr, lt * 64, sq);
         else
           seg_walk<CH, 0>(B.job[j].out, r, k, B.job[j].src, B.job[j].n, o, B.job[j].nseg, nullptr,
                           PtView{nullptr, 0}, 0, c);

@@ -335,3 +335,35 @@ def test_batch_from_several_threads():
     for x in th:
         x.join()
     assert not errs, errs
+
+
+@pytest.mark.parametrize("lens", [(1,), (3,), (1, 2), (5, 1, 1, 64)])
+def test_segment_ends_at_every_lane(lens):
+    """Segments of lengths cycling through `lens` over 70,000 elements k_i B
+    (small k_i): many lanes of a wave end a segment at the same element
+    (all 64 for length 1), more than a wave's flush queue holds, so queued
+    and directly stored sums mix.  CSR through ftx_segmented_sum and as a
+    batched job, and uniform segments as a batched job, equal the fixed-base
+    multiples of the segment sums of the k_i."""
+    n = 70000
+    k = (np.arange(n, dtype=np.uint64) * np.uint64(7919) + np.uint64(3)) % np.uint64(1 << 24)
+    X = ft.GeArray.from_small_scalars(k)
+    reps = -(-n // sum(lens)) * len(lens)
+    ln = np.resize(np.array(lens, dtype=np.int64), reps)
+    off = np.concatenate([[0], np.cumsum(ln)])
+    off = np.concatenate([off[off < n], [n]]).astype(np.int64)
+    want = ft.GeArray.from_small_scalars(np.add.reduceat(k, off[:-1])).to_bytes_folded()
+    got = ft.segmented_sum(off, X)
+    assert np.array_equal(got.to_bytes_folded(), want)
+    d = DevBuf(off)
+    out = ft.GeArray.alloc(len(off) - 1)
+    jobs = [ft.agg_job(ft.AGG_SEGSUM, out, X, d.p, len(off) - 1, n)]
+    if len(lens) == 1:
+        u = lens[0]
+        outu = ft.GeArray.alloc(-(-n // u))
+        jobs.append(ft.agg_job(ft.AGG_SEGSUM, outu, X, None, -(-n // u), n, u))
+    ft.aggregate_batch(jobs)
+    assert np.array_equal(out.to_bytes_folded(), want)
+    if len(lens) == 1:
+        wu = ft.GeArray.from_small_scalars(np.add.reduceat(k, np.arange(0, n, lens[0]))).to_bytes_folded()
+        assert np.array_equal(outu.to_bytes_folded(), wu)
