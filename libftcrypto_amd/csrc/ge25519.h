@@ -11,9 +11,14 @@
 // Z = F G, T = E H.  Bound contracts (classes of fe25519.h):
 //   ge_p3 / ge_p2 ......... every coordinate N
 //   ge_cached ............. YpX <= A1, YmX <= A1, Z N, T2d <= A2
-//   ge_precomp ............ ypx <= A1, ymx <= A1, xy2d <= A2
-//   ge_p1p1 (E,F,G,H) ..... E <= A2, F N, G <= A2, H <= A1
+//   ge_precomp ............ every field N
+//   ge_p1p1 (E,F,G,H) ..... from add / sub / dbl: E <= A2, F N, G <= A2,
+//                           H <= A1; from madd: E, G <= A1, F, H <= A2
 // which keeps every fe_mul operand pair inside the exact set.
+// ge_precomp is the affine form of a table entry with its sums halved:
+// ((y + x)/2, (y - x)/2, d x y).  Every product of a mixed add then comes
+// out halved, so its completed form (all four coordinates half of the usual
+// ones: the same point) needs neither D = 2 Z nor a carry of F = D - C.
 #pragma once
 #include "fe25519.h"
 
@@ -23,11 +28,11 @@ struct ge_p3 { fe X, Y, Z, T; };
 struct ge_p2 { fe X, Y, Z; };
 struct ge_p1p1 { fe E, F, G, H; };
 struct ge_cached { fe YpX, YmX, Z, T2d; };
-struct ge_precomp { fe ypx, ymx, xy2d; };
+struct ge_precomp { fe ypx_h, ymx_h, xyd; };   // (y + x)/2, (y - x)/2, d x y
 
 FT_HD void ge_p3_0(ge_p3 &h) { fe_0(h.X); fe_1(h.Y); fe_1(h.Z); fe_0(h.T); }
 FT_HD void ge_cached_0(ge_cached &h) { fe_1(h.YpX); fe_1(h.YmX); fe_1(h.Z); fe_0(h.T2d); }
-FT_HD void ge_precomp_0(ge_precomp &h) { fe_1(h.ypx); fe_1(h.ymx); fe_0(h.xy2d); }
+FT_HD void ge_precomp_0(ge_precomp &h) { h.ypx_h = FE_INV2; h.ymx_h = FE_INV2; fe_0(h.xyd); }
 
 FT_HD void ge_p1p1_to_p3(ge_p3 &r, const ge_p1p1 &p) {
   fe_mul(r.X, p.E, p.F);
@@ -86,20 +91,23 @@ FT_HD void ge_sub(ge_p1p1 &r, const ge_p3 &p, const ge_cached &q) {
   fe_carry(r.G);              // N
 }
 
-// r = p + q for an affine precomputed q
+// r = p + q for an affine precomputed q (halved sums, see above).  With
+// A = (Y - X)(y - x), B = (Y + X)(y + x), C = 2d T x y, D = 2 Z the usual
+// completed form is E = B - A, F = D - C, G = D + C, H = B + A; here each is
+// produced halved, from (X - Y) (y - x)/2 = -A/2, (Y + X) (y + x)/2 = B/2 and
+// T d x y = C/2: E/2 = B/2 + (-A/2) <= A1, H/2 = B/2 - (-A/2) <= A2,
+// G/2 = Z + C/2 <= A1, F/2 = Z - C/2 <= A2.
 FT_HD void ge_madd(ge_p1p1 &r, const ge_p3 &p, const ge_precomp &q) {
-  fe a, b, A, B, C, D;
-  fe_sub(a, p.Y, p.X);
-  fe_add(b, p.Y, p.X);
-  fe_mul(A, a, q.ymx);
-  fe_mul(B, b, q.ypx);
-  fe_mul(C, p.T, q.xy2d);
-  fe_add(D, p.Z, p.Z);
-  fe_sub(r.E, B, A);
-  fe_add(r.H, B, A);
-  fe_add(r.G, D, C);
-  fe_sub(r.F, D, C);
-  fe_carry(r.F);
+  fe a, b, nA, B, C;
+  fe_sub(a, p.X, p.Y);        // A2
+  fe_add(b, p.Y, p.X);        // A1
+  fe_mul(nA, a, q.ymx_h);
+  fe_mul(B, b, q.ypx_h);
+  fe_mul(C, p.T, q.xyd);
+  fe_add(r.E, B, nA);         // A1
+  fe_sub(r.H, B, nA);         // A2
+  fe_add(r.G, p.Z, C);        // A1
+  fe_sub(r.F, p.Z, C);        // A2
 }
 
 // r = 2 p: E = 2XY, G = Y^2 - X^2, H = X^2 + Y^2, F = 2Z^2 - G
@@ -152,14 +160,15 @@ FT_HD void ge_cached_cneg(ge_cached &q, uint32_t neg) {
   fe_cneg_mask(q.T2d, q.T2d, 0u - neg);
 }
 
+// -q for neg = 1: (-x, y) swaps the halved sums and negates d x y
 FT_HD void ge_precomp_cneg(ge_precomp &q, uint32_t neg) {
 #pragma unroll
   for (int l = 0; l < 9; ++l) {
-    const uint32_t a = q.ypx.v[l], b = q.ymx.v[l];
-    q.ypx.v[l] = neg ? b : a;
-    q.ymx.v[l] = neg ? a : b;
+    const uint32_t a = q.ypx_h.v[l], b = q.ymx_h.v[l];
+    q.ypx_h.v[l] = neg ? b : a;
+    q.ymx_h.v[l] = neg ? a : b;
   }
-  fe_cneg_mask(q.xy2d, q.xy2d, 0u - neg);
+  fe_cneg_mask(q.xyd, q.xyd, 0u - neg);
 }
 
 // full p3 + p3 -> p3 (9M)
@@ -188,16 +197,19 @@ FT_HD void ge_p3_neg(ge_p3 &r, const ge_p3 &p) {
   fe_neg(r.T, p.T); fe_carry(r.T);
 }
 
-// affine precomputed form of a point (one inversion)
+// affine precomputed form of a point with halved sums (one inversion):
+// ((y + x)/2, (y - x)/2, d x y)
 FT_HD void ge_p3_to_precomp(ge_precomp &r, const ge_p3 &p) {
-  fe zi, x, y, xy;
+  fe zi, x, y, xh, yh, xy;
   fe_invert(zi, p.Z);
   fe_mul(x, p.X, zi);
   fe_mul(y, p.Y, zi);
-  fe_add(r.ypx, y, x); fe_carry(r.ypx);
-  fe_sub(r.ymx, y, x); fe_carry(r.ymx);
+  fe_mul(xh, x, FE_INV2);
+  fe_mul(yh, y, FE_INV2);
+  fe_add(r.ypx_h, yh, xh); fe_carry(r.ypx_h);
+  fe_sub(r.ymx_h, yh, xh); fe_carry(r.ymx_h);
   fe_mul(xy, x, y);
-  fe_mul(r.xy2d, xy, FE_D2);
+  fe_mul(r.xyd, xy, FE_D);
 }
 
 // ---------------------------------------------------------------- encoding

@@ -7,8 +7,8 @@
 // two 16-byte loads per lane.  (The reference keeps an array-of-structures of
 // 160-byte ge25519_p3 in managed memory, ftcrypto.cu:21-27, 215-242.)
 //
-// A comb table holds 32 rows x 8 affine entries (y+x, y-x, 2dxy) in the
-// 9-limb form, each entry padded to 28 words (112 bytes): 28,672 bytes.
+// A comb table holds 32 rows x 8 affine entries ((y+x)/2, (y-x)/2, dxy) in
+// the 9-limb form, each entry padded to 28 words (112 bytes): 28,672 bytes.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "ge25519.h"
@@ -196,7 +196,7 @@ __device__ __forceinline__ void ld_pw_aos(PW &p, const PtView v, size_t base, si
 
 __device__ __forceinline__ void comb_entry_store(uint32_t *dst, const ge_precomp &e) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) { dst[i] = e.ypx.v[i]; dst[9 + i] = e.ymx.v[i]; dst[18 + i] = e.xy2d.v[i]; }
+  for (int i = 0; i < 9; ++i) { dst[i] = e.ypx_h.v[i]; dst[9 + i] = e.ymx_h.v[i]; dst[18 + i] = e.xyd.v[i]; }
   dst[27] = 0;
 }
 
@@ -220,9 +220,9 @@ __device__ __forceinline__ void comb_select(ge_precomp &t, const uint32_t *row, 
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      t.ypx.v[i] = m ? w[i] : t.ypx.v[i];
-      t.ymx.v[i] = m ? w[9 + i] : t.ymx.v[i];
-      t.xy2d.v[i] = m ? w[18 + i] : t.xy2d.v[i];
+      t.ypx_h.v[i] = m ? w[i] : t.ypx_h.v[i];
+      t.ymx_h.v[i] = m ? w[9 + i] : t.ymx_h.v[i];
+      t.xyd.v[i] = m ? w[18 + i] : t.xyd.v[i];
     }
   }
   ge_precomp_cneg(t, neg);

@@ -20,6 +20,25 @@ void h_fe_store(uint32_t *w, const uint32_t *a) { fe_store_words(w, ld(a)); }
 void h_fe_from_words(uint32_t *r, const uint32_t *w) { fe z; fe_from_words(z, w); st(r, z); }
 void h_fe_sub(uint32_t *r, const uint32_t *a, const uint32_t *b) { fe z; fe_sub(z, ld(a), ld(b)); st(r, z); }
 void h_constants(uint32_t *d, uint32_t *d2, uint32_t *s) { st(d, FE_D); st(d2, FE_D2); st(s, FE_SQRTM1); }
+void h_inv2(uint32_t *r) { st(r, FE_INV2); }
+// a + q (q negated when neg) by the mixed add over q's affine precomputed form
+// (the comb tables' entry); q = NULL adds the identity entry
+void h_madd_encoded(uint32_t *out, const uint32_t *a, const uint32_t *b, int neg) {
+  ge_p3 p, q, r;
+  ge_precomp e;
+  ge_p1p1 t;
+  ge_decode(p, a);
+  if (b) {
+    ge_decode(q, b);
+    ge_p3_to_precomp(e, q);
+  } else {
+    ge_precomp_0(e);
+  }
+  ge_precomp_cneg(e, (uint32_t)neg);
+  ge_madd(t, p, e);
+  ge_p1p1_to_p3(r, t);
+  ge_p3_encode(out, r);
+}
 int h_decode_encode(uint32_t *out, const uint32_t *in) {
   ge_p3 p;
   uint32_t ok = ge_decode(p, in);

@@ -150,6 +150,9 @@ def test_invert_pow_constants(H):
     H.h_constants(d, d2, s)
     assert val(d) == edref.D and val(d2) == 2 * edref.D % P
     assert val(s) == edref.SQRTM1
+    h = (ctypes.c_uint32 * 9)()
+    H.h_inv2(h)
+    assert val(h) == (P + 1) // 2 and max(h) <= N
 
 
 def enc_words(pt):
@@ -170,6 +173,16 @@ def test_group_formulas(H):
         assert words_val(out) == int.from_bytes(edref.encode(edref.add(pa, pa)), "little")
         H.h_neg_encoded(out, arr(enc_words(pa)))
         assert words_val(out) == int.from_bytes(edref.encode(edref.neg(pa)), "little")
+        # mixed add over the comb tables' entry form (halved sums, d x y),
+        # with and without the conditional negation, and the identity entry
+        H.h_madd_encoded(out, arr(enc_words(pa)), arr(enc_words(pb)), 0)
+        assert words_val(out) == int.from_bytes(edref.encode(edref.add(pa, pb)), "little")
+        H.h_madd_encoded(out, arr(enc_words(pa)), arr(enc_words(pb)), 1)
+        assert words_val(out) == int.from_bytes(edref.encode(edref.add(pa, edref.neg(pb))), "little")
+        H.h_madd_encoded(out, arr(enc_words(pa)), None, 0)
+        assert words_val(out) == int.from_bytes(edref.encode(pa), "little")
+        H.h_madd_encoded(out, arr(enc_words(pa)), arr(enc_words(pa)), 0)
+        assert words_val(out) == int.from_bytes(edref.encode(edref.add(pa, pa)), "little")
 
 
 def naf3(k):
