@@ -10,11 +10,15 @@
 // The completed form is kept as (E, F, G, H) with X = E F, Y = G H,
 // Z = F G, T = E H.  Bound contracts (classes of fe25519.h):
 //   ge_p3 / ge_p2 ......... every coordinate N
-//   ge_cached ............. YpX <= A1, YmX <= A1, Z N, T2d <= A2
+//   ge_cached ............. YpX, YmX <= A1, Z2 <= A1, T2d <= A1
 //   ge_precomp ............ every field N
-//   ge_p1p1 (E,F,G,H) ..... from add / sub / dbl: E <= A2, F N, G <= A2,
-//                           H <= A1; from madd: E, G <= A1, F, H <= A2
+//   ge_p1p1 (E,F,G,H) ..... from dbl: E <= A2, F N, G <= A2, H <= A1;
+//                           from add and madd: E, G <= A1, F, H <= A2;
+//                           from sub: E, G <= A2, F, H <= A1
 // which keeps every fe_mul operand pair inside the exact set.
+// ge_cached holds 2 Z (Z2) rather than Z: the add's D = 2 Z1 Z2 is then one
+// product, and with the sums arranged as below no field of the completed
+// form needs a carry.
 // ge_precomp is the affine form of a table entry with its sums halved:
 // ((y + x)/2, (y - x)/2, d x y).  Every product of a mixed add then comes
 // out halved, so its completed form (all four coordinates half of the usual
@@ -27,11 +31,11 @@ namespace ft {
 struct ge_p3 { fe X, Y, Z, T; };
 struct ge_p2 { fe X, Y, Z; };
 struct ge_p1p1 { fe E, F, G, H; };
-struct ge_cached { fe YpX, YmX, Z, T2d; };
+struct ge_cached { fe YpX, YmX, Z2, T2d; };   // Y + X, Y - X, 2 Z, 2d T
 struct ge_precomp { fe ypx_h, ymx_h, xyd; };   // (y + x)/2, (y - x)/2, d x y
 
 FT_HD void ge_p3_0(ge_p3 &h) { fe_0(h.X); fe_1(h.Y); fe_1(h.Z); fe_0(h.T); }
-FT_HD void ge_cached_0(ge_cached &h) { fe_1(h.YpX); fe_1(h.YmX); fe_1(h.Z); fe_0(h.T2d); }
+FT_HD void ge_cached_0(ge_cached &h) { fe_1(h.YpX); fe_1(h.YmX); fe_0(h.Z2); h.Z2.v[0] = 2; fe_0(h.T2d); }
 FT_HD void ge_precomp_0(ge_precomp &h) { h.ypx_h = FE_INV2; h.ymx_h = FE_INV2; fe_0(h.xyd); }
 
 FT_HD void ge_p1p1_to_p3(ge_p3 &r, const ge_p1p1 &p) {
@@ -51,29 +55,32 @@ FT_HD void ge_p3_to_cached(ge_cached &r, const ge_p3 &p) {
   fe_add(r.YpX, p.Y, p.X);
   fe_sub(r.YmX, p.Y, p.X);
   fe_carry(r.YmX);
-  r.Z = p.Z;
+  fe_add(r.Z2, p.Z, p.Z);
   fe_mul(r.T2d, p.T, FE_D2);
 }
 
-// r = p + q (unified: also correct for p == q and for the identity)
+// r = p + q (unified: also correct for p == q and for the identity).  With
+// A = (Y1 - X1)(Y2 - X2), B = (Y1 + X1)(Y2 + X2), C = 2d T1 T2, D = 2 Z1 Z2:
+// E = B - A, F = D - C, G = D + C, H = B + A, computed as E = B + (-A) from
+// (X1 - Y1)(Y2 - X2) = -A, so that E and G are sums (<= A1) and F and H
+// differences (<= A2): every product of the p1p1 conversions pairs A1 with A2.
 FT_HD void ge_add(ge_p1p1 &r, const ge_p3 &p, const ge_cached &q) {
-  fe a, b, A, B, C, D;
-  fe_sub(a, p.Y, p.X);        // A2
+  fe a, b, nA, B, C, D;
+  fe_sub(a, p.X, p.Y);        // A2
   fe_add(b, p.Y, p.X);        // A1
-  fe_mul(A, a, q.YmX);
+  fe_mul(nA, a, q.YmX);
   fe_mul(B, b, q.YpX);
   fe_mul(C, p.T, q.T2d);
-  fe_mul(D, p.Z, q.Z);
-  fe_add(D, D, D);            // A1
-  fe_sub(r.E, B, A);          // A2
-  fe_add(r.H, B, A);          // A1
-  fe_add(r.G, D, C);          // A2
-  fe_sub(r.F, D, C);          // <= 2^31 + 2^18
-  fe_carry(r.F);              // N
+  fe_mul(D, p.Z, q.Z2);
+  fe_add(r.E, B, nA);         // A1
+  fe_sub(r.H, B, nA);         // A2
+  fe_add(r.G, D, C);          // A1
+  fe_sub(r.F, D, C);          // A2
 }
 
-// r = p - q (p + (-q) with -q = (YmX, YpX, Z, -T2d) read in place); F and G
-// are carried so that every product of the p1p1 conversions stays in class
+// r = p - q = p + (-q), -q = (YmX, YpX, Z2, -T2d) read in place: A, B and D
+// as for the add of -q, C = 2d T1 T2 (so -q's C is -C); E = B - A, F = D + C,
+// G = D - C, H = B + A: E, G <= A2 and F, H <= A1.
 FT_HD void ge_sub(ge_p1p1 &r, const ge_p3 &p, const ge_cached &q) {
   fe a, b, A, B, C, D;
   fe_sub(a, p.Y, p.X);        // A2
@@ -81,14 +88,11 @@ FT_HD void ge_sub(ge_p1p1 &r, const ge_p3 &p, const ge_cached &q) {
   fe_mul(A, a, q.YpX);
   fe_mul(B, b, q.YmX);
   fe_mul(C, p.T, q.T2d);
-  fe_mul(D, p.Z, q.Z);
-  fe_add(D, D, D);            // A1
+  fe_mul(D, p.Z, q.Z2);
   fe_sub(r.E, B, A);          // A2
   fe_add(r.H, B, A);          // A1
-  fe_add(r.F, D, C);          // A2
-  fe_carry(r.F);              // N
-  fe_sub(r.G, D, C);          // <= 2^31 + 2^18
-  fe_carry(r.G);              // N
+  fe_add(r.F, D, C);          // A1
+  fe_sub(r.G, D, C);          // A2
 }
 
 // r = p + q for an affine precomputed q (halved sums, see above).  With

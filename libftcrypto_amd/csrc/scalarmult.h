@@ -187,7 +187,7 @@ __device__ __forceinline__ void cached_select(ge_cached &t, const ge_cached tab[
     for (int l = 0; l < 9; ++l) {
       t.YpX.v[l] = m ? tab[j].YpX.v[l] : t.YpX.v[l];
       t.YmX.v[l] = m ? tab[j].YmX.v[l] : t.YmX.v[l];
-      t.Z.v[l] = m ? tab[j].Z.v[l] : t.Z.v[l];
+      t.Z2.v[l] = m ? tab[j].Z2.v[l] : t.Z2.v[l];
       t.T2d.v[l] = m ? tab[j].T2d.v[l] : t.T2d.v[l];
     }
   }
@@ -334,7 +334,7 @@ __device__ __forceinline__ void cached_select4_coz_park(ge_cached &t, const ge_c
       t.T2d.v[l] = m ? w[18 + l] : t.T2d.v[l];
     }
   }
-  t.Z = zc;
+  fe_add(t.Z2, zc, zc);
   ge_cached_cneg(t, neg);
 }
 
@@ -356,7 +356,7 @@ __device__ __forceinline__ void cached_select4(ge_cached &t, const ge_cached tab
     for (int l = 0; l < 9; ++l) {
       t.YpX.v[l] = m ? tab[j].YpX.v[l] : t.YpX.v[l];
       t.YmX.v[l] = m ? tab[j].YmX.v[l] : t.YmX.v[l];
-      t.Z.v[l] = m ? tab[j].Z.v[l] : t.Z.v[l];
+      t.Z2.v[l] = m ? tab[j].Z2.v[l] : t.Z2.v[l];
       t.T2d.v[l] = m ? tab[j].T2d.v[l] : t.T2d.v[l];
     }
   }
