@@ -258,7 +258,8 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
 /* Launch schedules of the reduce, scan and segmented sum and the window widths
  * of the variable-base mults, one per size class of the input length (class
  * 0: n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).  seg1 and scan are log2
- * elements per lane (4..6) at segmented-sum level 1 and in the scan; segu
+ * elements per lane (4..7) at segmented-sum level 1 and at the scan's first
+ * level (the levels above it take 16 per lane); segu
  * (1..4) is log2 of the radix of the tree that adds the partial sums of
  * segments spanning several level-1 chunks; reduce_blocks (1..4096) bounds
  * the first reduce pass; u32_window

@@ -4,7 +4,7 @@
 // chunk per lane, so the work per lane is the same whatever the segment
 // lengths are (the reference runs one thread per segment, ftcrypto.cu:1116-1127,
 // and a recursive Blelloch scan, :1474-1562).  The chunk size of each kernel
-// is a template parameter in [2^4, 2^6]; the schedule used for an input size
+// is a template parameter in [2^4, 2^7]; the schedule used for an input size
 // comes from the tuning table (tune.hip).
 //
 // Scan: chunk sums are taken level by level until a level has at most 2^16
@@ -240,6 +240,7 @@ static hipError_t launch_chunk_sums(PtView sums, PtView v, size_t n, int ch, hip
     case 4: hipLaunchKernelGGL(k_chunk_sums<4>, grid, dim3(ATPB), 0, s, sums, v, n); break;
     case 5: hipLaunchKernelGGL(k_chunk_sums<5>, grid, dim3(ATPB), 0, s, sums, v, n); break;
     case 6: hipLaunchKernelGGL(k_chunk_sums<6>, grid, dim3(ATPB), 0, s, sums, v, n); break;
+    case 7: hipLaunchKernelGGL(k_chunk_sums<7>, grid, dim3(ATPB), 0, s, sums, v, n); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -253,6 +254,7 @@ static hipError_t launch_chunk_scan(PtView v, size_t n, PtView pre, bool exclusi
     case 4: hipLaunchKernelGGL(k_chunk_scan<4>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
     case 5: hipLaunchKernelGGL(k_chunk_scan<5>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
     case 6: hipLaunchKernelGGL(k_chunk_scan<6>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 7: hipLaunchKernelGGL(k_chunk_scan<7>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -335,6 +337,7 @@ static hipError_t launch_chunk_scan_q(PtView v, size_t n, PtView pre, bool exclu
     case 4: hipLaunchKernelGGL(k_chunk_scan_q<4>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
     case 5: hipLaunchKernelGGL(k_chunk_scan_q<5>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
     case 6: hipLaunchKernelGGL(k_chunk_scan_q<6>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 7: hipLaunchKernelGGL(k_chunk_scan_q<7>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1102,6 +1105,9 @@ static hipError_t launch_seg_level1(PtView out, PtView rec, int64_t *keys, PtVie
     case 24: FT_SEG1(6, 0); break;
     case 25: FT_SEG1(6, 1); break;
     case 26: FT_SEG1(6, 2); break;
+    case 28: FT_SEG1(7, 0); break;
+    case 29: FT_SEG1(7, 1); break;
+    case 30: FT_SEG1(7, 2); break;
     default: return hipErrorInvalidValue;
   }
 #undef FT_SEG1
@@ -1555,7 +1561,8 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
     switch (ch1) {
       case 4: launch_agg_batch<4>(p.B, rec, keys, at(p.part_base), ctr, s); break;
       case 5: launch_agg_batch<5>(p.B, rec, keys, at(p.part_base), ctr, s); break;
-      default: launch_agg_batch<6>(p.B, rec, keys, at(p.part_base), ctr, s); break;
+      case 6: launch_agg_batch<6>(p.B, rec, keys, at(p.part_base), ctr, s); break;
+      default: launch_agg_batch<7>(p.B, rec, keys, at(p.part_base), ctr, s); break;
     }
   }
   hipError_t e = hipGetLastError();

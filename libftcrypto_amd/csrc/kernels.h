@@ -66,7 +66,7 @@ struct AggTune {
   int encl;          // lanes per ciphertext of an encryption (1, or 2: C1 and C2 apart)
   int scanlb;        // scan: 0 chunk levels (two walks, upper levels between), 1 single-pass look-back
 };
-constexpr int CH_MIN = 4, CH_MAX = 6;             // allowed chunk log2 range
+constexpr int CH_MIN = 4, CH_MAX = 7;             // allowed chunk log2 range
 constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed run-tree radix log2 range
 constexpr unsigned REDUCE_BLOCKS_MAX = 4096;
 // reduces of at most this many points run their first pass on quad lanes

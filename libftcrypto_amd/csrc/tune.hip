@@ -3,10 +3,11 @@
 //
 // A schedule is chosen per size class of the input length n (n < 2^14,
 // < 2^18, < 2^22, larger):
-//   seg1    log2 elements per lane at segmented-sum level 1 (4..6)
+//   seg1    log2 elements per lane at segmented-sum level 1 (4..7)
 //   segu    log2 radix of the tree that adds the chunk partials of segments
 //           spanning several level-1 chunks (1..4: radix 2, 4, 8, 16)
-//   scan    log2 elements per lane at every scan level (4..6)
+//   scan    log2 elements per lane at a scan's first level (4..7; the
+//           levels above it take 16)
 //   reduce  most blocks of 256 lanes in the first reduce pass (512..4096)
 //   u32w    window width of ciphertext x 32-bit constant: 4 (radix 16,
 //           8-entry table, 9 windows) or 3 (radix 8, 4-entry table, 11)

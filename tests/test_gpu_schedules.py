@@ -33,6 +33,24 @@ def same(a, o):
     assert np.array_equal(a.to_bytes_folded(), o.to_bytes())
 
 
+class _devbuf:
+    """device copy of a numpy array (freed with the object)"""
+
+    def __init__(self, arr):
+        import ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        arr = np.ascontiguousarray(arr)
+        self.p = ctypes.c_void_p()
+        assert self.hip.hipMalloc(ctypes.byref(self.p), max(1, arr.nbytes)) == 0
+        assert self.hip.hipMemcpy(self.p, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1) == 0
+
+    def __del__(self):
+        self.hip.hipFree(self.p)
+
+
 def seg_case(n, seed):
     rng = np.random.default_rng(seed)
     lens = np.clip(np.round(rng.lognormal(np.log(16), 1.5, size=n)), 1, 900).astype(np.int64)
@@ -55,12 +73,16 @@ def test_every_segsum_schedule(n):
     rng = np.random.default_rng(n)
     cols = rng.integers(0, n, size=n, dtype=np.int32)
     want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
-    for ch1, chu in itertools.product((4, 5, 6), (1, 2, 3, 4)):
+    d_off = _devbuf(off)
+    for ch1, chu in itertools.product((4, 5, 6, 7), (1, 2, 3, 4)):
         ft.tune_set(c, ch1, chu, *base[2:])
         got = ft.segmented_sum(off, g)
         assert np.array_equal(got.to_bytes_folded(), want), (ch1, chu)
         got = ft.gather_sum(off, cols, g)
         assert np.array_equal(got.to_bytes_folded(), want_g), ("gather", ch1, chu)
+        out = ft.GeArray.alloc(len(off) - 1)       # the persistent batched walk
+        ft.aggregate_batch([ft.agg_job(ft.AGG_SEGSUM, out, g, d_off.p, len(off) - 1, n)])
+        assert np.array_equal(out.to_bytes_folded(), want), ("batch", ch1, chu)
     ft.tune_set(c, *base)
 
 
@@ -77,7 +99,7 @@ def test_segsum_one_segment_spanning_every_chunk():
     want = O.segmented_sum(off, o).to_bytes()
     cols = np.arange(n - 1, -1, -1, dtype=np.int32)
     want_g = O.segmented_sum(off, o, idx=cols).to_bytes()
-    for ch1, lr in itertools.product((4, 5, 6), (1, 2, 3, 4)):
+    for ch1, lr in itertools.product((4, 5, 6, 7), (1, 2, 3, 4)):
         ft.tune_set(c, ch1, lr, *base[2:])
         assert np.array_equal(ft.segmented_sum(off, g).to_bytes_folded(), want), (ch1, lr)
         assert np.array_equal(ft.gather_sum(off, cols, g).to_bytes_folded(), want_g), ("gather", ch1, lr)
@@ -92,7 +114,7 @@ def test_every_scan_schedule(n):
     o_inc, o_exc = O.Array.from_scalars(s), O.Array.from_scalars(s)
     o_inc.scan()
     o_exc.scan(exclusive=True)
-    for ch in (4, 5, 6):
+    for ch in (4, 5, 6, 7):
         ft.tune_set(c, base[0], base[1], ch, *base[3:])
         g = ft.GeArray.from_scalars(s)
         g.scan()
@@ -104,23 +126,23 @@ def test_every_scan_schedule(n):
 
 
 def test_scan_upper_chunk_levels_every_chunk_size():
-    """n = 2^22 + 5 points k_i B with small k_i: the first chunk level holds
+    """n = 2^23 + 5 points k_i B with small k_i: the first chunk level holds
     more than 2^16 sums for every first-level chunk size, so a second chunk
     level (16 per lane) runs below the block scan; inclusive and exclusive
     outputs at chunk, wave and level edges and at random positions equal the
     fixed-base multiples of the prefix sums of the k_i."""
-    n = (1 << 22) + 5
+    n = (1 << 23) + 5
     c = ft.size_class(n)
     base = ft.tune_table()[c]
     k = (np.arange(n, dtype=np.uint64) * np.uint64(40503) + np.uint64(7)) % np.uint64(1 << 20)
     inc = np.cumsum(k)
     exc = inc - k
     edges = [0, 1, 15, 16, 63, 64, 1023, 1024, 4095, 4096, 65535, 65536, (1 << 18) - 1, 1 << 18,
-             (1 << 20) + 1, (1 << 22) - 1, 1 << 22, n - 1]
+             (1 << 20) + 1, (1 << 22) - 1, 1 << 22, (1 << 23) - 1, 1 << 23, n - 1]
     pick = np.array(sorted(set(edges) | set(np.random.default_rng(5).integers(0, n, 300).tolist())))
     want_inc = ft.GeArray.from_small_scalars(inc[pick]).to_bytes_folded().reshape(-1, 32)
     want_exc = ft.GeArray.from_small_scalars(exc[pick]).to_bytes_folded().reshape(-1, 32)
-    for ch in (4, 5, 6):
+    for ch in (4, 5, 6, 7):
         ft.tune_set(c, base[0], base[1], ch, *base[3:])
         for exclusive, want in ((False, want_inc), (True, want_exc)):
             g = ft.GeArray.from_small_scalars(k)
@@ -259,7 +281,7 @@ def test_autotune_keeps_results(tmp_path):
     table = ft.autotune(max_class=2, path=path)
     assert os.path.exists(path)
     for c, (a, b, sc, r, w, v) in table.items():
-        assert 4 <= a <= 6 and 1 <= b <= 4 and 4 <= sc <= 6 and 1 <= r <= 4096 and w in (3, 4) and v in (3, 4)
+        assert 4 <= a <= 7 and 1 <= b <= 4 and 4 <= sc <= 7 and 1 <= r <= 4096 and w in (3, 4) and v in (3, 4)
     ft.tune_load(path)
     assert ft.tune_table() == table
     n = 20000

@@ -31,7 +31,8 @@ def test_set_get_save_load(tmp_path):
     assert ft.tune_table() == t
 
 
-@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048, 4, 4), (0, 4, 5, 4, 2048, 4, 4), (0, 4, 0, 4, 2048, 4, 4),
+@pytest.mark.parametrize("bad", [(0, 3, 4, 4, 2048, 4, 4), (0, 8, 4, 4, 2048, 4, 4), (0, 4, 4, 8, 2048, 4, 4),
+                                 (0, 4, 5, 4, 2048, 4, 4), (0, 4, 0, 4, 2048, 4, 4),
                                  (0, 4, 4, 4, 0, 4, 4),
                                  (0, 4, 4, 4, 4097, 4, 4), (0, 4, 4, 4, 2048, 5, 4), (0, 4, 4, 4, 2048, 2, 4),
                                  (0, 4, 4, 4, 2048, 4, 5)])
