@@ -17,12 +17,16 @@
 // leaves a partial sum and its segment id in the chunk's head slot (segment
 // began before the chunk) or tail slot (segment continues after it).  A
 // segment spanning chunks c0..c1 thus has one partial per chunk: c0's tail
-// slot, then the head slots of c0+1..c1.  A radix-R tree over each such run
-// (R = 2, 4, 8 or 16) adds them up in ceil(log_R(chunks)) passes of at most
-// R - 1 adds per lane, and the run's first slot goes to the output in the
-// pass that completes it.  At these sizes the GPU is latency-bound: a wave
-// waits for its lane with the most adds, so radix 2 (one add per lane and
-// pass) suits long, skewed runs, larger radices fewer passes over short ones.
+// slot, then the head slots of c0+1..c1: in record order a segment's
+// records are consecutive.  A single call sums them by a block walk on quads
+// (k_seg_up): each pass walks the records of 2^LR units (chunks, then blocks
+// of them) per quad, with a point spread over the quad's four lanes (three
+// dependent multiplies per add), writes the segments that end in the block
+// and leaves one head and one tail record per block for the next pass, whose
+// work is 2^LR times smaller.  The batched launch keeps a radix-R tree over
+// each run (R = 2, 4, 8 or 16: ceil(log_R(chunks)) passes of at most R - 1
+// adds per lane, the run's first slot written out in the pass that completes
+// it); a wave there waits for its lane with the most adds.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -1014,7 +1018,8 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   }
 }
 
-// One pass of the run tree with radix 2^LR and stride d: lane c (a level-1
+// One pass of the run tree of the batched launch (k_agg_tree; single calls use
+// the block walk k_seg_up) with radix 2^LR and stride d: lane c (a level-1
 // chunk) adds partials c + d, ..., c + (2^LR - 1) d of a run into its own
 // when its offset in the run is a multiple of 2^LR d, for the run its head
 // slot belongs to (offset >= 1) and for the run its tail slot starts (offset
@@ -1059,12 +1064,68 @@ __device__ __forceinline__ void seg_tree_lane(PtView out, PtView rec, const int6
   }
 }
 
-template <int LR>
-__global__ void __launch_bounds__(ATPB) k_seg_tree(PtView out, PtView rec, const int64_t *keys, size_t T,
-                                                   const int64_t *off, int ch1, size_t d) {
-  const size_t c = blockIdx.x * (size_t)ATPB + threadIdx.x;
-  if (c >= T) return;
-  seg_tree_lane<LR>(out, rec, keys, c, OffArr{off}, ch1, d);
+// Block walk over the chunk records, on quads (the latency-bound form of
+// the run tree).  The records of level p are two per unit (a unit is 2^(p LR)
+// level-1 chunks): a head record (the partial of the segment entering the
+// unit from before it) and a tail record (the partial of the segment that
+// starts in the unit and leaves it), key -1 when absent.  In record order
+// (head then tail, unit by unit) the records of one segment are consecutive:
+// the tail record of its first unit, then the head records of the units it
+// covers.  Quad b walks the records of units [2^LR b, 2^LR (b + 1)) in that
+// order with one coordinate per lane (a point add is three dependent field
+// multiplies on the quad), sums each segment's records, and writes the sum
+// of a segment that started in the block and ends in it to the output; a
+// segment entering the block from before leaves the block's head record, one
+// leaving it the block's tail record (level p + 1).  The last level has one
+// block, and every sum goes to the output.  Work per pass shrinks by 2^LR.
+template <int LR, class OffT>
+__global__ void __launch_bounds__(ATPB) k_seg_up(PtView out, PtView rin, const int64_t *kin, size_t nu,
+                                                 PtView rout, int64_t *kout, const OffT off, int ch1, int ulog) {
+  const int c = threadIdx.x & 3;
+  const size_t b = (blockIdx.x * (size_t)ATPB + threadIdx.x) >> 2;
+  const size_t u0 = b << LR;
+  if (u0 >= nu) return;
+  const size_t u1 = u0 + ((size_t)1 << LR) < nu ? u0 + ((size_t)1 << LR) : nu;
+  int64_t key = -1, last = 0;           // open segment and its last level-1 chunk
+  bool before = false;                  // it entered the block from before
+  fe acc, x;
+  int64_t hk = -1, tk = -1;             // keys of the block's records
+#pragma unroll 1
+  for (size_t u = u0; u < u1; ++u) {
+    const int64_t uend = (int64_t)((u + 1) << ulog);       // first chunk after unit u
+#pragma unroll
+    for (int slot = 0; slot < 2; ++slot) {
+      const int64_t k = kin[2 * u + slot];
+      if (k < 0) continue;
+      load_fe(x, coord(rin, c, 2 * u + slot));
+      if (k == key) {
+        quad_add(acc, x, c);
+      } else {                          // a segment's first record in the block
+        acc = x;
+        key = k;
+        before = slot == 0 && u == u0;
+        last = (off[k + 1] - 1) >> ch1;
+      }
+      if (last < uend) {                // the segment ends in unit u
+        if (before) {
+          store_fe(coord_w(rout, c, 2 * b), acc);
+          hk = key;
+        } else {
+          store_fe(coord_w(out, c, (size_t)key), acc);
+        }
+        key = -1;
+      }
+    }
+  }
+  if (key >= 0) {                       // the open segment leaves the block
+    store_fe(coord_w(rout, c, 2 * b + (before ? 0 : 1)), acc);
+    if (before) hk = key;
+    else tk = key;
+  }
+  if (c == 0 && kout) {
+    kout[2 * b] = hk;
+    kout[2 * b + 1] = tk;
+  }
 }
 
 // point-contiguous copy of src into the scratch planes' tails (aos_slot)
@@ -1114,29 +1175,45 @@ static hipError_t launch_seg_level1(PtView out, PtView rec, int64_t *keys, PtVie
   return hipGetLastError();
 }
 
-static hipError_t launch_seg_tree(PtView out, PtView rec, const int64_t *keys, size_t T, const int64_t *off,
-                                  int ch1, int lr, hipStream_t s) {
-  const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
-  for (size_t d = 1; d < T; d <<= lr) {
+// the block walk's passes, levels alternating between (r0, k0) and (r1, k1)
+static hipError_t launch_seg_up(PtView out, PtView r0, int64_t *k0, PtView r1, int64_t *k1, size_t T,
+                                const int64_t *off, int ch1, int lr, hipStream_t s) {
+  size_t nu = T;
+  int ulog = 0;
+  for (int p = 0; nu > 1; ++p) {
+    const size_t nb = (nu + ((size_t)1 << lr) - 1) >> lr;
+    const dim3 grid((unsigned)((4 * nb + ATPB - 1) / ATPB));
+    PtView ri = (p & 1) ? r1 : r0, ro = (p & 1) ? r0 : r1;
+    int64_t *ki = (p & 1) ? k1 : k0, *ko = nb > 1 ? ((p & 1) ? k0 : k1) : nullptr;
+    const OffArr o{off};
     switch (lr) {
-      case 1: hipLaunchKernelGGL(k_seg_tree<1>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
-      case 2: hipLaunchKernelGGL(k_seg_tree<2>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
-      case 3: hipLaunchKernelGGL(k_seg_tree<3>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
-      case 4: hipLaunchKernelGGL(k_seg_tree<4>, grid, dim3(ATPB), 0, s, out, rec, keys, T, off, ch1, d); break;
+      case 1: hipLaunchKernelGGL((k_seg_up<1, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
+      case 2: hipLaunchKernelGGL((k_seg_up<2, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
+      case 3: hipLaunchKernelGGL((k_seg_up<3, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
+      case 4: hipLaunchKernelGGL((k_seg_up<4, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
       default: return hipErrorInvalidValue;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    nu = nb;
+    ulog += lr;
   }
   return hipSuccess;
 }
 
 // Scratch of the segmented sum, as points of one array: the level-1 records
 // (two per chunk, for the smallest chunks so that any schedule fits) in plane
-// slots [0, R), their int64 segment keys in plane-0 slots [R, R + R/4).
-size_t segsum_scratch_points(size_t n) {
+// slots [0, R), their int64 segment keys in plane-0 slots [R, R + R/4); the
+// block walk's levels alternate between those and a second area of R/2 + 2
+// records from slot B = R + R/4 + 8 (keys in plane-0 slots after them).
+static size_t segsum_area2(size_t n) {
   const size_t R = 2 * tiles(n, CH_MIN);
   return R + R / 4 + 8;
+}
+size_t segsum_scratch_points(size_t n) {
+  const size_t R = 2 * tiles(n, CH_MIN);
+  const size_t R2 = R / 2 + 2;
+  return segsum_area2(n) + R2 + R2 / 4 + 8;
 }
 
 // the point-contiguous source copy starts at plane slot segsum_scratch_points(n)
@@ -1167,7 +1244,10 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
   int64_t *keys = reinterpret_cast<int64_t *>(scratch.w + 2 * tiles(n_elems, CH_MIN) * 8);
   e = launch_seg_level1(out, rec, keys, src, n_elems, d_off, nseg, gather, ch1, aos, aos_base, s);
   if (e != hipSuccess || T <= 1) return e;
-  return launch_seg_tree(out, rec, keys, T, d_off, ch1, lr, s);
+  const size_t a2 = segsum_area2(n_elems), R2 = tiles(n_elems, CH_MIN) + 2;
+  const PtView rec2{scratch.w + a2 * 8, scratch.cap};
+  int64_t *keys2 = reinterpret_cast<int64_t *>(scratch.w + (a2 + R2) * 8);
+  return launch_seg_up(out, rec, keys, rec2, keys2, T, d_off, ch1, lr, s);
 }
 
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
