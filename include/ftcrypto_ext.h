@@ -260,8 +260,8 @@ ft_error ftx_server_stats(ftx_server server, uint64_t *bytes_in_use, uint64_t *b
  * 0: n < 2^14, 1: < 2^18, 2: < 2^22, 3: larger).  seg1 and scan are log2
  * elements per lane (4..7) at segmented-sum level 1 and at the scan's first
  * level (the levels above it take 16 per lane); segu
- * (1..4) is log2 of the radix of the tree that adds the partial sums of
- * segments spanning several level-1 chunks; reduce_blocks (1..4096) bounds
+ * (1..4) is log2 of the units a quad walks per pass when the partial sums of
+ * segments spanning several level-1 chunks are added; reduce_blocks (1..4096) bounds
  * the first reduce pass; u32_window
  * (ciphertext x u32) and varbase_window (ft_mul, ft_scale) are 4 (radix-16
  * digits, 8-entry table) or 3 (radix 8, 4 entries); lanes per ciphertext of

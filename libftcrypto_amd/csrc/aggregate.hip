@@ -18,16 +18,17 @@
 // began before the chunk) or tail slot (segment continues after it).  A
 // segment spanning chunks c0..c1 thus has one partial per chunk: c0's tail
 // slot, then the head slots of c0+1..c1: in record order a segment's
-// records are consecutive.  A single call sums them by a block walk on quads
-// (k_seg_up): each pass walks the records of 2^LR units (chunks, then blocks
-// of them) per quad, with a point spread over the quad's four lanes (three
+// records are consecutive.  They are summed by a block walk on quads
+// (k_seg_up, and k_agg_up over every job of a batched launch): each pass
+// walks the records of 2^LR units (chunks, then blocks of them; LR 1..4,
+// tuned) per quad, with a point spread over the quad's four lanes (three
 // dependent multiplies per add), writes the segments that end in the block
 // and leaves one head and one tail record per block for the next pass, whose
-// work is 2^LR times smaller.  The batched launch keeps a radix-R tree over
-// each run (R = 2, 4, 8 or 16: ceil(log_R(chunks)) passes of at most R - 1
-// adds per lane, the run's first slot written out in the pass that completes
-// it); a wave there waits for its lane with the most adds.
+// work is 2^LR times smaller.  (A radix tree over each run with one lane per
+// chunk, which this replaced, spent its first passes on waves where a few
+// lanes added and the rest waited.)
 #include <algorithm>
+#include <vector>
 #include <atomic>
 #include <cstring>
 #include <mutex>
@@ -1018,54 +1019,8 @@ __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)
   }
 }
 
-// One pass of the run tree of the batched launch (k_agg_tree; single calls use
-// the block walk k_seg_up) with radix 2^LR and stride d: lane c (a level-1
-// chunk) adds partials c + d, ..., c + (2^LR - 1) d of a run into its own
-// when its offset in the run is a multiple of 2^LR d, for the run its head
-// slot belongs to (offset >= 1) and for the run its tail slot starts (offset
-// 0).  Runs that this pass completes write their sum to the output.  Reads
-// and writes of one pass touch disjoint slots.
-template <int LR>
-__device__ __forceinline__ void tree_step(PtView rec, size_t slot, size_t c, size_t c1, size_t d, ge_p3 &acc) {
-  load_p3(acc, rec, slot);
-#pragma unroll 1
-  for (int j = 1; j < (1 << LR); ++j) {
-    const size_t x = c + (size_t)j * d;
-    if (x > c1) break;
-    ge_p3 y;
-    load_p3(y, rec, 2 * x);
-    ge_p3_add(acc, acc, y);
-  }
-}
-
-template <int LR, class OffT>
-__device__ __forceinline__ void seg_tree_lane(PtView out, PtView rec, const int64_t *keys, size_t c,
-                                              const OffT off, int ch1, size_t d) {
-  const size_t span = d << LR;
-  const int64_t h = keys[2 * c];
-  if (h >= 0) {
-    const size_t c0 = (size_t)(off[h] >> ch1), c1 = (size_t)((off[h + 1] - 1) >> ch1);
-    if ((c - c0) % span == 0 && c + d <= c1) {
-      ge_p3 acc;
-      tree_step<LR>(rec, 2 * c, c, c1, d, acc);
-      store_p3(rec, 2 * c, acc);
-    }
-  }
-  const int64_t t = keys[2 * c + 1];
-  if (t >= 0) {
-    const size_t c1 = (size_t)((off[t + 1] - 1) >> ch1);
-    const size_t len = c1 - c + 1;                 // >= 2 chunks
-    if (d < len) {                                 // not completed by an earlier pass
-      ge_p3 acc;
-      tree_step<LR>(rec, 2 * c + 1, c, c1, d, acc);
-      if (span >= len) store_p3(out, (size_t)t, acc);
-      else store_p3(rec, 2 * c + 1, acc);
-    }
-  }
-}
-
-// Block walk over the chunk records, on quads (the latency-bound form of
-// the run tree).  The records of level p are two per unit (a unit is 2^(p LR)
+// Block walk over the chunk records, on quads (a latency-bound sum: a
+// few adds per quad and pass).  The records of level p are two per unit (a unit is 2^(p LR)
 // level-1 chunks): a head record (the partial of the segment entering the
 // unit from before it) and a tail record (the partial of the segment that
 // starts in the unit and leaves it), key -1 when absent.  In record order
@@ -1079,10 +1034,8 @@ __device__ __forceinline__ void seg_tree_lane(PtView out, PtView rec, const int6
 // leaving it the block's tail record (level p + 1).  The last level has one
 // block, and every sum goes to the output.  Work per pass shrinks by 2^LR.
 template <int LR, class OffT>
-__global__ void __launch_bounds__(ATPB) k_seg_up(PtView out, PtView rin, const int64_t *kin, size_t nu,
-                                                 PtView rout, int64_t *kout, const OffT off, int ch1, int ulog) {
-  const int c = threadIdx.x & 3;
-  const size_t b = (blockIdx.x * (size_t)ATPB + threadIdx.x) >> 2;
+__device__ __forceinline__ void seg_up_quad(PtView out, PtView rin, const int64_t *kin, size_t nu, PtView rout,
+                                            int64_t *kout, const OffT off, int ch1, int ulog, size_t b, int c) {
   const size_t u0 = b << LR;
   if (u0 >= nu) return;
   const size_t u1 = u0 + ((size_t)1 << LR) < nu ? u0 + ((size_t)1 << LR) : nu;
@@ -1126,6 +1079,13 @@ __global__ void __launch_bounds__(ATPB) k_seg_up(PtView out, PtView rin, const i
     kout[2 * b] = hk;
     kout[2 * b + 1] = tk;
   }
+}
+
+template <int LR>
+__global__ void __launch_bounds__(ATPB) k_seg_up(PtView out, PtView rin, const int64_t *kin, size_t nu,
+                                                 PtView rout, int64_t *kout, const int64_t *off, int ch1, int ulog) {
+  const size_t g = blockIdx.x * (size_t)ATPB + threadIdx.x;
+  seg_up_quad<LR>(out, rin, kin, nu, rout, kout, OffArr{off}, ch1, ulog, g >> 2, (int)(g & 3));
 }
 
 // point-contiguous copy of src into the scratch planes' tails (aos_slot)
@@ -1185,12 +1145,11 @@ static hipError_t launch_seg_up(PtView out, PtView r0, int64_t *k0, PtView r1, i
     const dim3 grid((unsigned)((4 * nb + ATPB - 1) / ATPB));
     PtView ri = (p & 1) ? r1 : r0, ro = (p & 1) ? r0 : r1;
     int64_t *ki = (p & 1) ? k1 : k0, *ko = nb > 1 ? ((p & 1) ? k0 : k1) : nullptr;
-    const OffArr o{off};
     switch (lr) {
-      case 1: hipLaunchKernelGGL((k_seg_up<1, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
-      case 2: hipLaunchKernelGGL((k_seg_up<2, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
-      case 3: hipLaunchKernelGGL((k_seg_up<3, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
-      case 4: hipLaunchKernelGGL((k_seg_up<4, OffArr>), grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, o, ch1, ulog); break;
+      case 1: hipLaunchKernelGGL(k_seg_up<1>, grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, off, ch1, ulog); break;
+      case 2: hipLaunchKernelGGL(k_seg_up<2>, grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, off, ch1, ulog); break;
+      case 3: hipLaunchKernelGGL(k_seg_up<3>, grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, off, ch1, ulog); break;
+      case 4: hipLaunchKernelGGL(k_seg_up<4>, grid, dim3(ATPB), 0, s, out, ri, ki, nu, ro, ko, off, ch1, ulog); break;
       default: return hipErrorInvalidValue;
     }
     const hipError_t e = hipGetLastError();
@@ -1408,18 +1367,65 @@ __device__ __forceinline__ int batch_job_of(const BBatch &B, size_t g, bool seg)
   return j;
 }
 
+// units of job j at level q of the block walk (level 0: its chunks; a job
+// already down to one unit has no further level)
+__device__ __forceinline__ size_t agg_units(const BJob &b, int q, int lr) {
+  size_t nu = b.cn;
+  for (int i = 0; i < q && nu > 1; ++i) nu = (nu + ((size_t)1 << lr) - 1) >> lr;
+  return nu;
+}
+__device__ __forceinline__ size_t agg_blocks(const BJob &b, int q, int lr) {
+  const size_t nu = agg_units(b, q, lr);
+  return nu > 1 ? (nu + ((size_t)1 << lr) - 1) >> lr : 0;
+}
+
+// Pass p (1-based) of the block walk over every chunk-kind job of a batch:
+// the jobs' blocks are numbered one job after another (job j from the sum of
+// the blocks of the jobs before it, which also places its records of the
+// pass's output level); level 0 is the persistent walk's records at the
+// job's chunk base, odd levels live in the second area (r1, k1), even ones
+// in the first.
 template <int LR>
-__global__ void __launch_bounds__(ATPB) k_agg_tree(const BBatch B, PtView rec, const int64_t *keys, int ch1,
-                                                   size_t d) {
-  const size_t g = blockIdx.x * (size_t)ATPB + threadIdx.x;
-  const int j = batch_job_of(B, g & ~(size_t)63, false);
+__global__ void __launch_bounds__(ATPB) k_agg_up(const BBatch B, PtView r0, int64_t *k0, PtView r1, int64_t *k1,
+                                                 int ch1, int p) {
+  const size_t g = (blockIdx.x * (size_t)ATPB + threadIdx.x) >> 2;
+  const int c = (int)(threadIdx.x & 3);
+  size_t base_out = 0, base_in = 0;
+  int j = -1;
+#pragma unroll 1
+  for (int q = 0; q < B.J; ++q) {
+    if (B.job[q].kind != 0) continue;
+    const size_t nb = agg_blocks(B.job[q], p - 1, LR);
+    if (g < base_out + nb) { j = q; break; }
+    base_out += nb;
+    base_in += agg_blocks(B.job[q], p - 2, LR);
+  }
   if (j < 0) return;
-  const size_t c = g - B.job[j].cb;
-  if (c >= B.job[j].cn || d >= B.job[j].cn) return;
-  const PtView r{rec.w + (size_t)B.job[j].cb * 2 * 8, rec.cap};
-  const int64_t *k = keys + 2 * (size_t)B.job[j].cb;
-  if (B.job[j].off) seg_tree_lane<LR>(B.job[j].out, r, k, c, OffArr{B.job[j].off}, ch1, d);
-  else seg_tree_lane<LR>(B.job[j].out, r, k, c, OffUni{B.job[j].len, (int64_t)B.job[j].n}, ch1, d);
+  const BJob &b = B.job[j];
+  const size_t nu = agg_units(b, p - 1, LR);
+  PtView rin, rout;
+  const int64_t *kin;
+  int64_t *kout;
+  if (p == 1) {
+    rin = PtView{r0.w + (size_t)b.cb * 2 * 8, r0.cap};
+    kin = k0 + 2 * (size_t)b.cb;
+  } else {
+    const PtView ra = (p - 1) & 1 ? r1 : r0;
+    int64_t *ka = (p - 1) & 1 ? k1 : k0;
+    rin = PtView{ra.w + base_in * 2 * 8, ra.cap};
+    kin = ka + 2 * base_in;
+  }
+  {
+    const PtView ra = p & 1 ? r1 : r0;
+    int64_t *ka = p & 1 ? k1 : k0;
+    rout = PtView{ra.w + base_out * 2 * 8, ra.cap};
+    kout = ka + 2 * base_out;
+  }
+  const int ulog = (p - 1) * LR;
+  if (b.off)
+    seg_up_quad<LR>(b.out, rin, kin, nu, rout, kout, OffArr{b.off}, ch1, ulog, g - base_out, c);
+  else
+    seg_up_quad<LR>(b.out, rin, kin, nu, rout, kout, OffUni{b.len, (int64_t)b.n}, ch1, ulog, g - base_out, c);
 }
 
 __global__ void __launch_bounds__(ATPB) k_agg_empty(const BBatch B) {
@@ -1440,6 +1446,7 @@ struct BatchPlan {
   size_t maxT = 0;                         // most chunks of one job
   int nred = 0;                            // reduce jobs (one block-sum region each)
   size_t rec_pts = 0, part_base = 0, red_base = 0, scan_base = 0, total = 0;
+  size_t rec2_base = 0, rec2_n = 0;        // the block walk's second record area
   size_t scan_at[AGG_BATCH_MAX];           // scan jobs (by caller's index): their scratch region
   int scan_ch[AGG_BATCH_MAX];
   int from[AGG_BATCH_MAX];                 // caller's index of each laid-out job
@@ -1509,9 +1516,13 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
   p.B.tiles = (unsigned)nt;
   p.fits = nt < (1ull << 32) && p.C < (1ull << 32) && p.Sg < (1ull << 32) && p.P < (1ull << 32);
   // scratch: records [0, 2C), keys (2C int64) and the tile counter in plane 0
-  // after them, reduce partials, one reduce's block sums, scan regions
+  // after them, the block walk's second record area (level-1 blocks: at most
+  // C + J records) with its keys in plane 0, reduce partials, one reduce's
+  // block sums, scan regions
   p.rec_pts = 2 * p.C + (2 * p.C * 8 + 31) / 32 + 1;
-  p.part_base = p.rec_pts;
+  p.rec2_base = p.rec_pts;
+  p.rec2_n = p.C + 2 * AGG_BATCH_MAX + 8;
+  p.part_base = p.rec2_base + p.rec2_n + (p.rec2_n * 8 + 31) / 32 + 1;
   p.red_base = p.part_base + p.P;
   p.scan_base = p.red_base + REDUCE_BLOCKS_MAX * (size_t)p.nred;
   size_t at = p.scan_base;
@@ -1647,15 +1658,27 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const dim3 tgrid((unsigned)((p.C + ATPB - 1) / ATPB));
-  for (size_t d = 1; d < p.maxT; d <<= lr) {
+  // block walk over the jobs' chunk records, a pass per level until every
+  // job is down to one unit
+  const PtView rec2 = at(p.rec2_base);
+  int64_t *keys2 = reinterpret_cast<int64_t *>(scratch.w + (p.rec2_base + p.rec2_n) * 8);
+  std::vector<size_t> nu(p.B.J);
+  for (int q = 0; q < p.B.J; ++q) nu[q] = p.B.job[q].kind == 0 ? p.B.job[q].cn : 0;
+  for (int pass = 1;; ++pass) {
+    size_t nb = 0;
+    for (int q = 0; q < p.B.J; ++q)
+      if (nu[q] > 1) nb += (nu[q] + ((size_t)1 << lr) - 1) >> lr;
+    if (!nb) break;
+    const dim3 ugrid((unsigned)((4 * nb + ATPB - 1) / ATPB));
     switch (lr) {
-      case 1: hipLaunchKernelGGL(k_agg_tree<1>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
-      case 2: hipLaunchKernelGGL(k_agg_tree<2>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
-      case 3: hipLaunchKernelGGL(k_agg_tree<3>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
-      default: hipLaunchKernelGGL(k_agg_tree<4>, tgrid, dim3(ATPB), 0, s, p.B, rec, keys, ch1, d); break;
+      case 1: hipLaunchKernelGGL(k_agg_up<1>, ugrid, dim3(ATPB), 0, s, p.B, rec, keys, rec2, keys2, ch1, pass); break;
+      case 2: hipLaunchKernelGGL(k_agg_up<2>, ugrid, dim3(ATPB), 0, s, p.B, rec, keys, rec2, keys2, ch1, pass); break;
+      case 3: hipLaunchKernelGGL(k_agg_up<3>, ugrid, dim3(ATPB), 0, s, p.B, rec, keys, rec2, keys2, ch1, pass); break;
+      default: hipLaunchKernelGGL(k_agg_up<4>, ugrid, dim3(ATPB), 0, s, p.B, rec, keys, rec2, keys2, ch1, pass); break;
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (int q = 0; q < p.B.J; ++q)
+      if (nu[q] > 1) nu[q] = (nu[q] + ((size_t)1 << lr) - 1) >> lr;
   }
   // the scans' upper levels and second walks and the reduces' block sums go
   // to auxiliary streams forked from s here (eagerly; inside a graph capture

@@ -58,7 +58,7 @@ hipError_t launch_reduce_blocks(PtView part, PtView v, size_t n, unsigned grid, 
 constexpr int TUNE_CLASSES = 4;
 struct AggTune {
   int seg1;          // log2 elements per lane, segmented-sum level 1
-  int segu;          // log2 radix of the segmented-sum run tree (1..4)
+  int segu;          // log2 units per quad and pass of the segmented-sum block walk (1..4)
   int scan;          // log2 elements per lane, every scan level
   int reduce;        // most blocks of 256 lanes in the first reduce pass
   int u32w;          // window width of the 32-bit-multiplier kernels (3 or 4)
@@ -67,7 +67,7 @@ struct AggTune {
   int scanlb;        // scan: 0 chunk levels (two walks, upper levels between), 1 single-pass look-back
 };
 constexpr int CH_MIN = 4, CH_MAX = 7;             // allowed chunk log2 range
-constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed run-tree radix log2 range
+constexpr int TREE_LR_MIN = 1, TREE_LR_MAX = 4;   // allowed log2 units per block-walk pass
 constexpr unsigned REDUCE_BLOCKS_MAX = 4096;
 // reduces of at most this many points run their first pass on quad lanes
 constexpr size_t REDUCE_QUAD_MAX = (size_t)1 << 18;

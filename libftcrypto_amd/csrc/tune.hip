@@ -4,8 +4,9 @@
 // A schedule is chosen per size class of the input length n (n < 2^14,
 // < 2^18, < 2^22, larger):
 //   seg1    log2 elements per lane at segmented-sum level 1 (4..7)
-//   segu    log2 radix of the tree that adds the chunk partials of segments
-//           spanning several level-1 chunks (1..4: radix 2, 4, 8, 16)
+//   segu    log2 of the units each quad walks per pass of the block walk
+//           that adds the chunk partials of segments spanning several
+//           level-1 chunks (1..4: 2, 4, 8 or 16 units)
 //   scan    log2 elements per lane at a scan's first level (4..7; the
 //           levels above it take 16)
 //   reduce  most blocks of 256 lanes in the first reduce pass (512..4096)
@@ -52,7 +53,7 @@ namespace ft {
 namespace {
 
 // Defaults per size class: what the tuner picks on MI355X (elements per lane
-// 2^4 below 2^22 and 2^6 above, radix-2 then radix-4 run trees, radix-16
+// 2^4 below 2^22 and 2^6 above, blocks of 2 then 4 units in the block walk, radix-16
 // windows for the variable base below 2^18, two lanes per ciphertext of an
 // encryption), so an untuned process is close to a tuned one.
 constexpr AggTune DEFAULT_TUNE[TUNE_CLASSES] = {
