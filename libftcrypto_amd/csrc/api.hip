@@ -47,6 +47,7 @@ struct Lib {
   int device = 0;
   hipStream_t stream = nullptr;
   uint32_t *btab = nullptr;        // comb table of B
+  uint32_t *btab32 = nullptr;      // radix-32 comb table of B
   int8_t *lnaf = nullptr;          // width-3 NAF digits of the group order l
   int lnaf_len = 0;
   unsigned long long *flag = nullptr;
@@ -363,6 +364,7 @@ int msg_digits(int bits) { return bits <= 0 || bits >= 64 ? 17 : (bits + 3) / 4 
 
 namespace ft {
 const uint32_t *base_table() { return G.btab; }
+const uint32_t *base_table32() { return G.btab32; }
 const uint32_t *key_table(const ftx_key_s *key) { return key->table; }
 }  // namespace ft
 
@@ -389,12 +391,14 @@ ft_error ft_crypto_init(void) {
   if (std::string(p.gcnArchName).rfind("gfx950", 0) != 0) return FT_ERR_INSUFFICIENT_DEVICE_OR_DRIVER;
   CK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
   CK(hipMalloc(&G.btab, COMB_TABLE_WORDS * 4));
+  CK(hipMalloc(&G.btab32, COMB32_TABLE_WORDS * 4));
   static int8_t naf[256];
   G.lnaf_len = naf3_of_l(naf);
   CK(hipMalloc(&G.lnaf, sizeof naf));
   CK(hipMalloc(&G.flag, 8));
   CK(hipMemcpyAsync(G.lnaf, naf, sizeof naf, hipMemcpyHostToDevice, G.stream));
   CK(launch_build_comb(G.btab, PtView{nullptr, 0}, 0, true, G.stream));
+  CK(launch_build_comb32(G.btab32, G.stream));
   CK(hipStreamSynchronize(G.stream));
   CK(noalloc_resolve(G.device));
   tune_init_from_env();
@@ -466,7 +470,7 @@ ft_error ft_array_init_scalar(ft_ge25519_array *array, size_t n, const uint8_t *
     } else {
       void *d;
       CKE(stage(scalar, 32, &d));
-      CK(launch_fixed_base(view(g.a), d, false, 1, G.btab, G.stream));
+      CK(launch_fixed_base(view(g.a), d, false, 1, G.btab32, G.stream));
       CK(launch_broadcast(view(g.a), n, view(g.a), 0, G.stream));
     }
   }
@@ -713,7 +717,7 @@ ft_error ft_array_from_scalars(ft_ge25519_array *array, const uint8_t *scalars, 
   CKE(alloc_array(&g.a, n));
   void *d;
   CKE(stage(scalars, n * 32, &d));
-  CK(launch_fixed_base(view(g.a), d, false, n, G.btab, G.stream));
+  CK(launch_fixed_base(view(g.a), d, false, n, G.btab32, G.stream));
   CK(hipStreamSynchronize(G.stream));
   *array = g.release();
   return FT_ERR_NO_ERROR;
@@ -725,7 +729,7 @@ ft_error ft_array_from_small_scalars(ft_ge25519_array *array, const uint64_t *sm
   CKE(alloc_array(&g.a, n));
   void *d;
   CKE(stage(small, n * 8, &d));
-  CK(launch_fixed_base(view(g.a), d, true, n, G.btab, G.stream));
+  CK(launch_fixed_base(view(g.a), d, true, n, G.btab32, G.stream));
   CK(hipStreamSynchronize(G.stream));
   *array = g.release();
   return FT_ERR_NO_ERROR;
@@ -1022,7 +1026,7 @@ ft_error ftx_mul_u32(ft_ge25519_array c1, ft_ge25519_array c2, const uint32_t *k
 }
 
 ft_error ftx_from_scalars_dev(ft_ge25519_array out, const void *d_scalars, void *stream) {
-  CK(launch_fixed_base(view(out), d_scalars, false, out->n, G.btab, S(stream)));
+  CK(launch_fixed_base(view(out), d_scalars, false, out->n, G.btab32, S(stream)));
   return FT_ERR_NO_ERROR;
 }
 

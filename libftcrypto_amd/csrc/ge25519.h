@@ -216,6 +216,16 @@ FT_HD void ge_p3_to_precomp(ge_precomp &r, const ge_p3 &p) {
   fe_mul(r.xyd, xy, FE_D);
 }
 
+// the extended point of an entry in that form (one product): x = ypx_h -
+// ymx_h, y = ypx_h + ymx_h, Z = 1, T = (d x y) / d; the fields may be
+// swapped and d x y negated (ge_precomp_cneg)
+FT_HD void ge_precomp_to_p3(ge_p3 &r, const ge_precomp &q) {
+  fe_sub(r.X, q.ypx_h, q.ymx_h); fe_carry(r.X);
+  fe_add(r.Y, q.ypx_h, q.ymx_h); fe_carry(r.Y);
+  fe_1(r.Z);
+  fe_mul(r.T, q.xyd, FE_INV_D);
+}
+
 // ---------------------------------------------------------------- encoding
 
 // compressed encoding words with affine x, y known (sodium_ge.cu:382-394)

@@ -10,7 +10,8 @@ hipError_t launch_broadcast(PtView d, size_t n, PtView src, size_t idx, hipStrea
 hipError_t launch_copy(PtView d, PtView src, size_t n, hipStream_t s);
 hipError_t launch_add(PtView res, PtView b, size_t n, bool sub, hipStream_t s);
 hipError_t launch_neg(PtView v, size_t n, hipStream_t s);
-hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table,
+// out[i] = k_i B over B's radix-32 comb table (launch_build_comb32)
+hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table32,
                              hipStream_t s);
 hipError_t launch_varbase(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, hipStream_t s);
 hipError_t launch_varbase_w(PtView pts, const uint32_t *sc, size_t sc_stride_words, size_t n, int w,
@@ -18,6 +19,7 @@ hipError_t launch_varbase_w(PtView pts, const uint32_t *sc, size_t sc_stride_wor
 hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s);
 hipError_t launch_mul_u32_w(PtView a, PtView b, bool two, const uint32_t *k, size_t n, int w, hipStream_t s);
 hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
+hipError_t launch_build_comb32(uint32_t *table, hipStream_t s);
 // gate: a device word; when it is nonzero at launch time the launch writes
 // nothing (the message-bound check of bounded encryptions), null: ungated
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
@@ -129,6 +131,7 @@ hipError_t noalloc_scratch(PtView *v);
 
 // api.hip: device tables owned by the library
 const uint32_t *base_table();
+const uint32_t *base_table32();
 const uint32_t *key_table(const struct ftx_key_s *key);
 
 }  // namespace ft

@@ -50,32 +50,39 @@ __global__ void __launch_bounds__(TPB) k_neg(PtView v, size_t n) {
 
 // ------------------------------------------------------ fixed-base comb
 
-// out[i] = k_i * P with P's comb table (sodium_ge.cu:743-792 structure; the
-// reference routes this through the variable-base routine on the GPU,
-// ftcrypto.cu:895-913; the result is the same point).  SRC: 0 = 32-byte
-// scalars, 1 = uint64 scalars (ftcrypto.cu:937-963 packing).
+// out[i] = k_i * B (the reference routes this through its variable-base
+// routine on the GPU, ftcrypto.cu:895-913, and keeps a radix-16 comb for the
+// CPU, sodium_ge.cu:743-792; the point is the same).  B's radix-32 comb has
+// 51 rows of 16 entries, (j+1) 32^r B (89 KiB in global memory, read by
+// scalar loads: no LDS): one mixed add per row and no doubling, 50 adds per
+// 256-bit scalar where a radix-16 comb takes 64 adds and four doublings.
+// The sum is the reference's multiple: k is rewritten to the multiple its
+// radix-16 digit rule gives (scalar_effective; the same as k below 2^255),
+// reduced mod l and negated mod l when negative, which leaves k B unchanged
+// (l B = 0).  SRC: 0 = 32-byte scalars, 1 = uint64 scalars over 13 rows
+// (ftcrypto.cu:937-963 packing).
 template <int SRC>
-__global__ void __launch_bounds__(TPB) k_fixed_base(PtView out, const void *sc, size_t n,
-                                                    const uint32_t *table) {
-  __shared__ __align__(16) uint32_t tab[COMB_TABLE_WORDS];
-  lds_load_table(tab, table);
-  __syncthreads();
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 4))) k_fixed_base32(PtView out, const void *sc, size_t n,
+                                                    const uint32_t *__restrict__ tab) {
   const size_t i = blockIdx.x * (size_t)TPB + threadIdx.x;
   if (i >= n) return;
-  uint32_t k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr int ND = SRC == 0 ? COMB32_ROWS : 13;
+  uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (SRC == 0) {
+    uint32_t k[8];
     load_scalar(k, static_cast<const uint32_t *>(sc), i);
+    const uint32_t neg = scalar_effective(m, k);
+    scalar_mod_l(m);
+    scalar_cneg_l(m, neg);
   } else {
     const uint64_t x = static_cast<const uint64_t *>(sc)[i];
-    k[0] = (uint32_t)x;
-    k[1] = (uint32_t)(x >> 32);
+    m[0] = (uint32_t)x;
+    m[1] = (uint32_t)(x >> 32);
   }
-  uint32_t e[16], odd[8], even[8];
-  recode16(e, k);
-  split_odd_even(odd, even, e);
+  uint32_t e[(ND + 3) / 4];
+  recode32<ND>(e, m);
   ge_p3 h;
-  ge_p3_0(h);
-  comb_run<8>(h, tab, tab, odd, even, 32, 32);
+  comb32_run<(ND + 3) / 4>(h, tab, e, ND);
   store_p3(out, i, h);
 }
 
@@ -128,6 +135,29 @@ __global__ void __launch_bounds__(TPB) k_build_comb(uint32_t *table, PtView src,
   ge_precomp en;
   ge_p3_to_precomp(en, h);
   comb_entry_store(table + (r * 8 + j) * COMB_ENTRY_WORDS, en);
+}
+
+// table[r][j] = (j+1) * 32^r * B in the same form, r < 51, j < 16: one
+// entry per lane
+__global__ void __launch_bounds__(TPB) k_build_comb32(uint32_t *table) {
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= COMB32_ROWS * 16) return;
+  const int r = t >> 4, j = t & 15;
+  uint32_t k[8] = {0, 0, 0, 0, 0, 0, 0, 0}, e[16];
+  const int bit = 5 * r, w = bit >> 5, sh = bit & 31;  // (j+1) << (5 r) < 2^255
+  k[w] = (uint32_t)(j + 1) << sh;
+  if (sh > 27) k[w + 1] = (uint32_t)(j + 1) >> (32 - sh);
+  recode16(e, k);
+  uint32_t bw[8];
+  bw[0] = 0x66666658u;
+#pragma unroll
+  for (int q = 1; q < 8; ++q) bw[q] = 0x66666666u;
+  ge_p3 p, h;
+  ge_decode(p, bw);
+  varbase_windows(h, p, e, 0);
+  ge_precomp en;
+  ge_p3_to_precomp(en, h);
+  comb_entry_store(table + r * COMB32_ROW_WORDS + j * COMB_ENTRY_WORDS, en);
 }
 
 // ------------------------------------------------------ ElGamal encryption
@@ -552,11 +582,11 @@ hipError_t launch_neg(PtView v, size_t n, hipStream_t s) {
   FT_LAUNCH_CHECK();
 }
 
-hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table,
+hipError_t launch_fixed_base(PtView out, const void *sc, bool u64, size_t n, const uint32_t *table32,
                              hipStream_t s) {
   if (!n) return hipSuccess;
-  if (u64) hipLaunchKernelGGL(k_fixed_base<1>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table);
-  else hipLaunchKernelGGL(k_fixed_base<0>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table);
+  if (u64) hipLaunchKernelGGL(k_fixed_base32<1>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table32);
+  else hipLaunchKernelGGL(k_fixed_base32<0>, dim3(grid_for(n)), dim3(TPB), 0, s, out, sc, n, table32);
   FT_LAUNCH_CHECK();
 }
 
@@ -578,6 +608,11 @@ hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_
 
 hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s) {
   hipLaunchKernelGGL(k_build_comb, dim3(1), dim3(TPB), 0, s, table, src, idx, (int)use_base);
+  FT_LAUNCH_CHECK();
+}
+
+hipError_t launch_build_comb32(uint32_t *table, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_comb32, dim3((COMB32_ROWS * 16 + TPB - 1) / TPB), dim3(TPB), 0, s, table);
   FT_LAUNCH_CHECK();
 }
 

@@ -23,6 +23,10 @@ struct PtView {
 constexpr int COMB_ROWS = 32;
 constexpr int COMB_ENTRY_WORDS = 28;
 constexpr int COMB_TABLE_WORDS = COMB_ROWS * 8 * COMB_ENTRY_WORDS;   // 7168 words
+// radix-32 comb of the base point: 51 rows of 16 entries, (j+1) 32^r B
+constexpr int COMB32_ROWS = 51;
+constexpr int COMB32_ROW_WORDS = 16 * COMB_ENTRY_WORDS;
+constexpr int COMB32_TABLE_WORDS = COMB32_ROWS * COMB32_ROW_WORDS;  // 22848 words, 89.25 KiB
 
 __device__ __forceinline__ void ld_words(uint32_t w[8], const uint32_t *p) {
   const uint4 a = *reinterpret_cast<const uint4 *>(p);
@@ -200,16 +204,18 @@ __device__ __forceinline__ void comb_entry_store(uint32_t *dst, const ge_precomp
   dst[27] = 0;
 }
 
-// Constant-time selection of |b| * row entry from an LDS-resident row:
-// every entry is read (the LDS address never depends on b) and merged with
-// v_cndmask; b < 0 negates the result.  b outside [-8, 8] selects the
-// identity (the reference's cmov8 behaviour, sodium_ge.cu:458-488).
+// Constant-time selection of |b| * row entry from an LDS-resident row of NE
+// entries: every entry is read (the LDS address never depends on b) and
+// merged with v_cndmask; b < 0 negates the result.  b outside [-NE, NE]
+// selects the identity (for NE = 8 the reference's cmov8 behaviour,
+// sodium_ge.cu:458-488).
+template <int NE = 8>
 __device__ __forceinline__ void comb_select(ge_precomp &t, const uint32_t *row, int b) {
   const uint32_t neg = (uint32_t)b >> 31;
   const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
   ge_precomp_0(t);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < NE; ++j) {
     const uint32_t m = ct_eq(babs, (uint32_t)(j + 1));
     const uint32_t *e = row + j * COMB_ENTRY_WORDS;
     uint32_t w[28];
@@ -223,6 +229,34 @@ __device__ __forceinline__ void comb_select(ge_precomp &t, const uint32_t *row, 
       t.ypx_h.v[i] = m ? w[i] : t.ypx_h.v[i];
       t.ymx_h.v[i] = m ? w[9 + i] : t.ymx_h.v[i];
       t.xyd.v[i] = m ? w[18 + i] : t.xyd.v[i];
+    }
+  }
+  ge_precomp_cneg(t, neg);
+}
+
+// The same selection from a row in global memory at a wave-uniform address:
+// the words come in through scalar loads and are merged with v_bfi_b32
+// under a per-lane mask (a v_cndmask cannot take a scalar operand besides
+// its lane mask), so a row costs no LDS traffic.  Constant time as above:
+// every entry is read, at an address that never depends on b.
+template <int NE>
+__device__ __forceinline__ void comb_select_uniform(ge_precomp &t, const uint32_t *__restrict__ row, int b) {
+  const uint32_t neg = (uint32_t)b >> 31;
+  const uint32_t babs = (uint32_t)(b < 0 ? -b : b);
+  ge_precomp_0(t);
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+    uint32_t mk = 0u - (uint32_t)(babs == (uint32_t)(j + 1));
+    // keep the mask opaque: known to be 0 or ~0, the merge below would be
+    // turned back into a select, which needs each scalar word moved to a
+    // vector register first
+    asm volatile("" : "+v"(mk));
+    const uint32_t *e = row + j * COMB_ENTRY_WORDS;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      t.ypx_h.v[i] = (e[i] & mk) | (t.ypx_h.v[i] & ~mk);
+      t.ymx_h.v[i] = (e[9 + i] & mk) | (t.ymx_h.v[i] & ~mk);
+      t.xyd.v[i] = (e[18 + i] & mk) | (t.xyd.v[i] & ~mk);
     }
   }
   ge_precomp_cneg(t, neg);

@@ -3,6 +3,7 @@
 // variable-base window loop.
 #pragma once
 #include "layout.h"
+#include "scalar.h"
 
 namespace ft {
 
@@ -146,6 +147,29 @@ __device__ __forceinline__ void dbl4(ge_p3 &h) {
   ge_p1p1_to_p2(s, r);
   ge_p2_dbl(r, s);
   ge_p1p1_to_p3(h, r);
+}
+
+// Radix-32 comb of the base point: h = sum over rows r < nrows of d_r *
+// row r, digits from a low-first byte stream of NW words, rows read from
+// global memory by scalar loads.  Row 0's entry gives h directly
+// (ge_precomp_to_p3); every later row costs a select over its 16 entries
+// and one mixed add, with no doubling.
+template <int NW>
+__device__ __forceinline__ void comb32_run(ge_p3 &h, const uint32_t *__restrict__ tab, uint32_t s[NW],
+                                           int nrows) {
+  ge_precomp t;
+  ge_p1p1 r;
+  comb_select_uniform<16>(t, tab, pop_lo<NW>(s, 0));
+  ge_precomp_to_p3(h, t);
+  asm volatile("" : "+s"(nrows));
+#pragma unroll 1
+  for (int q = 1; q < nrows; ++q) {
+    const int b = pop_lo<NW>(s, q);
+    comb_select_uniform<16>(t, tab + q * COMB32_ROW_WORDS, b);
+    ge_madd(r, h, t);
+    ge_p1p1_to_p3(h, r);
+  }
+  pin_p3(h);
 }
 
 __device__ __forceinline__ void lds_load_table(uint32_t *lds, const uint32_t *g) {

@@ -298,7 +298,7 @@ ft_error ftx_autotune(int max_class, const char *path) {
   for (size_t i = 0; i < nmax; ++i) small[i] = i + 1;
   TCK(hipMemcpyAsync(sc.p, small.data(), nmax * 8, hipMemcpyHostToDevice, st.s));
   const PtView P{(uint32_t *)pts.p, nmax};
-  TCK(launch_fixed_base(P, sc.p, true, nmax, base_table(), st.s));
+  TCK(launch_fixed_base(P, sc.p, true, nmax, base_table32(), st.s));
   TCK(hipStreamSynchronize(st.s));
   // skewed segment lengths: lognormal(ln 16, 1.5) clipped to [1, 65536]
   // (median 16, mean about 49, a few segments spanning thousands of chunks),

@@ -3,6 +3,7 @@
 #include <cstring>
 #include "../../libftcrypto_amd/csrc/fe25519.h"
 #include "../../libftcrypto_amd/csrc/ge25519.h"
+#include "../../libftcrypto_amd/csrc/scalar.h"
 
 using namespace ft;
 
@@ -21,6 +22,31 @@ void h_fe_from_words(uint32_t *r, const uint32_t *w) { fe z; fe_from_words(z, w)
 void h_fe_sub(uint32_t *r, const uint32_t *a, const uint32_t *b) { fe z; fe_sub(z, ld(a), ld(b)); st(r, z); }
 void h_constants(uint32_t *d, uint32_t *d2, uint32_t *s) { st(d, FE_D); st(d2, FE_D2); st(s, FE_SQRTM1); }
 void h_inv2(uint32_t *r) { st(r, FE_INV2); }
+void h_inv_d(uint32_t *r) { st(r, FE_INV_D); }
+void h_scalar_mod_l(uint32_t *m) { scalar_mod_l(m); }
+void h_scalar_cneg_l(uint32_t *m, uint32_t neg) { scalar_cneg_l(m, neg); }
+void h_recode32_51(uint32_t *e, const uint32_t *m) { recode32<51>(e, m); }
+void h_recode32_13(uint32_t *e, const uint32_t *m) { recode32<13>(e, m); }
+// (+-b) + a where +-b starts as the extended point of b's table entry form
+// (negated when neg; b = NULL: the identity entry) and a is mixed-added
+void h_precomp_start_encoded(uint32_t *out, const uint32_t *a, const uint32_t *b, int neg) {
+  ge_p3 p, q, h, r;
+  ge_precomp ea, eb;
+  ge_p1p1 t;
+  ge_decode(p, a);
+  ge_p3_to_precomp(ea, p);
+  if (b) {
+    ge_decode(q, b);
+    ge_p3_to_precomp(eb, q);
+  } else {
+    ge_precomp_0(eb);
+  }
+  ge_precomp_cneg(eb, (uint32_t)neg);
+  ge_precomp_to_p3(h, eb);
+  ge_madd(t, h, ea);
+  ge_p1p1_to_p3(r, t);
+  ge_p3_encode(out, r);
+}
 // a + q (q negated when neg) by the mixed add over q's affine precomputed form
 // (the comb tables' entry); q = NULL adds the identity entry
 void h_madd_encoded(uint32_t *out, const uint32_t *a, const uint32_t *b, int neg) {

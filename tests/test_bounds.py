@@ -153,6 +153,8 @@ def test_invert_pow_constants(H):
     h = (ctypes.c_uint32 * 9)()
     H.h_inv2(h)
     assert val(h) == (P + 1) // 2 and max(h) <= N
+    H.h_inv_d(h)
+    assert val(h) * edref.D % P == 1 and max(h) <= N
 
 
 def enc_words(pt):
@@ -182,6 +184,16 @@ def test_group_formulas(H):
         H.h_madd_encoded(out, arr(enc_words(pa)), None, 0)
         assert words_val(out) == int.from_bytes(edref.encode(pa), "little")
         H.h_madd_encoded(out, arr(enc_words(pa)), arr(enc_words(pa)), 0)
+        assert words_val(out) == int.from_bytes(edref.encode(edref.add(pa, pa)), "little")
+        # the radix-32 comb's start: row 0's entry as an extended point (also
+        # negated, and the identity entry), then a mixed add
+        H.h_precomp_start_encoded(out, arr(enc_words(pa)), arr(enc_words(pb)), 0)
+        assert words_val(out) == int.from_bytes(edref.encode(edref.add(pb, pa)), "little")
+        H.h_precomp_start_encoded(out, arr(enc_words(pa)), arr(enc_words(pb)), 1)
+        assert words_val(out) == int.from_bytes(edref.encode(edref.add(edref.neg(pb), pa)), "little")
+        H.h_precomp_start_encoded(out, arr(enc_words(pa)), None, 1)
+        assert words_val(out) == int.from_bytes(edref.encode(pa), "little")
+        H.h_precomp_start_encoded(out, arr(enc_words(pa)), arr(enc_words(edref.neg(pa))), 1)
         assert words_val(out) == int.from_bytes(edref.encode(edref.add(pa, pa)), "little")
 
 
@@ -225,6 +237,48 @@ def test_public_double_and_add(H):
         dig = (ctypes.c_int8 * len(d))(*d)
         H.h_mul_public_encoded(out, arr(enc_words(p)), dig, len(d))
         assert words_val(out) == int.from_bytes(edref.encode(edref.mul(k, p)), "little"), trial
+
+
+def _digits(e, nd):
+    b = np.frombuffer(bytes(e), dtype=np.int8)[:nd]
+    return [int(x) for x in b]
+
+
+def test_scalar_mod_l_negate_and_radix32_digits(H):
+    """scalar.h: m mod l for any 256-bit m (both outcomes of the q delta
+    subtraction), l - m, and the signed radix-32 digits of the base-point comb
+    (value, digit ranges, the top digit's bound that keeps it in a 16-entry row)"""
+    L = edref.L
+    rng = random.Random(11)
+    ms = [0, 1, L - 1, L, L + 1, 2 * L - 1, 2 * L, 2**252 - 1, 2**252, 2**253 - 1, 2**255, 15 * 2**252,
+          2**256 - 1, 15 * L + 7] + [rng.getrandbits(256) for _ in range(3000)]
+    for m in ms:
+        w = arr(to_words(m))
+        H.h_scalar_mod_l(w)
+        assert words_val(w) == m % L, hex(m)
+    small = [0, 1, 2, L - 1, L - 2] + [rng.randrange(L) for _ in range(500)]
+    for m in small:
+        for neg in (0, 1):
+            w = arr(to_words(m))
+            H.h_scalar_cneg_l(w, neg)
+            assert words_val(w) == ((L - m) if neg else m)
+    e = (ctypes.c_uint32 * 13)()
+    tops = []
+    for m in small + [L, 2**253 - 1, 2**252 + 2**251] + [rng.getrandbits(253) for _ in range(500)]:
+        H.h_recode32_51(e, arr(to_words(m)))
+        d = _digits(e, 51)
+        assert sum(x * 32**v for v, x in enumerate(d)) == m
+        assert all(-16 <= x <= 15 for x in d[:50]) and 0 <= d[50] <= 8
+        tops.append(d[50])
+    assert max(tops) == 8
+    e4 = (ctypes.c_uint32 * 4)()
+    for m in [0, 1, 2**63, 2**64 - 1, 2**60 - 1] + [rng.getrandbits(64) for _ in range(500)]:
+        H.h_recode32_13(e4, arr(to_words(m)))
+        d = _digits(e4, 13)
+        assert sum(x * 32**v for v, x in enumerate(d)) == m
+        assert all(-16 <= x <= 15 for x in d[:12]) and 0 <= d[12] <= 16
+    H.h_recode32_13(e4, arr(to_words(2**64 - 1)))
+    assert _digits(e4, 13)[12] == 16
 
 
 def test_decode_semantics(H):

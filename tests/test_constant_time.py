@@ -33,7 +33,7 @@ CSRC = os.path.join(ROOT, "libftcrypto_amd", "csrc")
 sys.path.insert(0, ROOT)
 
 SECRET_KERNELS = {
-    "kernels.hip": ["k_fixed_base", "k_mul_u32_r8", "k_mul_u32_r16", "k_encrypt", "k_encrypt_split",
+    "kernels.hip": ["k_fixed_base32", "k_mul_u32_r8", "k_mul_u32_r16", "k_encrypt", "k_encrypt_split",
                     "k_encrypt_split_wide", "k_encrypt_wide", "k_encrypt_batch", "k_encrypt_batch_wide"],
     "varbase.hip": ["k_varbase", "k_varbase_r8"],
     "serve.hip": ["kb_encrypt", "kb_encrypt_split", "kb_encrypt_split_wide", "kb_encrypt_wide", "kb_mul_u32_r8",
@@ -101,7 +101,8 @@ def test_secret_kernels_have_no_data_dependent_branches(asm):
             if name not in names:
                 continue
             region = _field_region(lines)
-            assert sum("v_mad_u64_u32" in l for l in region) > 1000, (src, name)
+            # at least a loop body of seven field multiplies
+            assert sum("v_mad_u64_u32" in l for l in region) > 600, (src, name)
             bad = [l.strip() for l in region if EXEC.search(l)]
             assert not bad, (src, name, bad[:5])
             branches = {l.split()[0] for l in region if re.match(r"^\s+s_cbranch", l)}

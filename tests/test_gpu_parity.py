@@ -61,6 +61,56 @@ def test_fixed_base_top_digit_beyond_precondition():
     same(ft.GeArray.from_scalars(s), O.Array.from_scalars(s))
 
 
+B_ENC = bytes([0x58] + [0x66] * 31)
+
+
+def _times_base_variable(sc32):
+    """k_i B through the variable-base kernel (B decoded from its encoding)"""
+    p, bad = ft.GeArray.from_bytes_folded(np.frombuffer(B_ENC * len(sc32), dtype=np.uint8))
+    assert bad == -1                                   # every point decoded
+    p.mul(sc32)
+    return p.to_bytes_folded()
+
+
+def test_fixed_base_every_element_matches_variable_base():
+    """the radix-32 comb of B (scalars reduced mod l after the radix-16
+    top-digit rule) equals the variable-base product of B on every element
+    of a launch of 2^18 + 3001, and the oracle on the edge scalars, top
+    digits above 8 and a sample"""
+    n = (1 << 18) + 3001
+    s = scalars(n, 303)
+    rng = np.random.default_rng(8)
+    top = rng.integers(0, 256, size=(64, 32), dtype=np.uint8)
+    top[:, 31] |= 0x80
+    full = np.full((4, 32), 255, dtype=np.uint8)
+    full[1, 31] = 0x8f                                 # top nibble 8 with a carry in
+    full[2, 31] = 0x87
+    full[3, :31] = 0
+    special = np.concatenate([edge_scalars(), top, full])
+    pos = np.arange(len(special)) * 4099 % n
+    pos[-1] = n - 1
+    s[pos] = special
+    g = ft.GeArray.from_scalars(s).to_bytes_folded()
+    assert np.array_equal(g, _times_base_variable(s))
+    idx = np.unique(np.concatenate([pos, rng.choice(n, 1500, replace=False)]))
+    assert np.array_equal(g.reshape(-1, 32)[idx], O.Array.from_scalars(s[idx]).to_bytes().reshape(-1, 32))
+
+
+def test_fixed_base_small_scalars_every_element():
+    n = (1 << 18) + 777
+    rng = np.random.default_rng(9)
+    m = rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True)
+    edges = np.array([0, 1, 2, 15, 16, 17, 2**32, 2**60 - 1, 2**63, 2**64 - 1, 2**64 - 2], dtype=np.uint64)
+    pos = np.arange(len(edges)) * 7919 % n
+    m[pos] = edges
+    g = ft.GeArray.from_small_scalars(m).to_bytes_folded()
+    sc32 = np.zeros((n, 32), dtype=np.uint8)
+    sc32[:, :8] = m.view(np.uint8).reshape(n, 8)
+    assert np.array_equal(g, _times_base_variable(sc32))
+    idx = np.unique(np.concatenate([pos, rng.choice(n, 800, replace=False)]))
+    assert np.array_equal(g.reshape(-1, 32)[idx], O.Array.from_small_scalars(m[idx]).to_bytes().reshape(-1, 32))
+
+
 def test_small_scalars():
     rng = np.random.default_rng(6)
     m = np.concatenate([np.array([0, 1, 2, 5, 2**32, 2**63, 2**64 - 1], dtype=np.uint64),
