@@ -54,7 +54,7 @@ POINT_BYTES = 128
 # copy kernel of known size) and VALU busy, read from the committed counter
 # summary below: its table has one row per kernel.
 PMC_PROFILE = os.path.join(ROOT, "profiles", "scalarmult_n22_pmc_counters.md")
-PMC_KERNEL = {"fixed_base": "k_fixed_base<0>", "variable_base": {3: "k_varbase_r8", 4: "k_varbase"},
+PMC_KERNEL = {"fixed_base": "k_fixed_base32<0>", "variable_base": {3: "k_varbase_r8", 4: "k_varbase"},
               "mul_u32": {3: "k_mul_u32_r8", 4: "k_mul_u32_r16"}}
 
 
