@@ -460,6 +460,14 @@ def bench_cfg2(ft, hip, key, seed, args):
     res["segmented_sum"]["segments"] = nseg
     ms_s = T(lambda: job_scan(stream))
     res["scan"] = adds_entry(ms_s, 2 * (ns - 1), 2 * POINT_BYTES)
+    # both halves of the segmented sum as two jobs of one persistent launch
+    gjobs = (ft.AggJob * 2)(ft.agg_job(ft.AGG_SEGSUM, s1, g1, d_off, nseg, ns),
+                            ft.agg_job(ft.AGG_SEGSUM, s2, g2, d_off, nseg, ns))
+    gbscr = ft.GeArray.alloc(L.ftx_aggregate_batch_scratch_points(gjobs, 2))
+    ms_gb = T(lambda: ft.check(L.ftx_aggregate_batch_dev(gjobs, 2, gbscr.h, stream)))
+    res["segmented_sum_one_launch"] = adds_entry(ms_gb, 2 * (ns - nseg), seg_bytes / (2 * (ns - nseg)))
+    res["segmented_sum_one_launch"]["note"] = "ftx_aggregate_batch_dev with the two halves as two jobs"
+    del gbscr
 
     if args.cfg2_alone_only:
         ft.check(L.ftx_synchronize(None))

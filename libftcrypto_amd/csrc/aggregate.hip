@@ -85,7 +85,6 @@ __device__ __forceinline__ void line_to_pw(PW &a, PW &b, PW &c, PW &d, int co, c
 template <int CH, bool TR>
 __device__ __forceinline__ void chunk_sum_walk(ge_p3 &acc, PtView v, size_t lo, size_t hi, uint4 *tr, size_t wave0,
                                                int lane) {
-  ge_p3 x;
 #pragma unroll 1
   for (size_t gi = lo; gi < hi; gi += 4) {
     size_t g = gi;
@@ -105,18 +104,18 @@ __device__ __forceinline__ void chunk_sum_walk(ge_p3 &acc, PtView v, size_t lo, 
     } else {
       FT_LOAD4(q0, q1, q2, q3, v, g, hi);
     }
-    un_pw(x, q0);
-    if (g == lo) acc = x;
-    else ge_p3_add(acc, acc, x);
-    if (TR || g + 1 < hi) { un_pw(x, q1); ge_p3_add(acc, acc, x); }
-    if (TR || g + 2 < hi) { un_pw(x, q2); ge_p3_add(acc, acc, x); }
-    if (TR || g + 3 < hi) { un_pw(x, q3); ge_p3_add(acc, acc, x); }
+    if (g == lo) un_pw(acc, q0);
+    else add_pw(acc, q0);
+    if (TR || g + 1 < hi) add_pw(acc, q1);
+    if (TR || g + 2 < hi) add_pw(acc, q2);
+    if (TR || g + 3 < hi) add_pw(acc, q3);
   }
 }
 
 // sums[t] = sum of elements [2^CH t, 2^CH (t + 1)), read a cache line at a time
 template <int CH>
-__global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size_t n) {
+__global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_chunk_sums(PtView sums, PtView v, size_t n) {
   __shared__ uint4 trs[ATPB / 64][64 * 8];
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -134,12 +133,15 @@ __global__ void __launch_bounds__(ATPB) k_chunk_sums(PtView sums, PtView v, size
 }
 
 // one scan element: q holds the element on entry and its output on return
-__device__ __forceinline__ void scan_step(ge_p3 &run, PW &q, int exclusive) {
-  ge_p3 x;
-  un_pw(x, q);
-  if (exclusive) pk_pw(q, run);
-  ge_p3_add(run, run, x);
-  if (!exclusive) pk_pw(q, run);
+template <bool EX>
+__device__ __forceinline__ void scan_step(ge_p3 &run, PW &q) {
+  ge_cached x;
+  ge_p1p1 t;
+  un_pw_cached(x, q);
+  if (EX) pk_pw(q, run);
+  ge_add(t, run, x);
+  ge_p1p1_to_p3(run, t);
+  if (!EX) pk_pw(q, run);
 }
 
 // Lane-transposed store of one coordinate of a group: every lane of the wave
@@ -173,8 +175,8 @@ __device__ __forceinline__ void st_line_tr(uint4 *tr, uint8_t *dst, int lane, co
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int CH, bool TR>
-__device__ __forceinline__ void chunk_walk(PtView v, size_t lo, size_t hi, ge_p3 &run, int exclusive, uint4 *tr,
+template <int CH, bool TR, bool EX>
+__device__ __forceinline__ void chunk_walk(PtView v, size_t lo, size_t hi, ge_p3 &run, uint4 *tr,
                                            size_t wave0, int lane) {
 #pragma unroll 1
   for (size_t gi = lo; gi < hi; gi += 4) {
@@ -197,10 +199,10 @@ __device__ __forceinline__ void chunk_walk(PtView v, size_t lo, size_t hi, ge_p3
     } else {
       FT_LOAD4(q0, q1, q2, q3, v, g, hi);
     }
-    scan_step(run, q0, exclusive);
-    if (TR || g + 1 < hi) scan_step(run, q1, exclusive);
-    if (TR || g + 2 < hi) scan_step(run, q2, exclusive);
-    if (TR || g + 3 < hi) scan_step(run, q3, exclusive);
+    scan_step<EX>(run, q0);
+    if (TR || g + 1 < hi) scan_step<EX>(run, q1);
+    if (TR || g + 2 < hi) scan_step<EX>(run, q2);
+    if (TR || g + 3 < hi) scan_step<EX>(run, q3);
     if (TR) {
       uint8_t *out = const_cast<uint8_t *>(base);
       st_line_tr<CH>(tr, out, lane, q0.x, q1.x, q2.x, q3.x);
@@ -218,9 +220,9 @@ __device__ __forceinline__ void chunk_walk(PtView v, size_t lo, size_t hi, ge_p3
 // all whole writes them through LDS, a whole 128-byte line per eight lanes
 // and store; the array's last wave stores each coordinate's line by
 // consecutive stores of its lane.
-template <int CH>
+template <int CH, bool EX>
 __global__ void __launch_bounds__(ATPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
+k_chunk_scan(PtView v, size_t n, PtView pre) {
   __shared__ uint4 trs[ATPB / 64][64 * 8];
   const size_t t = blockIdx.x * (size_t)ATPB + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -232,9 +234,9 @@ k_chunk_scan(PtView v, size_t n, PtView pre, int exclusive) {
   if (pre.w) load_p3(run, pre, t);
   else ge_p3_0(run);
   if (((wave0 + 64) << CH) <= n)   // wave-uniform: all 64 chunks whole
-    chunk_walk<CH, true>(v, lo, hi, run, exclusive, trs[threadIdx.x >> 6], wave0, lane);
+    chunk_walk<CH, true, EX>(v, lo, hi, run, trs[threadIdx.x >> 6], wave0, lane);
   else
-    chunk_walk<CH, false>(v, lo, hi, run, exclusive, nullptr, wave0, lane);
+    chunk_walk<CH, false, EX>(v, lo, hi, run, nullptr, wave0, lane);
 }
 
 static hipError_t launch_chunk_sums(PtView sums, PtView v, size_t n, int ch, hipStream_t s) {
@@ -255,13 +257,17 @@ static hipError_t launch_chunk_scan(PtView v, size_t n, PtView pre, bool exclusi
   const size_t T = tiles(n, ch);
   if (!T) return hipSuccess;
   const dim3 grid((unsigned)((T + ATPB - 1) / ATPB));
+#define FT_CSCAN(C)                                                                                     \
+  if (exclusive) hipLaunchKernelGGL((k_chunk_scan<C, true>), grid, dim3(ATPB), 0, s, v, n, pre);        \
+  else hipLaunchKernelGGL((k_chunk_scan<C, false>), grid, dim3(ATPB), 0, s, v, n, pre)
   switch (ch) {
-    case 4: hipLaunchKernelGGL(k_chunk_scan<4>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
-    case 5: hipLaunchKernelGGL(k_chunk_scan<5>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
-    case 6: hipLaunchKernelGGL(k_chunk_scan<6>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
-    case 7: hipLaunchKernelGGL(k_chunk_scan<7>, grid, dim3(ATPB), 0, s, v, n, pre, (int)exclusive); break;
+    case 4: FT_CSCAN(4); break;
+    case 5: FT_CSCAN(5); break;
+    case 6: FT_CSCAN(6); break;
+    case 7: FT_CSCAN(7); break;
     default: return hipErrorInvalidValue;
   }
+#undef FT_CSCAN
   return hipGetLastError();
 }
 
@@ -320,16 +326,28 @@ __global__ void __launch_bounds__(ATPB) k_chunk_scan_q(PtView v, size_t n, PtVie
   if (lo >= n) return;
   const size_t hi = (lo + (1u << CH) < n) ? lo + (1u << CH) : n;
   fe run, x;
-  if (pre.w) load_fe(run, coord(pre, c, t));
-  else quad_identity(run, c);
+  if (pre.w) {
+    load_fe(run, coord(pre, c, t));
+    quad_from_mem(run, c);
+  } else {
+    quad_identity(run, c);
+  }
   load_fe(x, coord(v, c, lo));
 #pragma unroll 1
   for (size_t g = lo; g < hi; ++g) {
-    fe nx;
+    fe nx, m;
     load_fe(nx, coord(v, c, g + 1 < hi ? g + 1 : g));
-    if (exclusive) store_fe(coord_w(v, c, g), run);
-    quad_add(run, x, c);
-    if (!exclusive) store_fe(coord_w(v, c, g), run);
+    if (exclusive) {
+      m = run;
+      quad_to_mem(m, c);
+      store_fe(coord_w(v, c, g), m);
+    }
+    quad_add<true>(run, x, c);
+    if (!exclusive) {
+      m = run;
+      quad_to_mem(m, c);
+      store_fe(coord_w(v, c, g), m);
+    }
     x = nx;
   }
 }
@@ -360,8 +378,12 @@ __global__ void __launch_bounds__(QBTPB) k_block_scan_q(PtView v, size_t m, PtVi
   const int wq = q & 15, w = q >> 4;
   const size_t i = blockIdx.x * (size_t)(QBTPB / 4) + q;
   fe x;
-  if (i < m) load_fe(x, coord(v, c, i));
-  else quad_identity(x, c);
+  if (i < m) {
+    load_fe(x, coord(v, c, i));
+    quad_from_mem(x, c);
+  } else {
+    quad_identity(x, c);
+  }
 #pragma unroll 1
   for (int d = 1; d < 16; d <<= 1) {
     fe y, z;
@@ -401,13 +423,18 @@ __global__ void __launch_bounds__(QBTPB) k_block_scan_q(PtView v, size_t m, PtVi
   } else {
     quad_identity(pre, c);
   }
-  if (tot.w && q == QBTPB / 4 - 1) store_fe(coord_w(tot, c, blockIdx.x), x);
+  if (tot.w && q == QBTPB / 4 - 1) {
+    fe t = x;
+    quad_to_mem(t, c);
+    store_fe(coord_w(tot, c, blockIdx.x), t);
+  }
   if (exclusive) {
     fe y;
     fe_shfl_up(y, x, 4);
 #pragma unroll
     for (int l = 0; l < 9; ++l) x.v[l] = wq == 0 ? pre.v[l] : y.v[l];
   }
+  quad_to_mem(x, c);
   if (i < m) store_fe(coord_w(v, c, i), x);
 }
 
@@ -420,7 +447,9 @@ __global__ void __launch_bounds__(QBTPB) k_block_add_q(PtView v, size_t m, PtVie
   fe x, p;
   load_fe(x, coord(v, c, i));
   load_fe(p, coord(pre, c, b));
-  quad_add(x, p, c);
+  quad_from_mem(x, c);
+  quad_add<true>(x, p, c);
+  quad_to_mem(x, c);
   store_fe(coord_w(v, c, i), x);
 }
 
@@ -556,19 +585,16 @@ __device__ __forceinline__ void shfl_xor_p3(ge_p3 &r, const ge_p3 &a, int d) {
   }
 }
 
-// sum of points [lo, hi) of v (lo < hi), read a line per coordinate at a time
+// acc += points [lo, hi) of v, read a line per coordinate at a time
 __device__ __forceinline__ void chunk_sum(ge_p3 &acc, const PtView v, size_t lo, size_t hi) {
-  ge_p3 x;
 #pragma unroll 1
   for (size_t g = lo; g < hi; g += 4) {
     PW q0, q1, q2, q3;
     FT_LOAD4(q0, q1, q2, q3, v, g, hi);
-    un_pw(x, q0);
-    if (g == lo) acc = x;
-    else ge_p3_add(acc, acc, x);
-    if (g + 1 < hi) { un_pw(x, q1); ge_p3_add(acc, acc, x); }
-    if (g + 2 < hi) { un_pw(x, q2); ge_p3_add(acc, acc, x); }
-    if (g + 3 < hi) { un_pw(x, q3); ge_p3_add(acc, acc, x); }
+    add_pw(acc, q0);
+    if (g + 1 < hi) add_pw(acc, q1);
+    if (g + 2 < hi) add_pw(acc, q2);
+    if (g + 3 < hi) add_pw(acc, q3);
   }
 }
 
@@ -577,8 +603,9 @@ __global__ void k_lb_init(unsigned *flag, size_t T) {
   if (i <= T) flag[i] = 0u;                      // flags, then the tile counter
 }
 
+template <bool EX>
 __global__ void __launch_bounds__(LB_TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_scan_lookback(PtView v, size_t n, int exclusive, PtView agg, PtView inc, unsigned *flag, unsigned *ctr) {
+k_scan_lookback(PtView v, size_t n, PtView agg, PtView inc, unsigned *flag, unsigned *ctr) {
   // each lane's exclusive prefix within its wave, then the wave totals, the
   // tile's aggregate and its exclusive prefix (packed points), so that no
   // point stays in registers across the look-back
@@ -612,13 +639,10 @@ k_scan_lookback(PtView v, size_t n, int exclusive, PtView agg, PtView inc, unsig
   __syncthreads();
   if (w == NW - 1) {
     // the tile's aggregate, published by one lane: point, then flag 1
-    ge_p3 A, t;
+    ge_p3 A;
     un_pw(A, s_pt[0]);
 #pragma unroll 1
-    for (int q = 1; q < NW; ++q) {
-      un_pw(t, s_pt[q]);
-      ge_p3_add(A, A, t);
-    }
+    for (int q = 1; q < NW; ++q) add_pw(A, s_pt[q]);
     if (lane == 63) {
       pk_pw(s_pt[NW], A);
       store_p3(agg, tile, A);
@@ -679,10 +703,13 @@ k_scan_lookback(PtView v, size_t n, int exclusive, PtView agg, PtView inc, unsig
   __syncthreads();
   if (w == 0) {
     // the tile's inclusive prefix E + A, published by one lane: point, then flag 2
-    ge_p3 A, P;
-    un_pw(A, s_pt[NW]);
-    if (tile > 0) ge_p3_add(P, E, A);
-    else P = A;
+    ge_p3 P;
+    if (tile > 0) {
+      P = E;
+      add_pw(P, s_pt[NW]);
+    } else {
+      un_pw(P, s_pt[NW]);
+    }
     if (lane == 0) {
       pk_pw(s_pt[NW + 1], E);
       store_p3(inc, tile, P);
@@ -692,17 +719,11 @@ k_scan_lookback(PtView v, size_t n, int exclusive, PtView agg, PtView inc, unsig
   __syncthreads();
   if (lo >= n) return;
   // walk 2 from E + (the earlier waves' totals) + (the wave's lane sums before this lane)
-  ge_p3 run, t;
+  ge_p3 run;
   un_pw(run, s_pt[NW + 1]);
 #pragma unroll 1
-  for (int q = 0; q < w; ++q) {
-    un_pw(t, s_pt[q]);
-    ge_p3_add(run, run, t);
-  }
-  if (lane > 0) {
-    un_pw(t, s_lane[threadIdx.x]);
-    ge_p3_add(run, run, t);
-  }
+  for (int q = 0; q < w; ++q) add_pw(run, s_pt[q]);
+  if (lane > 0) add_pw(run, s_lane[threadIdx.x]);
 #pragma unroll 1
   for (size_t gi = lo; gi < hi; gi += 4) {
     // an opaque copy of the element index: the plane addresses are then
@@ -713,10 +734,10 @@ k_scan_lookback(PtView v, size_t n, int exclusive, PtView agg, PtView inc, unsig
     asm volatile("" : "+v"(g));
     PW q0, q1, q2, q3;
     FT_LOAD4(q0, q1, q2, q3, v, g, hi);
-    scan_step(run, q0, exclusive);
-    if (g + 1 < hi) scan_step(run, q1, exclusive);
-    if (g + 2 < hi) scan_step(run, q2, exclusive);
-    if (g + 3 < hi) scan_step(run, q3, exclusive);
+    scan_step<EX>(run, q0);
+    if (g + 1 < hi) scan_step<EX>(run, q1);
+    if (g + 2 < hi) scan_step<EX>(run, q2);
+    if (g + 3 < hi) scan_step<EX>(run, q3);
     st_pw4(v, g, (int)(hi - g < 4 ? hi - g : 4), q0, q1, q2, q3);
   }
 }
@@ -729,8 +750,10 @@ hipError_t scan_lookback_launch(PtView v, size_t n, bool exclusive, PtView scrat
   const PtView agg{scratch.w, scratch.cap}, inc{scratch.w + T * 8, scratch.cap};
   unsigned *flag = reinterpret_cast<unsigned *>(scratch.w + 2 * T * 8);
   hipLaunchKernelGGL(k_lb_init, dim3((unsigned)((T + 1 + 255) / 256)), dim3(256), 0, s, flag, T);
-  hipLaunchKernelGGL(k_scan_lookback, dim3((unsigned)T), dim3(LB_TPB), 0, s, v, n, (int)exclusive, agg, inc, flag,
-                     flag + T);
+  if (exclusive)
+    hipLaunchKernelGGL(k_scan_lookback<true>, dim3((unsigned)T), dim3(LB_TPB), 0, s, v, n, agg, inc, flag, flag + T);
+  else
+    hipLaunchKernelGGL(k_scan_lookback<false>, dim3((unsigned)T), dim3(LB_TPB), 0, s, v, n, agg, inc, flag, flag + T);
   return hipGetLastError();
 }
 
@@ -875,7 +898,7 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
       bnd[k] = j <= nseg ? e : INT64_MAX;
     }
   };
-  ge_p3 acc, x;
+  ge_p3 acc;
   ge_p3_0(acc);
   bool touched = false;
   // advance the segment cursor to element i; true when i lies in segment s
@@ -906,6 +929,7 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
     if (cross) {
       if (!TR) flush(touched);
       touched = false;
+      ge_p3_0(acc);
       ++s;
       lo = bnd[0];
 #pragma unroll
@@ -918,9 +942,11 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
     }
     return s < nseg && lo <= i;
   };
-  auto accum = [&]() {
-    if (touched) ge_p3_add(acc, acc, x);
-    else acc = x;
+  // every element is added to the running sum, which a segment starts at
+  // the identity: the same code on every lane whether or not its segment
+  // just began
+  auto accum = [&](const PW &q) {
+    add_pw(acc, q);
     touched = true;
   };
   if (MODE != 0) {
@@ -958,10 +984,10 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
         ld_pw(q3, src, j3);
       }
       pin_pw(q0); pin_pw(q1); pin_pw(q2); pin_pw(q3);
-      if (take(g)) { un_pw(x, q0); accum(); }
-      if (g + 1 < cend && take(g + 1)) { un_pw(x, q1); accum(); }
-      if (g + 2 < cend && take(g + 2)) { un_pw(x, q2); accum(); }
-      if (g + 3 < cend && take(g + 3)) { un_pw(x, q3); accum(); }
+      if (take(g)) accum(q0);
+      if (g + 1 < cend && take(g + 1)) accum(q1);
+      if (g + 2 < cend && take(g + 2)) accum(q2);
+      if (g + 3 < cend && take(g + 3)) accum(q3);
     }
   } else {
 #pragma unroll 1
@@ -986,10 +1012,10 @@ __device__ __forceinline__ void seg_walk(PtView out, PtView rec, int64_t *keys, 
       } else {
         FT_LOAD4(q0, q1, q2, q3, src, (size_t)g, (size_t)cend);
       }
-      if (take(g)) { un_pw(x, q0); accum(); }
-      if (g + 1 < cend && take(g + 1)) { un_pw(x, q1); accum(); }
-      if (g + 2 < cend && take(g + 2)) { un_pw(x, q2); accum(); }
-      if (g + 3 < cend && take(g + 3)) { un_pw(x, q3); accum(); }
+      if (take(g)) accum(q0);
+      if (g + 1 < cend && take(g + 1)) accum(q1);
+      if (g + 2 < cend && take(g + 2)) accum(q2);
+      if (g + 3 < cend && take(g + 3)) accum(q3);
     }
   }
   if (TR) {
@@ -1052,25 +1078,29 @@ __device__ __forceinline__ void seg_up_quad(PtView out, PtView rin, const int64_
       if (k < 0) continue;
       load_fe(x, coord(rin, c, 2 * u + slot));
       if (k == key) {
-        quad_add(acc, x, c);
+        quad_add<true>(acc, x, c);
       } else {                          // a segment's first record in the block
         acc = x;
+        quad_from_mem(acc, c);
         key = k;
         before = slot == 0 && u == u0;
         last = (off[k + 1] - 1) >> ch1;
       }
       if (last < uend) {                // the segment ends in unit u
+        fe m = acc;
+        quad_to_mem(m, c);
         if (before) {
-          store_fe(coord_w(rout, c, 2 * b), acc);
+          store_fe(coord_w(rout, c, 2 * b), m);
           hk = key;
         } else {
-          store_fe(coord_w(out, c, (size_t)key), acc);
+          store_fe(coord_w(out, c, (size_t)key), m);
         }
         key = -1;
       }
     }
   }
   if (key >= 0) {                       // the open segment leaves the block
+    quad_to_mem(acc, c);
     store_fe(coord_w(rout, c, 2 * b + (before ? 0 : 1)), acc);
     if (before) hk = key;
     else tk = key;
@@ -1280,24 +1310,27 @@ static_assert(sizeof(BBatch) + 48 <= 3072, "job table must stay well inside the 
 // reduce tile: lane walks elements base, base + 64, ... (2^rch of them, below
 // n) with the next point's words in flight during each add
 __device__ __forceinline__ void reduce_tile(PtView part, size_t slot, PtView v, size_t n, size_t base, int rch) {
-  ge_p3 acc, q;
+  ge_p3 acc;
+  ge_cached q;
   PW nw;
   ge_p3_0(acc);
   const size_t end0 = base + ((size_t)64 << rch);
   const size_t end = end0 < n ? end0 : n;
   size_t i = base;
   if (i < n) { load_p3(acc, v, i); i += 64; }
-  if (i < n && i < end) load_p3(q, v, i);
+  if (i < n && i < end) load_cached(q, v, i);
 #pragma unroll 1
   for (; i < n && i < end; i += 64) {
     const size_t nx = (i + 64 < n && i + 64 < end) ? i + 64 : i;
     ld_pw(nw, v, nx);
     asm volatile("" ::: "memory");
-    pin_p3(q);
-    ge_p3_add(acc, acc, q);
+    pin_cached(q);
+    ge_p1p1 t;
+    ge_add(t, acc, q);
+    ge_p1p1_to_p3(acc, t);
     pin_p3(acc);
     pin_pw(nw);
-    un_pw(q, nw);
+    un_pw_cached(q, nw);
   }
   store_p3(part, slot, acc);
 }

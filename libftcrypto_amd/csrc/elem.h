@@ -10,11 +10,10 @@ namespace ft {
 // a[i] = a[i] +/- b[j]
 template <bool SUB>
 __device__ __forceinline__ void add_one(PtView a, size_t i, PtView b, size_t j) {
-  ge_p3 p, q;
-  load_p3(p, a, i);
-  load_p3(q, b, j);
+  ge_p3 p;
   ge_cached c;
-  ge_p3_to_cached(c, q);
+  load_p3(p, a, i);
+  load_cached(c, b, j);
   if (SUB) ge_cached_cneg(c, 1u);
   ge_p1p1 t;
   ge_add(t, p, c);
@@ -94,9 +93,7 @@ __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, cons
     ge_p3_0(a1);
     comb_run<8>(a1, tb, tb, o1, e1, 32, 32);
     if (ACC) {
-      ge_p3 o;
-      load_p3(o, c1, i);
-      ge_p3_add(a1, o, a1);
+      add_stored(a1, c1, i);
     }
     store_p3(c1, i, a1);
   }
@@ -128,9 +125,7 @@ __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, cons
   ge_p3_0(a2);
   comb_run<11>(a2, th, tb, odd, even, 32 + mo, 32 + me);
   if (ACC) {
-    ge_p3 o;
-    load_p3(o, c2, i);
-    ge_p3_add(a2, o, a2);
+    add_stored(a2, c2, i);
   }
   store_p3(c2, i, a2);
 }
@@ -170,9 +165,7 @@ __device__ __forceinline__ void encrypt_half(PtView c1, PtView c2, size_t i, uin
   comb_run<11>(a, half ? th : tb, tb, odd, even, 32 + mo, 32 + me);
   const PtView dst = half ? c2 : c1;
   if (ACC) {
-    ge_p3 o;
-    load_p3(o, dst, i);
-    ge_p3_add(a, o, a);
+    add_stored(a, dst, i);
   }
   store_p3(dst, i, a);
 }
@@ -230,9 +223,7 @@ __device__ __forceinline__ void encrypt_quarter(PtView c1, PtView c2, size_t i, 
   if (part) return;
   const PtView dst = half ? c2 : c1;
   if (ACC) {
-    ge_p3 o;
-    load_p3(o, dst, i);
-    ge_p3_add(a, o, a);
+    add_stored(a, dst, i);
   }
   store_p3(dst, i, a);
 }

@@ -260,7 +260,7 @@ FT_HD uint32_t fe_iszero(const fe &a) {
 // Constants in radix-2^29 limbs (class N), derived from their definitions and
 // checked against them by tests/test_bounds.py:
 //   d = -121665/121666 mod p, 2d, sqrt(-1) = 2^((p-1)/4) mod p,
-//   1/2 = (p + 1)/2 = 2^254 - 9, 1/d = -121666/121665 mod p.
+//   1/2 = (p + 1)/2 = 2^254 - 9, 1/d = -121666/121665 mod p, 1/(2d).
 constexpr fe FE_D = {{0x135978a3u, 0x0f5a6e50u, 0x10762addu, 0x00149a82u, 0x1e898007u,
                       0x003cbbbcu, 0x19ce331du, 0x1dc56dffu, 0x0052036cu}};
 constexpr fe FE_D2 = {{0x06b2f159u, 0x1eb4dca1u, 0x00ec55bau, 0x00293505u, 0x1d13000eu,
@@ -271,5 +271,7 @@ constexpr fe FE_INV2 = {{0x1ffffff7u, 0x1fffffffu, 0x1fffffffu, 0x1fffffffu, 0x1
                          0x1fffffffu, 0x1fffffffu, 0x1fffffffu, 0x003fffffu}};
 constexpr fe FE_INV_D = {{0x0dc9f843u, 0x0f0793b6u, 0x1e550b89u, 0x1bad3084u, 0x1cf660b5u,
                           0x108a66dcu, 0x190cac58u, 0x1a429ab9u, 0x0040907eu}};
+constexpr fe FE_INV_D2 = {{0x06e4fc18u, 0x1783c9dbu, 0x0f2a85c4u, 0x1dd69842u, 0x0e7b305au,
+                           0x0845336eu, 0x1c86562cu, 0x0d214d5cu, 0x0060483fu}};
 
 }  // namespace ft

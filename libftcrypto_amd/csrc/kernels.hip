@@ -449,22 +449,25 @@ __global__ void __launch_bounds__(TPB) k_reduce_blocks(PtView part, PtView v, si
   const size_t b0 = blockIdx.x * (size_t)TPB;
   size_t i = b0 + threadIdx.x;
   ge_p3 acc, q;
+  ge_cached qc;
   PW nw;
   ge_p3_0(acc);
   if (i < n) { load_p3(acc, v, i); i += lanes; }
-  if (i < n) load_p3(q, v, i);
+  if (i < n) load_cached(qc, v, i);
 #pragma unroll 1
   for (; i < n; i += lanes) {
     // the next point's words are loaded before the add and unpacked after
-    // it: the barrier keeps the loads first, pinning q keeps the add after
+    // it: the barrier keeps the loads first, pinning qc keeps the add after
     // them, pinning acc keeps the wait for the words after the add
     ld_pw(nw, v, i + lanes < n ? i + lanes : i);
     asm volatile("" ::: "memory");
-    pin_p3(q);
-    ge_p3_add(acc, acc, q);
+    pin_cached(qc);
+    ge_p1p1 t;
+    ge_add(t, acc, qc);
+    ge_p1p1_to_p3(acc, t);
     pin_p3(acc);
     pin_pw(nw);
-    un_pw(q, nw);
+    un_pw_cached(qc, nw);
   }
   // lanes of this block holding a point (all when the grid strides)
   const size_t act = n >= lanes ? TPB : (n > b0 ? (n - b0 < TPB ? n - b0 : TPB) : 0);
@@ -506,12 +509,16 @@ __global__ void __launch_bounds__(QTPB) k_reduce_quads(PtView part, PtView v, si
   size_t i = q0 + gq;
   fe acc;
   quad_identity(acc, c);
-  if (i < n) { load_fe(acc, v.w + (c * v.cap + i) * 8); i += nq; }
+  if (i < n) {
+    load_fe(acc, v.w + (c * v.cap + i) * 8);
+    quad_from_mem(acc, c);
+    i += nq;
+  }
 #pragma unroll 1
   for (; i < n; i += nq) {
     fe o;
     load_fe(o, v.w + (c * v.cap + i) * 8);
-    quad_add(acc, o, c);
+    quad_add<true>(acc, o, c);
   }
   const size_t act = n >= nq ? QPB : (n > q0 ? (n - q0 < QPB ? n - q0 : QPB) : 0);
 #pragma unroll 1
@@ -544,7 +551,10 @@ __global__ void __launch_bounds__(QTPB) k_reduce_quads(PtView part, PtView v, si
       quad_add(acc, o, c);
     }
   }
-  if (gq == 0) store_fe(part.w + (c * part.cap + blockIdx.x) * 8, acc);
+  if (gq == 0) {
+    quad_to_mem(acc, c);
+    store_fe(part.w + (c * part.cap + blockIdx.x) * 8, acc);
+  }
 }
 
 // ------------------------------------------------------ host launchers

@@ -52,18 +52,41 @@ __device__ __forceinline__ void store_fe(uint32_t *p, const fe &a) {
   st_words(p, w);
 }
 
+// Resident point format: planes X | Y | Z | 2d T.  The fourth plane holds T
+// premultiplied by 2d, the form in which the unified add takes its second
+// operand (ge_cached::T2d): the aggregation walks then add a loaded point
+// without a product of its own (add_pw), for one product per point
+// written.  load_p3 / un_pw give the extended point (T = plane 3 / 2d),
+// store_p3 / pk_pw write one.
 __device__ __forceinline__ void load_p3(ge_p3 &r, const PtView v, size_t i) {
+  fe t2d;
   load_fe(r.X, v.w + (0 * v.cap + i) * 8);
   load_fe(r.Y, v.w + (1 * v.cap + i) * 8);
   load_fe(r.Z, v.w + (2 * v.cap + i) * 8);
-  load_fe(r.T, v.w + (3 * v.cap + i) * 8);
+  load_fe(t2d, v.w + (3 * v.cap + i) * 8);
+  fe_mul(r.T, t2d, FE_INV_D2);
 }
 
 __device__ __forceinline__ void store_p3(const PtView v, size_t i, const ge_p3 &r) {
+  fe t2d;
+  fe_mul(t2d, r.T, FE_D2);
   store_fe(v.w + (0 * v.cap + i) * 8, r.X);
   store_fe(v.w + (1 * v.cap + i) * 8, r.Y);
   store_fe(v.w + (2 * v.cap + i) * 8, r.Z);
-  store_fe(v.w + (3 * v.cap + i) * 8, r.T);
+  store_fe(v.w + (3 * v.cap + i) * 8, t2d);
+}
+
+// the cached form (Y + X, Y - X, 2 Z, 2d T) of the point at v[i], no product
+__device__ __forceinline__ void load_cached(ge_cached &r, const PtView v, size_t i) {
+  fe x, y, z;
+  load_fe(x, v.w + (0 * v.cap + i) * 8);
+  load_fe(y, v.w + (1 * v.cap + i) * 8);
+  load_fe(z, v.w + (2 * v.cap + i) * 8);
+  load_fe(r.T2d, v.w + (3 * v.cap + i) * 8);
+  fe_add(r.YpX, y, x);
+  fe_sub(r.YmX, y, x);
+  fe_carry(r.YmX);
+  fe_add(r.Z2, z, z);
 }
 
 // identity (0, 1, 1, 0) written without arithmetic
@@ -103,10 +126,43 @@ __device__ __forceinline__ void ld_pw(PW &p, const PtView v, size_t i) {
 }
 
 __device__ __forceinline__ void un_pw(ge_p3 &r, const PW &p) {
+  fe t2d;
   fe_from_words(r.X, p.x);
   fe_from_words(r.Y, p.y);
   fe_from_words(r.Z, p.z);
-  fe_from_words(r.T, p.t);
+  fe_from_words(t2d, p.t);
+  fe_mul(r.T, t2d, FE_INV_D2);
+}
+
+// the cached form of a point from its resident words (no product)
+__device__ __forceinline__ void un_pw_cached(ge_cached &r, const PW &p) {
+  fe x, y, z;
+  fe_from_words(x, p.x);
+  fe_from_words(y, p.y);
+  fe_from_words(z, p.z);
+  fe_from_words(r.T2d, p.t);
+  fe_add(r.YpX, y, x);
+  fe_sub(r.YmX, y, x);
+  fe_carry(r.YmX);
+  fe_add(r.Z2, z, z);
+}
+
+// acc += v[i] (8 products: the stored point's 2d T enters the add as is)
+__device__ __forceinline__ void add_stored(ge_p3 &acc, const PtView v, size_t i) {
+  ge_cached c;
+  ge_p1p1 t;
+  load_cached(c, v, i);
+  ge_add(t, acc, c);
+  ge_p1p1_to_p3(acc, t);
+}
+
+// acc += the point whose resident words are q: 8 products
+__device__ __forceinline__ void add_pw(ge_p3 &acc, const PW &q) {
+  ge_cached c;
+  ge_p1p1 t;
+  un_pw_cached(c, q);
+  ge_add(t, acc, c);
+  ge_p1p1_to_p3(acc, t);
 }
 
 __device__ __forceinline__ void st_pw(const PtView v, size_t i, const PW &p) {
@@ -142,10 +198,12 @@ __device__ __forceinline__ void st_pw4(const PtView v, size_t g, int cnt, const 
 }
 
 __device__ __forceinline__ void pk_pw(PW &p, const ge_p3 &r) {
+  fe t2d;
+  fe_mul(t2d, r.T, FE_D2);
   fe_store_words(p.x, r.X);
   fe_store_words(p.y, r.Y);
   fe_store_words(p.z, r.Z);
-  fe_store_words(p.t, r.T);
+  fe_store_words(p.t, t2d);
 }
 
 __device__ __forceinline__ void pin_pw(PW &p) {
@@ -169,6 +227,13 @@ __device__ __forceinline__ void pin_p3(ge_p3 &p) {
   pin_fe(p.Y);
   pin_fe(p.Z);
   pin_fe(p.T);
+}
+
+__device__ __forceinline__ void pin_cached(ge_cached &p) {
+  pin_fe(p.YpX);
+  pin_fe(p.YmX);
+  pin_fe(p.Z2);
+  pin_fe(p.T2d);
 }
 
 // Point-contiguous copy kept in the tail slots of a scratch array's planes

@@ -148,16 +148,22 @@ __global__ void __launch_bounds__(MTPB) k_from_affine(PtView v, const uint32_t *
   store_p3(v, i, p);
 }
 
-// raw 160-byte records: the four resident 32-byte words X|Y|Z|T and 32 zero bytes
+// raw 160-byte records: X|Y|Z as their resident 32-byte words, T (the
+// resident fourth plane divided by 2d) as 32 bytes, then 32 zero bytes
 __global__ void __launch_bounds__(MTPB) k_to_raw(uint32_t *out, PtView v, size_t n) {
   const size_t i = blockIdx.x * (size_t)MTPB + threadIdx.x;
   if (i >= n) return;
   uint32_t w[8];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < 3; ++c) {
     ld_words(w, v.w + (c * v.cap + i) * 8);
     st_words(out + i * 40 + c * 8, w);
   }
+  fe t2d, t;
+  load_fe(t2d, v.w + (3 * v.cap + i) * 8);
+  fe_mul(t, t2d, FE_INV_D2);
+  fe_store_words(w, t);
+  st_words(out + i * 40 + 24, w);
 #pragma unroll
   for (int q = 0; q < 8; ++q) w[q] = 0;
   st_words(out + i * 40 + 32, w);
@@ -168,10 +174,15 @@ __global__ void __launch_bounds__(MTPB) k_from_raw(PtView v, const uint32_t *in,
   if (i >= n) return;
   uint32_t w[8];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < 3; ++c) {
     ld_words(w, in + i * 40 + c * 8);
     st_words(v.w + (c * v.cap + i) * 8, w);
   }
+  ld_words(w, in + i * 40 + 24);
+  fe t, t2d;
+  fe_from_words(t, w);
+  fe_mul(t2d, t, FE_D2);
+  store_fe(v.w + (3 * v.cap + i) * 8, t2d);
 }
 
 // ------------------------------------------------------- set membership

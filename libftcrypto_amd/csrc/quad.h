@@ -6,7 +6,9 @@
 // multiplies in sequence instead of nine, one per lane in each step, with
 // the operands moved inside the quad by DPP quad permutes:
 //   1: lane 0 (Y1 - X1)(Y2 - X2) = A, 1 (Y1 + X1)(Y2 + X2) = B, 2 T1 T2, 3 Z1 Z2
-//   2: lane 2 C = 2d T1 T2, lane 3 D = 2 Z1 Z2 (lanes 0, 1 multiply by 1);
+//   2: lane 2 C = 2d T1 T2, lane 3 D = 2 Z1 Z2 (lanes 0, 1 multiply by 1;
+//      with the second point read from memory, whose fourth plane is 2d T,
+//      step 1 already gives C and step 2 is a doubling on lane 3);
 //      then E = B - A, H = B + A, G = D + C, F = D - C, carried to class N
 //   3: lane 0 X3 = E F, 1 Y3 = G H, 2 Z3 = G F, 3 T3 = E H
 // the same formulas as ge_add of a cached point followed by the p1p1 to p3
@@ -40,7 +42,24 @@ __device__ __forceinline__ void quad_factor(fe &u, const fe &m, const fe &n, int
   for (int l = 0; l < 9; ++l) u.v[l] = n.v[l] + (t.v[l] & keep);
 }
 
+// Points read from memory hold 2d T in their fourth plane (the resident
+// form, layout.h); a quad's running point holds T.  quad_from_mem and
+// quad_to_mem convert coordinate c between the two (lane 3 multiplies by
+// 1/(2d) or 2d, the others by 1: one multiply step of the quad).
+__device__ __forceinline__ void quad_scale_t(fe &x, int c, const fe &k3) {
+  fe k;
+#pragma unroll
+  for (int l = 0; l < 9; ++l) k.v[l] = c == 3 ? k3.v[l] : 0u;
+  k.v[0] = c == 3 ? k3.v[0] : 1u;
+  fe_mul(x, x, k);
+}
+__device__ __forceinline__ void quad_from_mem(fe &x, int c) { quad_scale_t(x, c, FE_INV_D2); }
+__device__ __forceinline__ void quad_to_mem(fe &x, int c) { quad_scale_t(x, c, FE_D2); }
+
 // acc += o: the quad's point plus the point whose coordinate c is o
+// (MEM: o in the resident form, whose 2d T makes lane 2's product C at once,
+// so the second step only doubles lane 3's Z1 Z2: two multiply steps)
+template <bool MEM = false>
 __device__ __forceinline__ void quad_add(fe &acc, const fe &o, int c) {
   fe n1, n2, u1, u2, p, r;
   fe_qperm<QP_SWAP>(n1, acc);
@@ -48,13 +67,20 @@ __device__ __forceinline__ void quad_add(fe &acc, const fe &o, int c) {
   quad_factor(u1, acc, n1, c);            // A2 (c = 0), A1 (c = 1), N
   quad_factor(u2, o, n2, c);
   fe_carry(u2);                           // N, so every product is exact
-  fe_mul(p, u1, u2);                      // A, B, T1 T2, Z1 Z2
-  // lanes 2 and 3 scale by 2d and 2, lanes 0 and 1 by 1
-  fe k;
+  fe_mul(p, u1, u2);                      // A, B, T1 T2 (C for MEM), Z1 Z2
+  if (MEM) {
+    // lane 3 doubles (<= 2N), the others keep their product
+    const uint32_t sh = c == 3 ? 1u : 0u;
 #pragma unroll
-  for (int l = 0; l < 9; ++l) k.v[l] = c == 2 ? FE_D2.v[l] : 0u;
-  k.v[0] = c == 2 ? FE_D2.v[0] : (c == 3 ? 2u : 1u);
-  fe_mul(r, p, k);                        // A, B, C, D
+    for (int l = 0; l < 9; ++l) r.v[l] = p.v[l] << sh;
+  } else {
+    // lanes 2 and 3 scale by 2d and 2, lanes 0 and 1 by 1
+    fe k;
+#pragma unroll
+    for (int l = 0; l < 9; ++l) k.v[l] = c == 2 ? FE_D2.v[l] : 0u;
+    k.v[0] = c == 2 ? FE_D2.v[0] : (c == 3 ? 2u : 1u);
+    fe_mul(r, p, k);                      // A, B, C, D
+  }
   fe s, w;
   fe_qperm<QP_SWAP>(s, r);                // B, A, D, C
   // lane 0: E = B - A, 1: H = B + A, 2: G = D + C, 3: F = D - C

@@ -23,6 +23,7 @@ void h_fe_sub(uint32_t *r, const uint32_t *a, const uint32_t *b) { fe z; fe_sub(
 void h_constants(uint32_t *d, uint32_t *d2, uint32_t *s) { st(d, FE_D); st(d2, FE_D2); st(s, FE_SQRTM1); }
 void h_inv2(uint32_t *r) { st(r, FE_INV2); }
 void h_inv_d(uint32_t *r) { st(r, FE_INV_D); }
+void h_inv_d2(uint32_t *r) { st(r, FE_INV_D2); }
 void h_scalar_mod_l(uint32_t *m) { scalar_mod_l(m); }
 void h_scalar_cneg_l(uint32_t *m, uint32_t neg) { scalar_cneg_l(m, neg); }
 void h_recode32_51(uint32_t *e, const uint32_t *m) { recode32<51>(e, m); }

@@ -155,6 +155,8 @@ def test_invert_pow_constants(H):
     assert val(h) == (P + 1) // 2 and max(h) <= N
     H.h_inv_d(h)
     assert val(h) * edref.D % P == 1 and max(h) <= N
+    H.h_inv_d2(h)
+    assert val(h) * 2 * edref.D % P == 1 and max(h) <= N
 
 
 def enc_words(pt):
