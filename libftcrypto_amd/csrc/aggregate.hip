@@ -1239,10 +1239,24 @@ hipError_t segsum_launch_ch(PtView out, const int64_t *d_off, size_t nseg, size_
   return launch_seg_up(out, rec, keys, rec2, keys2, T, d_off, ch1, lr, s);
 }
 
+// Level 1's chunk for n elements: the class's tuned chunk, made smaller
+// while the level would have fewer lanes than it had at the length the class
+// was tuned at (up to 2^17 lanes, two waves on each of 1024 SIMDs).  A class
+// spans a factor of 16 in length; without this, the top class's chunk timed
+// at 2^24 elements left half the GPU idle at 2^22.
+static int seg1_for(size_t n) {
+  const AggTune t = tune_for(n);
+  const size_t lanes_t = tune_class_size(tune_class(n)) >> t.seg1;
+  const size_t want = lanes_t < ((size_t)1 << 17) ? lanes_t : ((size_t)1 << 17);
+  int ch = t.seg1;
+  while (ch > CH_MIN && (n >> ch) < want) --ch;
+  return ch;
+}
+
 hipError_t segsum_launch(PtView out, const int64_t *d_off, size_t nseg, size_t n_elems, const int32_t *gather,
                          PtView src, size_t n_src, PtView scratch, hipStream_t s) {
   const AggTune t = tune_for(n_elems);
-  return segsum_launch_ch(out, d_off, nseg, n_elems, gather, src, n_src, scratch, t.seg1, t.segu, s);
+  return segsum_launch_ch(out, d_off, nseg, n_elems, gather, src, n_src, scratch, seg1_for(n_elems), t.segu, s);
 }
 
 hipError_t launch_seg_empty(PtView out, const int64_t *off, size_t nseg, hipStream_t s) {

@@ -74,6 +74,7 @@ constexpr unsigned REDUCE_BLOCKS_MAX = 4096;
 // reduces of at most this many points run their first pass on quad lanes
 constexpr size_t REDUCE_QUAD_MAX = (size_t)1 << 18;
 int tune_class(size_t n);
+size_t tune_class_size(int c);                   // the length each class is tuned at
 AggTune tune_for(size_t n);
 void tune_init_from_env();
 

@@ -97,6 +97,10 @@ void tune_store(int c, const AggTune &t) {
 
 }  // namespace
 
+// the length at which ftx_autotune times each class
+static const int TUNE_LOGN[TUNE_CLASSES] = {12, 16, 20, 24};
+size_t tune_class_size(int c) { return (size_t)1 << TUNE_LOGN[c]; }
+
 int tune_class(size_t n) {
   if (n < ((size_t)1 << 14)) return 0;
   if (n < ((size_t)1 << 18)) return 1;
@@ -277,8 +281,7 @@ ft_error ftx_autotune(int max_class, const char *path) {
   if (max_class < 0 || max_class >= TUNE_CLASSES) return FT_ERR_OUT_OF_BOUNDS;
   if (!base_table()) return FT_ERR_INSUFFICIENT_DEVICE_OR_DRIVER;
   if (!T_ready) tune_defaults();
-  static const int LOGN[TUNE_CLASSES] = {12, 16, 20, 24};
-  const size_t nmax = (size_t)1 << LOGN[max_class];
+  const size_t nmax = tune_class_size(max_class);
   Stream st;
   TCK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
   Events ev;
@@ -327,7 +330,7 @@ ft_error ftx_autotune(int max_class, const char *path) {
   TCK(hipMemcpyAsync(er.p, pts.p, nenc * 32, hipMemcpyDeviceToDevice, st.s));
   const PtView E1{(uint32_t *)e1.p, nenc}, E2{(uint32_t *)e2.p, nenc};
   for (int c = 0; c <= max_class; ++c) {
-    const size_t n = (size_t)1 << LOGN[c];
+    const size_t n = tune_class_size(c);
     const int reps = c < 2 ? 10 : 3;
     // segments covering [0, n): the prefix of the offsets, last one clipped
     size_t nseg = 0;
