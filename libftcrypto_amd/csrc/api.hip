@@ -30,7 +30,8 @@ struct ft_ge25519_array_s {
 };
 
 struct ftx_key_s {
-  uint32_t *table;  // device comb table of H
+  uint32_t *table;    // device comb table of H
+  uint32_t *table32;  // device radix-32 comb table of H
 };
 
 struct ftx_graph_s {
@@ -366,6 +367,7 @@ namespace ft {
 const uint32_t *base_table() { return G.btab; }
 const uint32_t *base_table32() { return G.btab32; }
 const uint32_t *key_table(const ftx_key_s *key) { return key->table; }
+const uint32_t *key_table32(const ftx_key_s *key) { return key->table32; }
 }  // namespace ft
 
 extern "C" {
@@ -398,7 +400,7 @@ ft_error ft_crypto_init(void) {
   CK(hipMalloc(&G.flag, 8));
   CK(hipMemcpyAsync(G.lnaf, naf, sizeof naf, hipMemcpyHostToDevice, G.stream));
   CK(launch_build_comb(G.btab, PtView{nullptr, 0}, 0, true, G.stream));
-  CK(launch_build_comb32(G.btab32, G.stream));
+  CK(launch_build_comb32(G.btab32, PtView{nullptr, 0}, 0, true, G.stream));
   CK(hipStreamSynchronize(G.stream));
   CK(noalloc_resolve(G.device));
   tune_init_from_env();
@@ -897,15 +899,21 @@ ft_error ftx_key_create(ftx_key *key, const_ft_ge25519_array pk, size_t pk_idx) 
   if (pk_idx >= pk->n) return FT_ERR_OUT_OF_BOUNDS;
   auto *k = new (std::nothrow) ftx_key_s;
   if (!k) return FT_ERR_OUT_OF_MEMORY;
-  if (hipMalloc(&k->table, COMB_TABLE_WORDS * 4) != hipSuccess) {
+  k->table = nullptr;
+  k->table32 = nullptr;
+  if (hipMalloc(&k->table, COMB_TABLE_WORDS * 4) != hipSuccess ||
+      hipMalloc(&k->table32, COMB32_TABLE_WORDS * 4) != hipSuccess) {
     (void)hipGetLastError();
+    if (k->table) (void)hipFree(k->table);
     delete k;
     return FT_ERR_OUT_OF_MEMORY;
   }
   hipError_t e = launch_build_comb(k->table, view(pk), pk_idx, false, G.stream);
+  if (e == hipSuccess) e = launch_build_comb32(k->table32, view(pk), pk_idx, false, G.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
   if (e != hipSuccess) {
     (void)hipFree(k->table);
+    (void)hipFree(k->table32);
     delete k;
     return herr(e);
   }
@@ -917,6 +925,7 @@ void ftx_key_free(ftx_key key) {
   if (!key) return;
   (void)hipStreamSynchronize(G.stream);
   (void)hipFree(key->table);
+  (void)hipFree(key->table32);
   delete key;
 }
 
@@ -936,7 +945,8 @@ ft_error ftx_encrypt_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key key, 
   auto *fail = static_cast<unsigned long long *>(d_fail);
   if (fail) CK(launch_msg_bound((const uint64_t *)d_m, bounded ? c1->n : 0, msg_bits, fail, S(stream)));
   CK(launch_encrypt(view(c1), view(c2), G.btab, key->table, (const uint64_t *)d_m, msg_digits(msg_bits),
-                    (const uint32_t *)d_r, c1->n, false, S(stream), bounded ? fail : nullptr));
+                    (const uint32_t *)d_r, c1->n, false, S(stream), bounded ? fail : nullptr, G.btab32,
+                    key->table32));
   return FT_ERR_NO_ERROR;
 }
 
@@ -984,7 +994,7 @@ ft_error ftx_encrypt(ft_ge25519_array *c1, ft_ge25519_array *c2, ftx_key key, co
   CKE(dm.upload(m, n * 8));
   CKE(dr.upload(r, n * 32));
   CK(launch_encrypt(view(a1.a), view(a2.a), G.btab, key->table, (const uint64_t *)dm.p, msg_digits(msg_bits),
-                    (const uint32_t *)dr.p, n, false, G.stream));
+                    (const uint32_t *)dr.p, n, false, G.stream, nullptr, G.btab32, key->table32));
   CK(hipStreamSynchronize(G.stream));
   *c1 = a1.release();
   *c2 = a2.release();
@@ -996,7 +1006,7 @@ ft_error ftx_rerandomize_dev(ft_ge25519_array c1, ft_ge25519_array c2, ftx_key k
   if (c1->n != c2->n) return FT_ERR_SIZE_MISMATCH;
   if (c1->d == c2->d) return FT_ERR_BAD_SIZE;
   CK(launch_encrypt(view(c1), view(c2), G.btab, key->table, nullptr, 0, (const uint32_t *)d_r, c1->n, true,
-                    S(stream)));
+                    S(stream), nullptr, G.btab32, key->table32));
   return FT_ERR_NO_ERROR;
 }
 

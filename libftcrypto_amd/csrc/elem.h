@@ -170,6 +170,34 @@ __device__ __forceinline__ void encrypt_half(PtView c1, PtView c2, size_t i, uin
   store_p3(dst, i, a);
 }
 
+// One half of an encryption over radix-32 tables: half 0 C1 = r B, half 1
+// C2 = r H + m B (ACC: C1 += r B, C2 += r H).  The half must be the same on
+// every lane of the wave: the rows are read by scalar loads from the table
+// the half selects.  r is rewritten by the radix-16 top-digit rule and
+// reduced mod l as for the fixed base (the same multiple of B and H, both
+// of order l); the message takes nd32 rows of B's table.
+template <bool ACC>
+__device__ __forceinline__ void encrypt_half32(PtView c1, PtView c2, size_t i, uint32_t half,
+                                               const uint32_t *__restrict__ tb32, const uint32_t *__restrict__ th32,
+                                               uint64_t x, int nd32, const uint32_t *r) {
+  uint32_t k[8], mm[8], e[13];
+  ld_words(k, r);
+  const uint32_t neg = scalar_effective(mm, k);
+  scalar_mod_l(mm);
+  scalar_cneg_l(mm, neg);
+  recode32<COMB32_ROWS>(e, mm);
+  ge_p3 a;
+  comb32_run<13>(a, half ? th32 : tb32, e, COMB32_ROWS);
+  if (!ACC && half) {
+    uint32_t d[4];
+    recode32_msg(d, x, nd32);
+    comb32_add_rows<4>(a, tb32, d, nd32);
+  }
+  const PtView dst = half ? c2 : c1;
+  if (ACC) add_stored(a, dst, i);
+  store_p3(dst, i, a);
+}
+
 // Four lanes per ciphertext (lane q of the group: C1 for q < 2, C2 for
 // q >= 2; the odd digits of that comb for even q, the even digits for odd q):
 // each lane runs one phase of its comb, 32 windows plus the message's, the

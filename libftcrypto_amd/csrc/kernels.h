@@ -19,15 +19,20 @@ hipError_t launch_varbase_w(PtView pts, const uint32_t *sc, size_t sc_stride_wor
 hipError_t launch_mul_u32(PtView a, PtView b, bool two, const uint32_t *k, size_t n, hipStream_t s);
 hipError_t launch_mul_u32_w(PtView a, PtView b, bool two, const uint32_t *k, size_t n, int w, hipStream_t s);
 hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
-hipError_t launch_build_comb32(uint32_t *table, hipStream_t s);
+hipError_t launch_build_comb32(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s);
 // gate: a device word; when it is nonzero at launch time the launch writes
 // nothing (the message-bound check of bounded encryptions), null: ungated
+// btab / htab: radix-16 comb tables of B and H (staged in LDS by the one-
+// and four-lane schedules); btab32 / htab32: their radix-32 tables, which
+// the two-lane schedule reads through scalar loads (required for it)
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
-                          hipStream_t s, const unsigned long long *gate = nullptr);
+                          hipStream_t s, const unsigned long long *gate = nullptr,
+                          const uint32_t *btab32 = nullptr, const uint32_t *htab32 = nullptr);
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
-                            hipStream_t s, const unsigned long long *gate = nullptr);
+                            hipStream_t s, const unsigned long long *gate = nullptr,
+                            const uint32_t *btab32 = nullptr, const uint32_t *htab32 = nullptr);
 // fail = 1 + the largest index i with m[i] >= 2^bits, 0 when none (0 < bits < 64)
 hipError_t launch_msg_bound(const uint64_t *m, size_t n, int bits, unsigned long long *fail, hipStream_t s);
 // several ciphertext arrays encrypted in one launch (two lanes per ciphertext);
@@ -134,5 +139,6 @@ hipError_t noalloc_scratch(PtView *v);
 const uint32_t *base_table();
 const uint32_t *base_table32();
 const uint32_t *key_table(const struct ftx_key_s *key);
+const uint32_t *key_table32(const struct ftx_key_s *key);
 
 }  // namespace ft

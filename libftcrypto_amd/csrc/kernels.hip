@@ -137,9 +137,9 @@ __global__ void __launch_bounds__(TPB) k_build_comb(uint32_t *table, PtView src,
   comb_entry_store(table + (r * 8 + j) * COMB_ENTRY_WORDS, en);
 }
 
-// table[r][j] = (j+1) * 32^r * B in the same form, r < 51, j < 16: one
-// entry per lane
-__global__ void __launch_bounds__(TPB) k_build_comb32(uint32_t *table) {
+// table[r][j] = (j+1) * 32^r * P in the same form, r < 51, j < 16, for P =
+// B or src[idx]: one entry per lane
+__global__ void __launch_bounds__(TPB) k_build_comb32(uint32_t *table, PtView src, size_t idx, int use_base) {
   const int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= COMB32_ROWS * 16) return;
   const int r = t >> 4, j = t & 15;
@@ -148,12 +148,16 @@ __global__ void __launch_bounds__(TPB) k_build_comb32(uint32_t *table) {
   k[w] = (uint32_t)(j + 1) << sh;
   if (sh > 27) k[w + 1] = (uint32_t)(j + 1) >> (32 - sh);
   recode16(e, k);
-  uint32_t bw[8];
-  bw[0] = 0x66666658u;
-#pragma unroll
-  for (int q = 1; q < 8; ++q) bw[q] = 0x66666666u;
   ge_p3 p, h;
-  ge_decode(p, bw);
+  if (use_base) {
+    uint32_t bw[8];
+    bw[0] = 0x66666658u;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) bw[q] = 0x66666666u;
+    ge_decode(p, bw);
+  } else {
+    load_p3(p, src, idx);
+  }
   varbase_windows(h, p, e, 0);
   ge_precomp en;
   ge_p3_to_precomp(en, h);
@@ -208,41 +212,21 @@ __global__ void __launch_bounds__(TPB) k_encrypt(PtView c1, PtView c2, const uin
   encrypt_block<ACC, TPB>(c1, c2, btab, htab, m, nd, r, n, gate);
 }
 
-// small batches: two lanes per ciphertext (encrypt_half)
-template <bool ACC>
-__global__ void __launch_bounds__(TPB) k_encrypt_split(PtView c1, PtView c2, const uint32_t *btab,
-                                                       const uint32_t *htab, const uint64_t *m, int nd,
-                                                       const uint32_t *r, size_t n, const unsigned long long *gate) {
+// Two waves per 64 ciphertexts over radix-32 tables (encrypt_half32): wave
+// pair k holds ciphertexts [64 k, 64 k + 64), its even wave C1, its odd wave
+// C2, so that each wave reads one table, at wave-uniform addresses, through
+// scalar loads.  No LDS: BLK-lane blocks at four waves per SIMD.
+template <bool ACC, int BLK>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_encrypt_w32(PtView c1, PtView c2, const uint32_t *__restrict__ btab32, const uint32_t *__restrict__ htab32,
+              const uint64_t *m, int nd32, const uint32_t *r, size_t n, const unsigned long long *gate) {
+  static_assert(BLK % 128 == 0, "whole wave pairs per block");
   if (gated(gate)) return;
-  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
-  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
-  lds_load_table(tb, btab);
-  lds_load_table(th, htab);
-  __syncthreads();
-  const size_t t = blockIdx.x * (size_t)TPB + threadIdx.x;
-  const size_t i = t >> 1;
+  const size_t wv = (size_t)blockIdx.x * (BLK / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t half = (uint32_t)(wv & 1);
+  const size_t i = ((wv >> 1) << 6) + (threadIdx.x & 63);
   if (i >= n) return;
-  encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
-}
-
-// the same in 512-lane blocks at four waves per SIMD: two blocks (two comb
-// table pairs, 112 KB of LDS) fit a CU, so a launch running beside another
-// one (the two arrays of a ciphertext batch on two streams) reaches four
-// waves per SIMD where 256-lane blocks stop at two
-template <bool ACC>
-__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_encrypt_split_wide(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
-                     const uint32_t *r, size_t n, const unsigned long long *gate) {
-  if (gated(gate)) return;
-  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
-  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
-  lds_load_table(tb, btab);
-  lds_load_table(th, htab);
-  __syncthreads();
-  const size_t t = blockIdx.x * (size_t)ETPB + threadIdx.x;
-  const size_t i = t >> 1;
-  if (i >= n) return;
-  encrypt_half<ACC>(c1, c2, i, (uint32_t)(t & 1), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
+  encrypt_half32<ACC>(c1, c2, i, half, btab32, htab32, ACC ? 0ull : m[i], nd32, r + i * 8);
 }
 
 // four lanes per ciphertext (encrypt_quarter) in 512-lane blocks at four
@@ -621,17 +605,51 @@ hipError_t launch_build_comb(uint32_t *table, PtView src, size_t idx, bool use_b
   FT_LAUNCH_CHECK();
 }
 
-hipError_t launch_build_comb32(uint32_t *table, hipStream_t s) {
-  hipLaunchKernelGGL(k_build_comb32, dim3((COMB32_ROWS * 16 + TPB - 1) / TPB), dim3(TPB), 0, s, table);
+hipError_t launch_build_comb32(uint32_t *table, PtView src, size_t idx, bool use_base, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_comb32, dim3((COMB32_ROWS * 16 + TPB - 1) / TPB), dim3(TPB), 0, s, table, src, idx,
+                     (int)use_base);
   FT_LAUNCH_CHECK();
 }
 
 // lanes: 1 (one lane per ciphertext), 2 (C1 and C2 on separate lanes) or
 // 4 (each comb's odd and even digits on separate lanes as well)
+// radix-32 message digits for a radix-16 count nd (msg_digits: bits / 4
+// rounded up, plus one; 17 for any uint64): (bits + 5) / 5 with bits the
+// largest the count allows
+static int msg_digits32(int nd) {
+  if (nd <= 0) return 0;
+  const int bits = nd >= 17 ? 64 : 4 * (nd - 1);
+  return (bits + 5) / 5;
+}
+
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
-                            hipStream_t s, const unsigned long long *gate) {
+                            hipStream_t s, const unsigned long long *gate, const uint32_t *btab32,
+                            const uint32_t *htab32) {
   if (!n) return hipSuccess;
+  if (lanes == 2) {
+    if (!btab32 || !htab32) return hipErrorInvalidValue;
+    const size_t waves = 2 * ((n + 63) / 64);
+    const int nd32 = accumulate ? 0 : msg_digits32(nd);
+    if (2 * n >= ENC_SPLIT_WIDE_MIN) {
+      const dim3 grid((unsigned)((waves * 64 + ETPB - 1) / ETPB));
+      if (accumulate)
+        hipLaunchKernelGGL((k_encrypt_w32<true, ETPB>), grid, dim3(ETPB), 0, s, c1, c2, btab32, htab32, m, 0, r, n,
+                           gate);
+      else
+        hipLaunchKernelGGL((k_encrypt_w32<false, ETPB>), grid, dim3(ETPB), 0, s, c1, c2, btab32, htab32, m, nd32, r,
+                           n, gate);
+    } else {
+      const dim3 grid((unsigned)((waves * 64 + TPB - 1) / TPB));
+      if (accumulate)
+        hipLaunchKernelGGL((k_encrypt_w32<true, TPB>), grid, dim3(TPB), 0, s, c1, c2, btab32, htab32, m, 0, r, n,
+                           gate);
+      else
+        hipLaunchKernelGGL((k_encrypt_w32<false, TPB>), grid, dim3(TPB), 0, s, c1, c2, btab32, htab32, m, nd32, r, n,
+                           gate);
+    }
+    FT_LAUNCH_CHECK();
+  }
   if (lanes == 1 && n >= ENC_WIDE_MIN) {
     const dim3 grid(grid_for(n, ETPB));
     if (accumulate)
@@ -644,18 +662,6 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
       hipLaunchKernelGGL(k_encrypt<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
     else
       hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
-  } else if (lanes == 2 && 2 * n >= ENC_SPLIT_WIDE_MIN) {
-    const dim3 grid(grid_for(2 * n, ETPB));
-    if (accumulate)
-      hipLaunchKernelGGL(k_encrypt_split_wide<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
-    else
-      hipLaunchKernelGGL(k_encrypt_split_wide<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
-  } else if (lanes == 2) {
-    const dim3 grid(grid_for(2 * n));
-    if (accumulate)
-      hipLaunchKernelGGL(k_encrypt_split<true>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
-    else
-      hipLaunchKernelGGL(k_encrypt_split<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else if (lanes == 4) {
     static_assert(ETPB % 4 == 0, "whole groups of four lanes per block");
     const dim3 grid(grid_for(4 * n, ETPB));
@@ -709,8 +715,9 @@ hipError_t launch_msg_bound_batch(EncBatch &B, int bits, unsigned long long *fai
 
 hipError_t launch_encrypt(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                           const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate,
-                          hipStream_t s, const unsigned long long *gate) {
-  return launch_encrypt_l(c1, c2, btab, htab, m, nd, r, n, accumulate, tune_for(n).encl, s, gate);
+                          hipStream_t s, const unsigned long long *gate, const uint32_t *btab32,
+                          const uint32_t *htab32) {
+  return launch_encrypt_l(c1, c2, btab, htab, m, nd, r, n, accumulate, tune_for(n).encl, s, gate, btab32, htab32);
 }
 
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s) {

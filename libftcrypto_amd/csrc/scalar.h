@@ -80,4 +80,24 @@ FT_HD void recode32(uint32_t e[(ND + 3) / 4], const uint32_t m[8]) {
   }
 }
 
+// signed radix-32 digits of a message x < 2^bits over nd = (bits + 5) / 5
+// digits (nd <= 13), the top one keeping the last carry: at most 16 (its
+// field holds at most 4 bits of x), as bytes like recode32; digits from nd
+// on are zero
+FT_HD void recode32_msg(uint32_t e[4], uint64_t x, int nd) {
+  int carry = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) e[q] = 0;
+#pragma unroll
+  for (int v = 0; v < 13; ++v) {
+    const int d = (int)((x >> (5 * v)) & 31u) + carry;
+    const int c = (d + 16) >> 5;
+    const bool top = v == nd - 1;
+    const int out = top ? d : d - c * 32;
+    carry = top ? 0 : c;
+    const uint32_t db = v < nd ? ((uint32_t)out & 0xffu) : 0u;
+    e[v >> 2] |= db << (8 * (v & 3));
+  }
+}
+
 }  // namespace ft

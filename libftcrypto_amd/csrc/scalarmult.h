@@ -172,6 +172,24 @@ __device__ __forceinline__ void comb32_run(ge_p3 &h, const uint32_t *__restrict_
   pin_p3(h);
 }
 
+// h += sum over rows q < nrows of d_q * row q of a radix-32 table, digits
+// from a low-first byte stream of NW words
+template <int NW>
+__device__ __forceinline__ void comb32_add_rows(ge_p3 &h, const uint32_t *__restrict__ tab, uint32_t s[NW],
+                                                int nrows) {
+  ge_precomp t;
+  ge_p1p1 r;
+  asm volatile("" : "+s"(nrows));
+#pragma unroll 1
+  for (int q = 0; q < nrows; ++q) {
+    const int b = pop_lo<NW>(s, q);
+    comb_select_uniform<16>(t, tab + q * COMB32_ROW_WORDS, b);
+    ge_madd(r, h, t);
+    ge_p1p1_to_p3(h, r);
+  }
+  pin_p3(h);
+}
+
 __device__ __forceinline__ void lds_load_table(uint32_t *lds, const uint32_t *g) {
   for (int i = threadIdx.x; i < COMB_TABLE_WORDS / 4; i += blockDim.x)
     reinterpret_cast<uint4 *>(lds)[i] = reinterpret_cast<const uint4 *>(g)[i];

@@ -407,7 +407,7 @@ ft_error ftx_autotune(int max_class, const char *path) {
       const size_t ne = std::min(n, ENC_TUNE_MAX);
       ft_error e = time_ms(&t, ev, st.s, c < 2 ? 3 : 1, [&] {
         return launch_encrypt_l(E1, E2, base_table(), base_table(), (const uint64_t *)sc.p, 1,
-                                (const uint32_t *)er.p, ne, false, l, st.s);
+                                (const uint32_t *)er.p, ne, false, l, st.s, nullptr, base_table32(), base_table32());
       });
       if (e != FT_ERR_NO_ERROR) return e;
       if (t < bt) { bt = t; best.encl = l; }
