@@ -976,7 +976,7 @@ ft_error ftx_encrypt_batch_dev(const ftx_enc_job *jobs, size_t n_jobs, ftx_key k
       for (int q = 0; q < C.J; ++q) C.job[q].n = 0;
     CK(launch_msg_bound_batch(C, bounded ? msg_bits : 1, fail, S(stream)));
   }
-  CK(launch_encrypt_batch(B, G.btab, key->table, msg_digits(msg_bits), S(stream), bounded ? fail : nullptr));
+  CK(launch_encrypt_batch(B, G.btab32, key->table32, msg_digits(msg_bits), S(stream), bounded ? fail : nullptr));
   return FT_ERR_NO_ERROR;
 }
 

@@ -130,46 +130,6 @@ __device__ __forceinline__ void encrypt_one(PtView c1, PtView c2, size_t i, cons
   store_p3(c2, i, a2);
 }
 
-// One half of an encryption per lane, for small batches (two lanes per
-// ciphertext keep twice as many waves busy): half 0 computes C1 = r B, half 1
-// C2 = r H + m B.  Both run the same window count, the message windows of
-// half 0 selecting the identity, so the lanes of a wave never diverge.
-template <bool ACC>
-__device__ __forceinline__ void encrypt_half(PtView c1, PtView c2, size_t i, uint32_t half, const uint32_t *tb,
-                                             const uint32_t *th, uint64_t x, int nd, const uint32_t *r) {
-  uint32_t k[8], e[16], odd[11], even[11];
-  ld_words(k, r);
-  recode16(e, k);
-#pragma unroll
-  for (int w = 8; w < 11; ++w) { odd[w] = 0; even[w] = 0; }
-  if (!ACC) {
-    const uint64_t xm = half ? x : 0ull;
-    int carry = 0;
-#pragma unroll
-    for (int q = 0; q < 17; ++q) {
-      int d = (q < 16 ? (int)((xm >> (4 * q)) & 15u) : 0) + carry;
-      carry = (d + 8) >> 4;
-      d -= carry << 4;
-      if (q == nd - 1) d += carry << 4;
-      const uint32_t db = q < nd ? ((uint32_t)d & 0xffu) : 0u;
-      const int h = q >> 1;
-      if (q & 1) odd[8 + (h >> 2)] |= db << (8 * (h & 3));
-      else even[8 + (h >> 2)] |= db << (8 * (h & 3));
-      if (q >= nd - 1) carry = 0;
-    }
-  }
-  const int mo = ACC ? 0 : nd >> 1, me = ACC ? 0 : (nd + 1) >> 1;
-  split_odd_even(odd, even, e);
-  ge_p3 a;
-  ge_p3_0(a);
-  comb_run<11>(a, half ? th : tb, tb, odd, even, 32 + mo, 32 + me);
-  const PtView dst = half ? c2 : c1;
-  if (ACC) {
-    add_stored(a, dst, i);
-  }
-  store_p3(dst, i, a);
-}
-
 // One half of an encryption over radix-32 tables: half 0 C1 = r B, half 1
 // C2 = r H + m B (ACC: C1 += r B, C2 += r H).  The half must be the same on
 // every lane of the wave: the rows are read by scalar loads from the table
@@ -204,8 +164,8 @@ __device__ __forceinline__ void encrypt_half32(PtView c1, PtView c2, size_t i, u
 // same count on every lane (the odd-digit lane adds an identity when the
 // message has an odd number of digits), the odd-digit lane multiplies its sum
 // by 16 (the comb's four doublings), and the pair adds its two sums: the
-// same point as encrypt_half's for about 5 % more instructions over twice
-// the lanes, for batches too small to fill the GPU two lanes per ciphertext.
+// same point as a whole comb's over twice the lanes, for batches too small
+// to fill the GPU two lanes per ciphertext.
 // Lanes q and q ^ 1 are neighbours: their sums cross by a DPP quad swap.
 template <bool ACC>
 __device__ __forceinline__ void encrypt_quarter(PtView c1, PtView c2, size_t i, uint32_t q, const uint32_t *tb,

@@ -36,7 +36,7 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
 // fail = 1 + the largest index i with m[i] >= 2^bits, 0 when none (0 < bits < 64)
 hipError_t launch_msg_bound(const uint64_t *m, size_t n, int bits, unsigned long long *fail, hipStream_t s);
 // several ciphertext arrays encrypted in one launch (two lanes per ciphertext);
-// the table passes by value as a kernel argument; bases (multiples of 32
+// the table passes by value as a kernel argument; bases (multiples of 64
 // ciphertexts) are filled in
 constexpr int ENC_BATCH_MAX = 16;
 struct EncJob {
@@ -49,8 +49,17 @@ struct EncBatch {
   int J;
   EncJob job[ENC_BATCH_MAX];
 };
-hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s,
+// over the radix-32 tables of B and H (two waves per 64 ciphertexts)
+hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab32, const uint32_t *htab32, int nd, hipStream_t s,
                                 const unsigned long long *gate = nullptr);
+// radix-32 message digits for a radix-16 digit count nd (bits / 4 rounded
+// up, plus one; 17 for any uint64): (bits + 5) / 5 for the largest bits the
+// count allows, the top digit keeping the last carry (at most 16)
+inline int msg_digits32(int nd) {
+  if (nd <= 0) return 0;
+  const int bits = nd >= 17 ? 64 : 4 * (nd - 1);
+  return (bits + 5) / 5;
+}
 // bit j of fail set when a message of job j is >= 2^bits, 0 when none
 hipError_t launch_msg_bound_batch(EncBatch &B, int bits, unsigned long long *fail, hipStream_t s);
 hipError_t launch_encode(uint32_t *out, PtView v, size_t n, hipStream_t s);

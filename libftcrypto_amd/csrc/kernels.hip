@@ -248,42 +248,34 @@ k_encrypt_quarter(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *ht
   encrypt_quarter<ACC>(c1, c2, i, (uint32_t)(t & 3), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
 }
 
-// Several ciphertext arrays encrypted in one launch, two lanes per
-// ciphertext, the jobs' ciphertexts laid end to end from multiples of 32 (a
-// wave's 32 ciphertexts), so that the job of a wave is one: lane pair t finds
-// it in the table (a kernel argument), as a scalar, and runs encrypt_half.
+// Several ciphertext arrays encrypted in one launch over radix-32 tables,
+// wave pairs as in k_encrypt_w32: the jobs' ciphertexts are laid end to end
+// from multiples of 64, so that a wave pair's job is one, which each wave
+// finds in the job table (a kernel argument) as a scalar.
 template <int BS>
-__device__ __forceinline__ void encrypt_batch_body(const EncBatch &B, const uint32_t *btab, const uint32_t *htab,
-                                                   int nd, uint64_t total, const unsigned long long *gate) {
+__device__ __forceinline__ void encrypt_batch_body32(const EncBatch &B, const uint32_t *__restrict__ btab32,
+                                                     const uint32_t *__restrict__ htab32, int nd32, uint64_t total,
+                                                     const unsigned long long *gate) {
+  static_assert(BS % 128 == 0, "whole wave pairs per block");
   if (gated(gate)) return;
-  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
-  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
-  lds_load_table(tb, btab);
-  lds_load_table(th, htab);
-  __syncthreads();
-  const uint64_t t2 = blockIdx.x * (uint64_t)BS + threadIdx.x;
-  const uint64_t t = t2 >> 1;
-  if (t >= total) return;
+  const uint64_t wv = (uint64_t)blockIdx.x * (BS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t half = (uint32_t)(wv & 1);
+  const uint64_t t0 = (wv >> 1) << 6;             // the wave pair's first ciphertext: wave-uniform,
+  if (t0 >= total) return;                        // so the job and its fields are scalars
   int j = 0;
   for (int q = 1; q < B.J; ++q)
-    if (B.job[q].base <= t) j = q;
-  j = __builtin_amdgcn_readfirstlane(j);          // the same for every lane of the wave
-  const uint64_t i = t - B.job[j].base;
-  if (i >= B.job[j].n) return;                    // padding up to the next job's start
-  encrypt_half<false>(B.job[j].c1, B.job[j].c2, i, (uint32_t)(t2 & 1), tb, th, B.job[j].m[i], nd,
-                      B.job[j].r + i * 8);
+    if (B.job[q].base <= t0) j = q;
+  const EncJob &J = B.job[j];
+  const uint64_t i = t0 - J.base + (threadIdx.x & 63);
+  if (i >= J.n) return;                           // padding up to the next job's start
+  encrypt_half32<false>(J.c1, J.c2, i, half, btab32, htab32, J.m[i], nd32, J.r + i * 8);
 }
 
-__global__ void __launch_bounds__(TPB) k_encrypt_batch(const EncBatch B, const uint32_t *btab,
-                                                       const uint32_t *htab, int nd, uint64_t total,
-                                                       const unsigned long long *gate) {
-  encrypt_batch_body<TPB>(B, btab, htab, nd, total, gate);
-}
-
-__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_encrypt_batch_wide(const EncBatch B, const uint32_t *btab, const uint32_t *htab, int nd, uint64_t total,
-                     const unsigned long long *gate) {
-  encrypt_batch_body<ETPB>(B, btab, htab, nd, total, gate);
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_encrypt_batch32(const EncBatch B, const uint32_t *btab32, const uint32_t *htab32, int nd32, uint64_t total,
+                  const unsigned long long *gate) {
+  encrypt_batch_body32<BS>(B, btab32, htab32, nd32, total, gate);
 }
 
 // message bound of a batch: bit j of fail is set when a message of job j has
@@ -613,15 +605,6 @@ hipError_t launch_build_comb32(uint32_t *table, PtView src, size_t idx, bool use
 
 // lanes: 1 (one lane per ciphertext), 2 (C1 and C2 on separate lanes) or
 // 4 (each comb's odd and even digits on separate lanes as well)
-// radix-32 message digits for a radix-16 count nd (msg_digits: bits / 4
-// rounded up, plus one; 17 for any uint64): (bits + 5) / 5 with bits the
-// largest the count allows
-static int msg_digits32(int nd) {
-  if (nd <= 0) return 0;
-  const int bits = nd >= 17 ? 64 : 4 * (nd - 1);
-  return (bits + 5) / 5;
-}
-
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
                             hipStream_t s, const unsigned long long *gate, const uint32_t *btab32,
@@ -675,27 +658,31 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
   FT_LAUNCH_CHECK();
 }
 
-// bases of the batch's jobs (multiples of 32 ciphertexts) and their total
+// bases of the batch's jobs (multiples of 64 ciphertexts: a wave pair's) and
+// their total
 static uint64_t batch_bases(EncBatch &B) {
   uint64_t total = 0;
   for (int q = 0; q < B.J; ++q) {
-    total = (total + 31) & ~(uint64_t)31;
+    total = (total + 63) & ~(uint64_t)63;
     B.job[q].base = total;
     total += B.job[q].n;
   }
   return total;
 }
 
-hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab, const uint32_t *htab, int nd, hipStream_t s,
+hipError_t launch_encrypt_batch(EncBatch &B, const uint32_t *btab32, const uint32_t *htab32, int nd, hipStream_t s,
                                 const unsigned long long *gate) {
-  if (B.J < 0 || B.J > ENC_BATCH_MAX) return hipErrorInvalidValue;
-  const uint64_t total = batch_bases(B);            // a wave holds 32 ciphertexts
+  if (B.J < 0 || B.J > ENC_BATCH_MAX || !btab32 || !htab32) return hipErrorInvalidValue;
+  const uint64_t total = batch_bases(B);            // a wave pair holds 64 ciphertexts
   if (!total) return hipSuccess;
-  if (2 * total >= ENC_SPLIT_WIDE_MIN)
-    hipLaunchKernelGGL(k_encrypt_batch_wide, dim3(grid_for(2 * total, ETPB)), dim3(ETPB), 0, s, B, btab, htab, nd,
-                       total, gate);
+  const uint64_t lanes = (total + 63) / 64 * 128;   // a wave pair per 64 ciphertexts
+  const int nd32 = msg_digits32(nd);
+  if (lanes >= ENC_SPLIT_WIDE_MIN)
+    hipLaunchKernelGGL(k_encrypt_batch32<ETPB>, dim3((unsigned)((lanes + ETPB - 1) / ETPB)), dim3(ETPB), 0, s, B,
+                       btab32, htab32, nd32, total, gate);
   else
-    hipLaunchKernelGGL(k_encrypt_batch, dim3(grid_for(2 * total)), dim3(TPB), 0, s, B, btab, htab, nd, total, gate);
+    hipLaunchKernelGGL(k_encrypt_batch32<TPB>, dim3((unsigned)((lanes + TPB - 1) / TPB)), dim3(TPB), 0, s, B,
+                       btab32, htab32, nd32, total, gate);
   FT_LAUNCH_CHECK();
 }
 

@@ -88,33 +88,20 @@ __global__ void __launch_bounds__(TPB) kb_encrypt(const BItem *items, int cnt, u
   kb_encrypt_body<TPB>(items, cnt, total, btab, htab, nd);
 }
 
-// small batches: two lanes per ciphertext (as k_encrypt_split)
+// two waves per 64 ciphertexts over the radix-32 tables (as k_encrypt_w32):
+// the wave's half, hence its table, is uniform; each lane finds its item
 template <int BS>
-__device__ __forceinline__ void kb_encrypt_split_body(const BItem *items, int cnt, uint64_t total,
-                                                      const uint32_t *btab, const uint32_t *htab, int nd) {
-  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
-  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
-  lds_load_table(tb, btab);
-  lds_load_table(th, htab);
-  __syncthreads();
-  const uint64_t t2 = blockIdx.x * (uint64_t)BS + threadIdx.x;
-  const uint64_t t = t2 >> 1;
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4, 4)))
+kb_encrypt_w32(const BItem *items, int cnt, uint64_t total, const uint32_t *__restrict__ btab32,
+               const uint32_t *__restrict__ htab32, int nd32) {
+  static_assert(BS % 128 == 0, "whole wave pairs per block");
+  const uint64_t wv = (uint64_t)blockIdx.x * (BS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t half = (uint32_t)(wv & 1);
+  const uint64_t t = ((wv >> 1) << 6) + (threadIdx.x & 63);
   if (t >= total) return;
   const BItem &it = items[find_item(items, cnt, t)];
   const uint64_t i = t - it.start;
-  encrypt_half<false>(it.a1, it.a2, i, (uint32_t)(t2 & 1), tb, th, it.m[i], nd, it.r + i * 8);
-}
-
-__global__ void __launch_bounds__(TPB) kb_encrypt_split(const BItem *items, int cnt, uint64_t total,
-                                                        const uint32_t *btab, const uint32_t *htab, int nd) {
-  kb_encrypt_split_body<TPB>(items, cnt, total, btab, htab, nd);
-}
-
-// two lanes per ciphertext in 512-lane blocks at four waves per SIMD
-__global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-kb_encrypt_split_wide(const BItem *items, int cnt, uint64_t total, const uint32_t *btab, const uint32_t *htab,
-                      int nd) {
-  kb_encrypt_split_body<ETPB>(items, cnt, total, btab, htab, nd);
+  encrypt_half32<false>(it.a1, it.a2, i, half, btab32, htab32, it.m[i], nd32, it.r + i * 8);
 }
 
 // large batches: 512-lane blocks at four waves per SIMD (as k_encrypt_wide)
@@ -228,6 +215,7 @@ struct Slot {
 struct ftx_server_s {
   ftx_key key = nullptr;
   const uint32_t *btab = nullptr;
+  const uint32_t *btab32 = nullptr;
   bool borrow = false;
   size_t max_batch = 0;
   Arena arena;
@@ -506,11 +494,13 @@ void launch(ftx_server_s *S, int idx, std::vector<Piece> &batch) {
         hipLaunchKernelGGL(kb_encrypt, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab,
                            key_table(S->key), nd);
       else if (2 * start >= ENC_SPLIT_WIDE_MIN)
-        hipLaunchKernelGGL(kb_encrypt_split_wide, dim3((unsigned)((2 * start + ETPB - 1) / ETPB)), dim3(ETPB), 0,
-                           sl.st, ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
+        hipLaunchKernelGGL(kb_encrypt_w32<ETPB>, dim3((unsigned)(((start + 63) / 64 * 128 + ETPB - 1) / ETPB)),
+                           dim3(ETPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab32, key_table32(S->key),
+                           msg_digits32(nd));
       else
-        hipLaunchKernelGGL(kb_encrypt_split, dim3((unsigned)((2 * start + TPB - 1) / TPB)), dim3(TPB), 0, sl.st,
-                           ditems, (int)batch.size(), start, S->btab, key_table(S->key), nd);
+        hipLaunchKernelGGL(kb_encrypt_w32<TPB>, dim3((unsigned)(((start + 63) / 64 * 128 + TPB - 1) / TPB)),
+                           dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start, S->btab32, key_table32(S->key),
+                           msg_digits32(nd));
     } else if (kind == K_ADD) {
       hipLaunchKernelGGL(kb_add, dim3(grid), dim3(TPB), 0, sl.st, ditems, (int)batch.size(), start);
     } else if (kind == K_MUL) {
@@ -708,6 +698,7 @@ ft_error ftx_server_create(ftx_server *out, ftx_key key, size_t hbm_budget_bytes
   S->key = key;
   S->borrow = (flags & FTX_SERVER_BORROW_INPUTS) != 0;
   S->btab = base_table();
+  S->btab32 = base_table32();
   S->max_batch = max_batch ? max_batch : FTX_SERVER_DEFAULT_BATCH;
   if (hipMalloc(&S->arena_mem, hbm_budget_bytes) != hipSuccess) {
     (void)hipGetLastError();

@@ -34,9 +34,9 @@ sys.path.insert(0, ROOT)
 
 SECRET_KERNELS = {
     "kernels.hip": ["k_fixed_base32", "k_encrypt_w32", "k_mul_u32_r8", "k_mul_u32_r16", "k_encrypt",
-                    "k_encrypt_wide", "k_encrypt_batch", "k_encrypt_batch_wide"],
+                    "k_encrypt_wide", "k_encrypt_batch32"],
     "varbase.hip": ["k_varbase", "k_varbase_r8"],
-    "serve.hip": ["kb_encrypt", "kb_encrypt_split", "kb_encrypt_split_wide", "kb_encrypt_wide", "kb_mul_u32_r8",
+    "serve.hip": ["kb_encrypt", "kb_encrypt_w32", "kb_encrypt_wide", "kb_mul_u32_r8",
                   "kb_mul_u32_r16"],
 }
 EXEC = re.compile(r"^\s+(s_cbranch_exec\w+|s_\w+_saveexec_b64|v_cmpx_\w+|s_\w+_b64\s+exec\b)")
