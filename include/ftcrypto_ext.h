@@ -21,7 +21,9 @@
 extern "C" {
 #endif
 
-/* Opaque comb table of a public key H: 32 x 8 affine multiples (28 KiB). */
+/* Opaque comb tables of a public key H: 32 x 8 affine multiples (28 KiB, the
+   radix-16 comb) and 51 x 16 (89 KiB, the radix-32 comb of the two-lane
+   encryption schedule), built once by ftx_key_create. */
 struct ftx_key_s;
 typedef struct ftx_key_s *ftx_key;
 
