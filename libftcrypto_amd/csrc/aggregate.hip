@@ -1588,12 +1588,14 @@ BatchPlan plan_batch(const AggJob *jobs, int J, int ch1) {
 // block sums) onto library-owned streams (default priority, created on first
 // use) and its join back into the caller's stream.  Held under a mutex from fork to join, so calls from several host
 // threads enqueue their record / wait pairs in order.  While s is being
-// captured into a graph, next() returns s itself (no fork).
+// captured into a graph, or when the caller asks for no fork (want false),
+// next() returns s itself.
 constexpr int AUX_STREAMS = 4;
 
 class TailFork {
  public:
-  explicit TailFork(hipStream_t s) : s_(s), lk_(mu(), std::defer_lock) {
+  explicit TailFork(hipStream_t s, bool want = true) : s_(s), lk_(mu(), std::defer_lock) {
+    if (!want) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     lk_.lock();
@@ -1693,7 +1695,7 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
   const PtView rec = at(0);
   int64_t *keys = reinterpret_cast<int64_t *>(scratch.w + 2 * p.C * 8);
   unsigned *ctr = reinterpret_cast<unsigned *>(scratch.w + (p.rec_pts - 1) * 8);
-  hipLaunchKernelGGL(k_agg_ctr0, dim3(1), dim3(1), 0, s, ctr);
+  if (p.B.tiles) hipLaunchKernelGGL(k_agg_ctr0, dim3(1), dim3(1), 0, s, ctr);
   if (p.Sg) hipLaunchKernelGGL(k_agg_empty, dim3((unsigned)((p.Sg + ATPB - 1) / ATPB)), dim3(ATPB), 0, s, p.B);
   if (p.B.tiles) {
     switch (ch1) {
@@ -1730,8 +1732,16 @@ hipError_t agg_batch_launch(const AggJob *jobs, int J, PtView scratch, hipStream
   // the scans' upper levels and second walks and the reduces' block sums go
   // to auxiliary streams forked from s here (eagerly; inside a graph capture
   // they stay on s), so that these tails, latency-bound at their small top
-  // levels, fill each other's gaps; each has its own scratch region
-  TailFork fk(s);
+  // levels, fill each other's gaps; each has its own scratch region.  Only
+  // reduces short enough to be summed straight from their source (two
+  // launches of a few microseconds each) stay on s: there the fork's event
+  // records and waits would cost more than the overlap gains.
+  bool fork = false;
+  for (int q = 0; q < p.B.J; ++q) {
+    const BJob &b = p.B.job[q];
+    if (b.kind == 1 ? b.len >= 0 : scan_kind(jobs[p.from[q]].kind)) fork = true;
+  }
+  TailFork fk(s, fork);
   for (int i = 0; i < J && e == hipSuccess; ++i) {
     if (!scan_kind(jobs[i].kind)) continue;
     e = scan_launch_ch(jobs[i].out, jobs[i].n, jobs[i].kind == AGG_XSCAN, at(p.scan_at[i]), p.scan_ch[i],

@@ -251,7 +251,10 @@ k_encrypt_quarter(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *ht
 // Several ciphertext arrays encrypted in one launch over radix-32 tables,
 // wave pairs as in k_encrypt_w32: the jobs' ciphertexts are laid end to end
 // from multiples of 64, so that a wave pair's job is one, which each wave
-// finds in the job table (a kernel argument) as a scalar.
+// finds in the job table (a kernel argument) as a scalar.  The tables'
+// kernel arguments are __restrict__: without it the compiler cannot show
+// that no store of the kernel writes them and reads the rows through vector
+// loads instead of scalar ones.
 template <int BS>
 __device__ __forceinline__ void encrypt_batch_body32(const EncBatch &B, const uint32_t *__restrict__ btab32,
                                                      const uint32_t *__restrict__ htab32, int nd32, uint64_t total,
@@ -273,8 +276,8 @@ __device__ __forceinline__ void encrypt_batch_body32(const EncBatch &B, const ui
 
 template <int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_encrypt_batch32(const EncBatch B, const uint32_t *btab32, const uint32_t *htab32, int nd32, uint64_t total,
-                  const unsigned long long *gate) {
+k_encrypt_batch32(const EncBatch B, const uint32_t *__restrict__ btab32, const uint32_t *__restrict__ htab32,
+                  int nd32, uint64_t total, const unsigned long long *gate) {
   encrypt_batch_body32<BS>(B, btab32, htab32, nd32, total, gate);
 }
 
