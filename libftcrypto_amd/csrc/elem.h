@@ -158,62 +158,32 @@ __device__ __forceinline__ void encrypt_half32(PtView c1, PtView c2, size_t i, u
   store_p3(dst, i, a);
 }
 
-// Four lanes per ciphertext (lane q of the group: C1 for q < 2, C2 for
-// q >= 2; the odd digits of that comb for even q, the even digits for odd q):
-// each lane runs one phase of its comb, 32 windows plus the message's, the
-// same count on every lane (the odd-digit lane adds an identity when the
-// message has an odd number of digits), the odd-digit lane multiplies its sum
-// by 16 (the comb's four doublings), and the pair adds its two sums: the
-// same point as a whole comb's over twice the lanes, for batches too small
-// to fill the GPU two lanes per ciphertext.
-// Lanes q and q ^ 1 are neighbours: their sums cross by a DPP quad swap.
+// One wave's part of a ciphertext half in the four-wave schedule: the rows of
+// the radix-32 comb of B (half 0, C1) or H (half 1, C2) are split at row
+// `split` (wave-uniform), part 0 summing rows [0, split) and part 1 rows
+// [split, 51), plus, for C2, the message rows over B's table.  The two parts'
+// sums add up to encrypt_half32's point.
 template <bool ACC>
-__device__ __forceinline__ void encrypt_quarter(PtView c1, PtView c2, size_t i, uint32_t q, const uint32_t *tb,
-                                                const uint32_t *th, uint64_t x, int nd, const uint32_t *r) {
-  const uint32_t half = q >> 1, part = q & 1;
-  uint32_t k[8], e[16], odd[11], even[11];
+__device__ __forceinline__ void encrypt_part32(ge_p3 &a, uint32_t half, uint32_t part, int split,
+                                               const uint32_t *__restrict__ tb32, const uint32_t *__restrict__ th32,
+                                               uint64_t x, int nd32, const uint32_t *r) {
+  uint32_t k[8], mm[8], e[13];
   ld_words(k, r);
-  recode16(e, k);
-#pragma unroll
-  for (int w = 8; w < 11; ++w) { odd[w] = 0; even[w] = 0; }
-  if (!ACC) {
-    const uint64_t xm = half ? x : 0ull;
-    int carry = 0;
-#pragma unroll
-    for (int j = 0; j < 17; ++j) {
-      int d = (j < 16 ? (int)((xm >> (4 * j)) & 15u) : 0) + carry;
-      carry = (d + 8) >> 4;
-      d -= carry << 4;
-      if (j == nd - 1) d += carry << 4;
-      const uint32_t db = j < nd ? ((uint32_t)d & 0xffu) : 0u;
-      const int h = j >> 1;
-      if (j & 1) odd[8 + (h >> 2)] |= db << (8 * (h & 3));
-      else even[8 + (h >> 2)] |= db << (8 * (h & 3));
-      if (j >= nd - 1) carry = 0;
+  const uint32_t neg = scalar_effective(mm, k);
+  scalar_mod_l(mm);
+  scalar_cneg_l(mm, neg);
+  recode32<COMB32_ROWS>(e, mm);
+  const uint32_t *tab = half ? th32 : tb32;
+  if (!part) {
+    comb32_run<13>(a, tab, e, split);
+  } else {
+    comb32_run_from<13>(a, tab, e, split, COMB32_ROWS);
+    if (!ACC && half) {
+      uint32_t d[4];
+      recode32_msg(d, x, nd32);
+      comb32_add_rows<4>(a, tb32, d, nd32);
     }
   }
-  split_odd_even(odd, even, e);
-  uint32_t st[11];
-#pragma unroll
-  for (int w = 0; w < 11; ++w) st[w] = part ? even[w] : odd[w];
-  // even digits: ceil(nd / 2) message windows, odd: floor(nd / 2) and a zero digit
-  const int nw = 32 + (ACC ? 0 : (nd + 1) >> 1);
-  ge_p3 a;
-  ge_p3_0(a);
-  comb_run<11>(a, half ? th : tb, tb, st, st, nw, 0);
-  if (!part) dbl4(a);
-  ge_p3 b;
-  fe_qperm<QP_SWAP>(b.X, a.X);
-  fe_qperm<QP_SWAP>(b.Y, a.Y);
-  fe_qperm<QP_SWAP>(b.Z, a.Z);
-  fe_qperm<QP_SWAP>(b.T, a.T);
-  ge_p3_add(a, a, b);
-  if (part) return;
-  const PtView dst = half ? c2 : c1;
-  if (ACC) {
-    add_stored(a, dst, i);
-  }
-  store_p3(dst, i, a);
 }
 
 }  // namespace ft

@@ -229,23 +229,63 @@ k_encrypt_w32(PtView c1, PtView c2, const uint32_t *__restrict__ btab32, const u
   encrypt_half32<ACC>(c1, c2, i, half, btab32, htab32, ACC ? 0ull : m[i], nd32, r + i * 8);
 }
 
-// four lanes per ciphertext (encrypt_quarter) in 512-lane blocks at four
-// waves per SIMD: for batches that two lanes per ciphertext leave at two
-// waves per SIMD or fewer
+// Four waves per 64 ciphertexts over radix-32 tables, for arrays that two
+// waves per 64 leave at two waves per SIMD or fewer: wave quad k holds
+// ciphertexts [64 k, 64 k + 64), its waves 0 and 1 C1 and waves 2 and 3 C2,
+// the even wave of each pair the comb's first rows and the odd wave the rest
+// (encrypt_part32).  The odd wave passes its sum to the even one through LDS
+// (a limb plane per word, lane-contiguous), which adds the two and stores.
+// Every wave reaches the barrier: lanes past n compute element n - 1 again.
+// A quad wholly past n stores nothing (a wave-uniform branch); in the quad
+// holding element n - 1, the lanes past it store that element's value, the
+// same bits to the same place as its own lane, in the same store instruction
+// of the same wave (with ACC all of them load it before any of them stores),
+// so that no branch depends on the lane.
 template <bool ACC>
 __global__ void __launch_bounds__(ETPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_encrypt_quarter(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab, const uint64_t *m, int nd,
-                  const uint32_t *r, size_t n, const unsigned long long *gate) {
-  if (gated(gate)) return;
-  __shared__ __align__(16) uint32_t tb[COMB_TABLE_WORDS];
-  __shared__ __align__(16) uint32_t th[COMB_TABLE_WORDS];
-  lds_load_table(tb, btab);
-  lds_load_table(th, htab);
+k_encrypt_w32q(PtView c1, PtView c2, const uint32_t *__restrict__ btab32, const uint32_t *__restrict__ htab32,
+               const uint64_t *m, int nd32, const uint32_t *r, size_t n, const unsigned long long *gate) {
+  static_assert(ETPB % 256 == 0, "whole wave quads per block");
+  __shared__ uint32_t xs[ETPB / 128][36][64];
+  if (gated(gate)) return;                        // the same for every wave of the grid
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  const size_t wv = (size_t)blockIdx.x * (ETPB / 64) + w;
+  const uint32_t half = (uint32_t)((wv >> 1) & 1), part = (uint32_t)(wv & 1);
+  const size_t i0 = ((wv >> 2) << 6) + lane;
+  const size_t i = i0 < n ? i0 : n - 1;
+  // C2's split point takes half of the message rows to the even wave's side
+  const int split = half ? (COMB32_ROWS + nd32 + 1) / 2 : (COMB32_ROWS + 1) / 2;
+  ge_p3 a;
+  encrypt_part32<ACC>(a, half, part, split, btab32, htab32, ACC ? 0ull : m[i], nd32, r + i * 8);
+  uint32_t (*x)[64] = xs[w >> 1];
+  if (part) {
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      x[l][lane] = a.X.v[l];
+      x[9 + l][lane] = a.Y.v[l];
+      x[18 + l][lane] = a.Z.v[l];
+      x[27 + l][lane] = a.T.v[l];
+    }
+  }
   __syncthreads();
-  const size_t t = blockIdx.x * (size_t)ETPB + threadIdx.x;
-  const size_t i = t >> 2;
-  if (i >= n) return;                     // whole groups of four (ETPB is a multiple of 4)
-  encrypt_quarter<ACC>(c1, c2, i, (uint32_t)(t & 3), tb, th, ACC ? 0ull : m[i], nd, r + i * 8);
+  if (part || ((wv >> 2) << 6) >= n) return;
+  ge_p3 b;
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    b.X.v[l] = x[l][lane];
+    b.Y.v[l] = x[9 + l][lane];
+    b.Z.v[l] = x[18 + l][lane];
+    b.T.v[l] = x[27 + l][lane];
+  }
+  ge_cached bc;
+  ge_p3_to_cached(bc, b);
+  ge_p1p1 t;
+  ge_add(t, a, bc);
+  ge_p1p1_to_p3(a, t);
+  const PtView dst = half ? c2 : c1;
+  if (ACC) add_stored(a, dst, i);
+  store_p3(dst, i, a);
 }
 
 // Several ciphertext arrays encrypted in one launch over radix-32 tables,
@@ -606,8 +646,8 @@ hipError_t launch_build_comb32(uint32_t *table, PtView src, size_t idx, bool use
   FT_LAUNCH_CHECK();
 }
 
-// lanes: 1 (one lane per ciphertext), 2 (C1 and C2 on separate lanes) or
-// 4 (each comb's odd and even digits on separate lanes as well)
+// lanes: 1 (one lane per ciphertext), 2 (C1 and C2 on separate waves) or
+// 4 (each comb's rows split over two waves as well)
 hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const uint32_t *htab,
                             const uint64_t *m, int nd, const uint32_t *r, size_t n, bool accumulate, int lanes,
                             hipStream_t s, const unsigned long long *gate, const uint32_t *btab32,
@@ -649,12 +689,14 @@ hipError_t launch_encrypt_l(PtView c1, PtView c2, const uint32_t *btab, const ui
     else
       hipLaunchKernelGGL(k_encrypt<false>, grid, dim3(TPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
   } else if (lanes == 4) {
-    static_assert(ETPB % 4 == 0, "whole groups of four lanes per block");
-    const dim3 grid(grid_for(4 * n, ETPB));
+    if (!btab32 || !htab32) return hipErrorInvalidValue;
+    const size_t waves = 4 * ((n + 63) / 64);    // a wave quad per 64 ciphertexts
+    const dim3 grid((unsigned)((waves * 64 + ETPB - 1) / ETPB));
     if (accumulate)
-      hipLaunchKernelGGL(k_encrypt_quarter<true>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, 0, r, n, gate);
+      hipLaunchKernelGGL(k_encrypt_w32q<true>, grid, dim3(ETPB), 0, s, c1, c2, btab32, htab32, m, 0, r, n, gate);
     else
-      hipLaunchKernelGGL(k_encrypt_quarter<false>, grid, dim3(ETPB), 0, s, c1, c2, btab, htab, m, nd, r, n, gate);
+      hipLaunchKernelGGL(k_encrypt_w32q<false>, grid, dim3(ETPB), 0, s, c1, c2, btab32, htab32, m,
+                         msg_digits32(nd), r, n, gate);
   } else {
     return hipErrorInvalidValue;
   }

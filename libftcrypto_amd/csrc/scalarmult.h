@@ -172,6 +172,28 @@ __device__ __forceinline__ void comb32_run(ge_p3 &h, const uint32_t *__restrict_
   pin_p3(h);
 }
 
+// h = sum over rows q in [q0, q1) of d_q * row q of a radix-32 table (q0 <
+// q1, both wave-uniform): the stream's first q0 digits are popped unused
+template <int NW>
+__device__ __forceinline__ void comb32_run_from(ge_p3 &h, const uint32_t *__restrict__ tab, uint32_t s[NW], int q0,
+                                                int q1) {
+  ge_precomp t;
+  ge_p1p1 r;
+  asm volatile("" : "+s"(q0), "+s"(q1));
+#pragma unroll 1
+  for (int q = 0; q < q0; ++q) (void)pop_lo<NW>(s, q);
+  comb_select_uniform<16>(t, tab + q0 * COMB32_ROW_WORDS, pop_lo<NW>(s, q0));
+  ge_precomp_to_p3(h, t);
+#pragma unroll 1
+  for (int q = q0 + 1; q < q1; ++q) {
+    const int b = pop_lo<NW>(s, q);
+    comb_select_uniform<16>(t, tab + q * COMB32_ROW_WORDS, b);
+    ge_madd(r, h, t);
+    ge_p1p1_to_p3(h, r);
+  }
+  pin_p3(h);
+}
+
 // h += sum over rows q < nrows of d_q * row q of a radix-32 table, digits
 // from a low-first byte stream of NW words
 template <int NW>

@@ -19,9 +19,9 @@
 //           decoupled look-back over 8,192-point tiles
 //   encl    lanes per ciphertext of an encryption or rerandomisation: 1,
 //           2 (C1 on one lane, C2 on the next; twice the waves of a small
-//           batch, each lane with one comb pass) or 4 (each comb's odd and
-//           even digits on two lanes, joined by four doublings and an add:
-//           twice the waves again, 5 % more work).  The nine limbs of a field
+//           batch, each lane with one comb pass) or 4 (each radix-32
+//           comb's rows split over two waves, joined by one add through
+//           LDS: twice the waves again, 2 % more work).  The nine limbs of a field
 //           element always stay in one lane: splitting them would exchange
 //           partial column sums between lanes inside every multiply.
 // Every schedule gives the same group element (point addition is
@@ -57,7 +57,7 @@ namespace {
 // windows for the variable base below 2^18, two lanes per ciphertext of an
 // encryption), so an untuned process is close to a tuned one.
 constexpr AggTune DEFAULT_TUNE[TUNE_CLASSES] = {
-    {4, 1, 4, 2048, 3, 4, 2, 0}, {4, 1, 4, 512, 3, 4, 2, 0}, {4, 1, 4, 512, 3, 3, 2, 0}, {6, 2, 4, 2048, 3, 3, 2, 0}};
+    {4, 1, 4, 2048, 3, 4, 4, 0}, {4, 1, 4, 512, 3, 4, 4, 0}, {4, 1, 4, 512, 3, 3, 2, 0}, {6, 2, 4, 2048, 3, 3, 2, 0}};
 
 std::atomic<int> T_seg1[TUNE_CLASSES], T_segu[TUNE_CLASSES], T_scan[TUNE_CLASSES], T_reduce[TUNE_CLASSES],
     T_u32w[TUNE_CLASSES], T_vbw[TUNE_CLASSES], T_encl[TUNE_CLASSES], T_scanlb[TUNE_CLASSES];

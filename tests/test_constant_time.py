@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 SECRET_KERNELS = {
     "kernels.hip": ["k_fixed_base32", "k_encrypt_w32", "k_mul_u32_r8", "k_mul_u32_r16", "k_encrypt",
-                    "k_encrypt_wide", "k_encrypt_batch32"],
+                    "k_encrypt_wide", "k_encrypt_batch32", "k_encrypt_w32q"],
     "varbase.hip": ["k_varbase", "k_varbase_r8"],
     "serve.hip": ["kb_encrypt", "kb_encrypt_w32", "kb_encrypt_wide", "kb_mul_u32_r8",
                   "kb_mul_u32_r16"],

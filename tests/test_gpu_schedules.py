@@ -297,9 +297,10 @@ def test_autotune_keeps_results(tmp_path):
 @pytest.mark.parametrize("n", [1000, 300000])
 def test_every_ct_lanes(n):
     """Encryption and rerandomisation with one lane per ciphertext, with two
-    (C1 and C2 on separate lanes) and with four (each comb's odd and even
-    digits apart) against the oracle; at 300,000 the one-lane form is the
-    512-lane-block kernel"""
+    (C1 and C2 on separate waves) and with four (each comb's rows split
+    over two waves, the C2 split moving with the message's digit count:
+    8-bit and full 64-bit messages) against the oracle; at 300,000 the
+    one-lane form is the 512-lane-block kernel"""
     c = ft.size_class(n)
     before = ft.ct_lanes_table()[c]
     x = ft.random_scalars(1, 4800, 0)
@@ -321,5 +322,12 @@ def test_every_ct_lanes(n):
             ft.rerandomize(g1, g2, key, rho)
             same(g1, q1)
             same(g2, q2)
+        m64 = rng.integers(0, 2**64, size=n, dtype=np.uint64)
+        w1, w2 = O.encrypt(H, m64, r)
+        for lanes in (2, 4):
+            ft.tune_set_ct_lanes(c, lanes)
+            g1, g2 = ft.encrypt(key, m64, r)
+            same(g1, w1)
+            same(g2, w2)
     finally:
         ft.tune_set_ct_lanes(c, before)

@@ -77,11 +77,12 @@ def test_ct_lanes_set_get_save_load(tmp_path):
             with pytest.raises(ft.FtError) as e:
                 ft.tune_set_ct_lanes(1, bad)
             assert e.value.code == ft.FT_ERR_BAD_SIZE
-        # a file without the lanes column (the older format) loads with the defaults
+        # a file without the lanes column (the older format) loads with the
+        # defaults: four waves per 64 ciphertexts below 2^18, two above
         q = tmp_path / "old.txt"
         q.write_text("".join("%d 4 1 4 512 3 4\n" % c for c in range(4)))
         ft.tune_load(str(q))
-        assert ft.ct_lanes_table() == {0: 2, 1: 2, 2: 2, 3: 2}
+        assert ft.ct_lanes_table() == {0: 4, 1: 4, 2: 2, 3: 2}
     finally:
         for c, v in before.items():
             ft.tune_set_ct_lanes(c, v)
